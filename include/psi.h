@@ -1,0 +1,105 @@
+/*
+ * psi.h - C ABI of the MI355X (gfx950) strategy-improvement parity-game
+ * solver.  Plain C types only: int32 vertex ids, host or device pointers as
+ * stated per function, int status codes (0 = ok, < 0 = error, message in
+ * psi_last_error()).  No function prints or exits.
+ *
+ * The reference exposes this path only as C++ classes (no C ABI).  Each entry
+ * point below names the reference interface it replaces, file:line in
+ * jfearnley/parallel-si.
+ *
+ * Threading: a handle is used by one host thread at a time; distinct handles
+ * may be used concurrently.  The standalone list-ranking entry points share
+ * one library workspace and must not be called concurrently.
+ */
+#ifndef PSI_H
+#define PSI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PSI_OK 0
+#define PSI_ERR_INVALID -1   /* bad argument or malformed input               */
+#define PSI_ERR_HIP -2       /* HIP runtime error (message has the details)   */
+#define PSI_ERR_NO_DEVICE -3 /* no GPU visible                                */
+#define PSI_ERR_CAP -4       /* round cap reached before convergence          */
+#define PSI_ERR_BUDGET -5    /* device-memory budget exceeded (admission)     */
+
+/* Last error message of the calling thread (replaces gpu_assert's
+ * print-and-exit, gpu_util.cpp:10-17). */
+const char* psi_last_error(void);
+
+/* ---- runtime (gpu_util.h:8-16 roles) ------------------------------------ */
+int psi_init(void);                       /* checks a device, creates the stream */
+int psi_device_count(int* count);
+int psi_synchronize(void);                /* waits for all library work          */
+int psi_malloc(void** ptr, size_t bytes); /* init_zero / init_from_vector roles,  */
+int psi_free(void* ptr);                  /* gpu_util.cpp:24-37                   */
+int psi_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int psi_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int psi_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int psi_event_create(void** ev);
+int psi_event_destroy(void* ev);
+int psi_event_record(void* ev);           /* on the library stream               */
+int psi_event_elapsed_ms(void* start, void* end, float* ms);
+
+/* ---- list ranking ------------------------------------------------------- */
+/* Replaces the list-ranking core of GPUList::compute_valuation
+ * (gpu_list_ranking.cu:322-369: d_pick_splitters :139, d_traverse_sublists
+ * :196, process_reduced :240, d_broadcast_val :287) and of CPUList
+ * (cpu_list_ranking.cpp:154-250) as a standalone entry with unit weights.
+ * succ[i] in [-1, n) (-1 ends a list); heads[] holds the first node of every
+ * list.  rank[i] = distance of i from its head, -1 if no head reaches i.  */
+#define PSI_LR_AUTO 0      /* WALK if mean list length n/n_heads is short, else HJ */
+#define PSI_LR_HJ 1        /* Helman-JaJa: hashed block splitters + head slots     */
+#define PSI_LR_WYLLIE 2    /* pointer jumping; heads = nodes without predecessor   */
+#define PSI_LR_WALK 3      /* head slots only (one walk per list, hop-capped)      */
+#define PSI_LR_CHECKED 0x100 /* flag: pre-fill rank with -1 so unreached nodes read -1 */
+
+/* Device pointers; asynchronous on the library stream. */
+int psi_list_rank(const int* d_succ, int n, const int* d_heads, int n_heads, int* d_rank,
+                  int method);
+/* Host pointers; synchronous; always CHECKED; validates ranges. */
+int psi_list_rank_host(const int* succ, int n, const int* heads, int n_heads, int* rank,
+                       int method);
+/* Device error bits of the last call (1: index out of range, 2: a walk did not
+ * end) and the number of overflow sublists the level-0 walk created. */
+int psi_list_rank_status(int* error_bits, int* overflow_used);
+
+/* Tuning knobs (the autotuner sets these). */
+#define PSI_LRT_LOG_D0 1        /* level-0 splitter density 2^-v                 */
+#define PSI_LRT_LOCAL_BITS 2    /* level-0 hop cap 2^v                           */
+#define PSI_LRT_LOG_DK 3        /* reduced-level splitter density 2^-v           */
+#define PSI_LRT_HOP_CAP_K 4     /* reduced-level hop cap                         */
+#define PSI_LRT_WALK_MEAN_MAX 5 /* AUTO threshold on mean list length            */
+int psi_list_rank_tune(int param, int value);
+
+/* ---- seeded generators (host only, no device calls) --------------------- */
+/* Random game: owner Bernoulli(1/2), priority U[0,pri_bound), out-degree
+ * U[deg_lo,deg_hi] distinct uniform successors; CSR with offsets[n+1]. */
+int psi_gen_random_game(uint64_t seed, int n, int pri_bound, int deg_lo, int deg_hi, int* owner,
+                        int* priority, int* offsets, int* targets, int64_t targets_cap,
+                        int64_t* m_out);
+/* Chain game: priority i, owner i%2, edge i->i+1 (last -> 0), plus one random
+ * edge with probability 1/10. */
+int psi_gen_chain_game(uint64_t seed, int n, int* owner, int* priority, int* offsets,
+                       int* targets, int64_t targets_cap, int64_t* m_out);
+/* count sizes log-uniform in [lo, hi], rounded. */
+int psi_gen_sizes(uint64_t seed, int count, double lo, double hi, int* out);
+int psi_gen_permutation(uint64_t seed, int n, int* perm);
+/* One list over a random permutation of 0..n-1. */
+int psi_gen_list(uint64_t seed, int n, int* succ, int* head);
+/* n_lists lists, n_nodes nodes, log-uniform raw lengths in [1,len_hi]. */
+int psi_gen_forest(uint64_t seed, int n_nodes, int n_lists, int len_hi, int* succ, int* heads,
+                   int* lengths_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSI_H */

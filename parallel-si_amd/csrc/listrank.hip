@@ -1,0 +1,624 @@
+// List ranking on gfx950: Helman-JaJa sublist walks and Wyllie pointer
+// jumping, chosen by mean list length.
+//
+// Reference roles: the splitter pick, sublist walk, reduced-list walk and
+// broadcast of gpu_list_ranking.cu:96-110, :153-193, :197-240 (host),
+// :244-277 and cpu_list_ranking.cpp:154-250; the tree-Wyllie pointer jumping of
+// gpu_parity.cu:119-146.  The design here differs:
+//   * splitters are one per aligned block of D node ids, at a hashed offset
+//     inside the block, so "is x a splitter" and "which sublist does it start"
+//     are pure arithmetic on x (no marker writes into succ, which stays
+//     read-only), and structured lists (Euler tours) get evenly mixed splitters;
+//   * every list head is a splitter of its own (a "head slot");
+//   * a walk longer than Hc hops is cut and continued as a new overflow
+//     sublist, which bounds the local rank so (sublist, local rank) packs into
+//     the 32-bit output word during the walk;
+//   * the reduced list is ranked on the device, recursively by the same walk
+//     with weights, and the top level (<= a few thousand nodes) by pointer
+//     jumping inside one workgroup's LDS: no host round trip;
+//   * Wyllie ranks along predecessors with the (pointer, prefix) pair packed in
+//     one 64-bit word per node (ping-pong buffers), with a device-side count of
+//     still-active nodes so rounds after convergence exit at once.
+// Output: rank[i] = number of nodes before i on its list (0 at the head), or -1
+// when no given head reaches i (cycles, or heads that were not passed).
+#include "common.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace psi {
+namespace lr {
+
+constexpr int kTopThreads = 1024;
+
+struct Tuning {
+    int log_d0 = 5;      // level-0 splitter density: one per 2^log_d0 ids
+    int local_bits = 12; // level-0 hop cap 2^local_bits (also the packed local field)
+    int log_dk = 5;      // reduced-level splitter density
+    int hop_cap_k = 1024;
+    int walk_mean_max = 64; // AUTO: mean list length at or below this -> head walks only
+    uint32_t salt = 0x2545F491u;
+};
+Tuning g_tune;
+
+__device__ __forceinline__ bool is_split(uint32_t x, int log_d, uint32_t salt)
+{
+    uint32_t mask = (1u << log_d) - 1u;
+    return (x & mask) == (fmix32((x >> log_d) ^ salt) & mask);
+}
+
+__device__ __forceinline__ int block_splitter(int t, int log_d, uint32_t salt)
+{
+    return (t << log_d) + (int)(fmix32((uint32_t)t ^ salt) & ((1u << log_d) - 1u));
+}
+
+template <class T> __global__ void fill_kernel(T* p, long long n, T v)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <class T> int fill(T* p, long long n, T v, hipStream_t s)
+{
+    if (n <= 0) return PSI_OK;
+    fill_kernel<T><<<nblocks(n, 256), 256, 0, s>>>(p, n, v);
+    PSI_LAUNCH_CHECK();
+    return PSI_OK;
+}
+
+// ctl[0]: overflow sublists allocated; ctl[1]: error bits (1 bad index, 2 overflow exhausted)
+
+// Level-0 walk with unit weights.  One lane per sublist start; writes
+// packed[x] = (sublist << LB) | position-in-sublist for every node it visits.
+__global__ void __launch_bounds__(256) walk0_unit(const int* __restrict__ succ, int N,
+                                                  const int* __restrict__ heads, int H, int B,
+                                                  int log_d, uint32_t salt, int LB,
+                                                  int* __restrict__ packed, int* __restrict__ rnext,
+                                                  int* __restrict__ rlen, int* __restrict__ ctl,
+                                                  int over_base, int over_cap)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B + H) return;
+    const int hc = 1 << LB;
+    int rid = t, cur;
+    if (t < B) {
+        cur = block_splitter(t, log_d, salt);
+        if (cur >= N) return;                      // slot stays unused (-2)
+    } else {
+        cur = heads[t - B];
+        if ((unsigned)cur >= (unsigned)N) {
+            atomicOr(&ctl[1], 1);
+            return;
+        }
+        if (B > 0 && is_split((uint32_t)cur, log_d, salt)) {
+            rnext[rid] = cur >> log_d;
+            rlen[rid] = 0;
+            return;
+        }
+    }
+    int local = 0;
+    for (;;) {
+        packed[cur] = (rid << LB) | local;
+        local++;
+        int nxt = succ[cur];
+        if (nxt < 0 || nxt >= N) {
+            if (nxt != -1) atomicOr(&ctl[1], 1);
+            rnext[rid] = -1;
+            rlen[rid] = local;
+            return;
+        }
+        if (B > 0 && is_split((uint32_t)nxt, log_d, salt)) {
+            rnext[rid] = nxt >> log_d;
+            rlen[rid] = local;
+            return;
+        }
+        if (local == hc) {
+            int o = atomicAdd(&ctl[0], 1);
+            if (o >= over_cap) {
+                atomicOr(&ctl[1], 2);
+                rnext[rid] = -1;
+                rlen[rid] = local;
+                return;
+            }
+            int nr = over_base + o;
+            rnext[rid] = nr;
+            rlen[rid] = local;
+            rid = nr;
+            local = 0;
+        }
+        cur = nxt;
+    }
+}
+
+// Weighted walk of a reduced level.  Heads are the slots [hb, hb+H).
+__global__ void __launch_bounds__(256) walk_w(const int* __restrict__ rnext_in,
+                                              const int* __restrict__ w_in, int R, int hb, int H,
+                                              int B, int log_d, uint32_t salt, int hc,
+                                              int* __restrict__ rid_out, int* __restrict__ loc_out,
+                                              int* __restrict__ rnext_out, int* __restrict__ w_out,
+                                              int* __restrict__ ctl, int over_base, int over_cap)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B + H) return;
+    int rid = t, cur;
+    if (t < B) {
+        cur = block_splitter(t, log_d, salt);
+        if (cur >= R) return;
+    } else {
+        cur = hb + (t - B);
+        if (B > 0 && is_split((uint32_t)cur, log_d, salt)) {
+            rnext_out[rid] = cur >> log_d;
+            w_out[rid] = 0;
+            return;
+        }
+    }
+    int acc = 0, hops = 0;
+    for (;;) {
+        rid_out[cur] = rid;
+        loc_out[cur] = acc;
+        acc += w_in[cur];
+        hops++;
+        int nxt = rnext_in[cur];
+        if (nxt < 0 || nxt >= R) {
+            rnext_out[rid] = -1;
+            w_out[rid] = acc;
+            return;
+        }
+        if (B > 0 && is_split((uint32_t)nxt, log_d, salt)) {
+            rnext_out[rid] = nxt >> log_d;
+            w_out[rid] = acc;
+            return;
+        }
+        if (hops == hc) {
+            int o = atomicAdd(&ctl[0], 1);
+            if (o >= over_cap) {
+                atomicOr(&ctl[1], 2);
+                rnext_out[rid] = -1;
+                w_out[rid] = acc;
+                return;
+            }
+            int nr = over_base + o;
+            rnext_out[rid] = nr;
+            w_out[rid] = acc;
+            rid = nr;
+            acc = 0;
+            hops = 0;
+        }
+        cur = nxt;
+    }
+}
+
+// Top level: exclusive weighted prefix from the head slot of each list, by
+// pointer jumping along predecessors inside one workgroup's LDS.
+// pred state: >= 0 predecessor still to absorb, -1 resolved, -3 unreachable.
+constexpr int kTopCap = 16384;
+__global__ void __launch_bounds__(kTopThreads) top_lds(int R, const int* __restrict__ rnext,
+                                                       const int* __restrict__ w, int hb, int H,
+                                                       int* __restrict__ rok, int* __restrict__ roff)
+{
+    __shared__ int P[kTopCap];
+    __shared__ int S[kTopCap];
+    constexpr int per = kTopCap / kTopThreads;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < R; i += kTopThreads) P[i] = -3;
+    __syncthreads();
+    for (int i = tid; i < R; i += kTopThreads) {
+        int nx = rnext[i];
+        if (nx >= 0 && nx < R) P[nx] = i;
+    }
+    __syncthreads();
+    for (int i = tid; i < R; i += kTopThreads) {
+        int p = P[i];
+        if (p >= 0) {
+            S[i] = w[p];
+        } else {
+            bool head = i >= hb && i < hb + H;
+            P[i] = head ? -1 : -3;
+            S[i] = 0;
+        }
+    }
+    __syncthreads();
+    int rounds = ceil_log2_dev(R) + 1;
+    for (int r = 0; r < rounds; r++) {
+        int np[per], ns[per];
+#pragma unroll
+        for (int k = 0; k < per; k++) {
+            int i = tid + k * kTopThreads;
+            np[k] = -1;
+            ns[k] = 0;
+            if (i < R) {
+                int p = P[i], s = S[i];
+                if (p >= 0) {
+                    int pp = P[p];
+                    s += S[p];
+                    p = pp;
+                }
+                np[k] = p;
+                ns[k] = s;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < per; k++) {
+            int i = tid + k * kTopThreads;
+            if (i < R) {
+                P[i] = np[k];
+                S[i] = ns[k];
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < R; i += kTopThreads) {
+        rok[i] = P[i] == -1 ? 1 : 0;
+        roff[i] = S[i];
+    }
+}
+
+// Wyllie pointer jumping along predecessors.  PS[i] = (pred-state, prefix).
+// hb < 0: every node without a predecessor is a head (level-0 Wyllie).
+__global__ void wy_pred(const int* __restrict__ nxt, int R, int* __restrict__ pred)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    int nx = nxt[i];
+    if (nx >= 0 && nx < R) pred[nx] = i;
+}
+
+template <bool UNIT>
+__global__ void wy_init(const int* __restrict__ pred, const int* __restrict__ w, int R, int hb,
+                        int H, int2* __restrict__ ps)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    int p = pred[i];
+    int2 v;
+    if (p >= 0) {
+        v = make_int2(p, UNIT ? 1 : w[p]);
+    } else {
+        bool head = hb < 0 ? true : (i >= hb && i < hb + H);
+        v = make_int2(head ? -1 : -3, 0);
+    }
+    ps[i] = v;
+}
+
+__global__ void wy_round(const int2* __restrict__ in, int2* __restrict__ out, int R,
+                         int* __restrict__ active, int r)
+{
+    if (r > 0 && active[r - 1] == 0) return;   // converged: nothing left to jump
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = false;
+    if (i < R) {
+        int2 v = in[i];
+        if (v.x >= 0) {
+            int2 u = in[v.x];
+            v.y += u.y;
+            v.x = u.x;
+            act = v.x >= 0;
+        }
+        out[i] = v;
+    }
+    unsigned long long m = __ballot(act);
+    if (m && (threadIdx.x & (kWave - 1)) == (unsigned)(__ffsll((long long)m) - 1))
+        atomicAdd(&active[r], 1);
+}
+
+// Copies the converged buffer forward when the last rounds exited early.
+__global__ void wy_finish(const int2* __restrict__ a, const int2* __restrict__ b, int R,
+                          const int* __restrict__ active, int rounds, int* __restrict__ rok,
+                          int* __restrict__ roff)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    // Round r writes buffer (r+1)&1.  Find the last round that ran.
+    int last = -1;
+    for (int r = 0; r < rounds; r++) {
+        if (r > 0 && active[r - 1] == 0) break;
+        last = r;
+    }
+    const int2* src = last < 0 ? a : (((last + 1) & 1) ? b : a);
+    int2 v = src[i];
+    rok[i] = v.x == -1 ? 1 : 0;
+    roff[i] = v.x == -1 ? v.y : -1;
+}
+
+__global__ void prop_kernel(int R, const int* __restrict__ rid, const int* __restrict__ loc,
+                            const int* __restrict__ rok_up, const int* __restrict__ roff_up,
+                            int* __restrict__ rok, int* __restrict__ roff)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    int id = rid[r];
+    if (id < 0 || !rok_up[id]) {
+        rok[r] = 0;
+        roff[r] = 0;
+    } else {
+        rok[r] = 1;
+        roff[r] = roff_up[id] + loc[r];
+    }
+}
+
+__global__ void final0(int N, int LB, int R1, int* __restrict__ rank, const int* __restrict__ rok1,
+                       const int* __restrict__ roff1)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= N) return;
+    int p = rank[x];
+    if (p < 0) return;                             // never visited: stays -1
+    int id = p >> LB;
+    int loc = p & ((1 << LB) - 1);
+    rank[x] = (id < R1 && rok1[id]) ? roff1[id] + loc : -1;
+}
+
+// ---------------------------------------------------------------------------
+// Host orchestration.
+
+struct Arena {
+    void* buf = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    template <class T> T* take(long long n)
+    {
+        size_t bytes = ((size_t)std::max(n, 1LL) * sizeof(T) + 255) & ~(size_t)255;
+        T* p = (T*)((char*)buf + used);
+        used += bytes;
+        return p;
+    }
+};
+
+struct Level {
+    int R = 0, hb = 0, H = 0;
+    int *rnext = nullptr, *w = nullptr;           // this level's nodes
+    int *rid = nullptr, *loc = nullptr;           // filled when this level is walked
+    int *rok = nullptr, *roff = nullptr;          // final offsets of this level's nodes
+};
+
+struct Plan {
+    int N, H, LB, B0, over0;
+    bool blocks0;
+    std::vector<Level> lv;                         // lv[0] is reduced level 1
+    std::vector<int> B, over, log_d;               // walk parameters of each reduced level
+    bool top_wyllie = false;
+};
+
+static size_t level_bytes(long long R) { return 6 * (((size_t)R * 4 + 255) & ~(size_t)255); }
+
+Arena g_arena;
+int* g_ctl = nullptr;    // 2 ints per level + 64 Wyllie round counters, device
+constexpr int kMaxLevels = 8;
+constexpr int kCtlInts = kMaxLevels * 2 + 64;
+
+static int ensure_ctl()
+{
+    if (g_ctl) return PSI_OK;
+    PSI_HIP(hipMalloc(&g_ctl, kCtlInts * sizeof(int)));
+    return PSI_OK;
+}
+
+// Ranks the reduced level lv[k] (offsets into lv[k].rok/roff), recursively.
+static int rank_level(Plan& pl, int k, hipStream_t s)
+{
+    Level& L = pl.lv[k];
+    const Tuning& tu = g_tune;
+    if (L.R <= kTopCap) {
+        top_lds<<<1, kTopThreads, 0, s>>>(L.R, L.rnext, L.w, L.hb, L.H, L.rok, L.roff);
+        PSI_LAUNCH_CHECK();
+        return PSI_OK;
+    }
+    bool heads_dominate = (long long)L.H * 4 >= L.R || k + 1 >= kMaxLevels;
+    if (heads_dominate) {
+        // Global Wyllie on this level.
+        int* pred = g_arena.take<int>(L.R);
+        int2* ps0 = g_arena.take<int2>(L.R);
+        int2* ps1 = g_arena.take<int2>(L.R);
+        int* active = g_ctl + kMaxLevels * 2;
+        int rounds = ceil_log2(L.R) + 1;
+        if (int rc = fill<int>(pred, L.R, -1, s)) return rc;
+        if (int rc = fill<int>(active, 64, 0, s)) return rc;
+        wy_pred<<<nblocks(L.R, 256), 256, 0, s>>>(L.rnext, L.R, pred);
+        wy_init<false><<<nblocks(L.R, 256), 256, 0, s>>>(pred, L.w, L.R, L.hb, L.H, ps0);
+        for (int r = 0; r < rounds; r++)
+            wy_round<<<nblocks(L.R, 256), 256, 0, s>>>((r & 1) ? ps1 : ps0, (r & 1) ? ps0 : ps1,
+                                                      L.R, active, r);
+        wy_finish<<<nblocks(L.R, 256), 256, 0, s>>>(ps0, ps1, L.R, active, rounds, L.rok, L.roff);
+        PSI_LAUNCH_CHECK();
+        return PSI_OK;
+    }
+    // Walk this level into a new reduced level.
+    int log_d = tu.log_dk;
+    int B = (int)((L.R + (1LL << log_d) - 1) >> log_d);
+    int hc = tu.hop_cap_k;
+    int over = L.R / hc + 1;
+    Level nx;
+    nx.R = B + L.H + over;
+    nx.hb = B;
+    nx.H = L.H;
+    nx.rnext = g_arena.take<int>(nx.R);
+    nx.w = g_arena.take<int>(nx.R);
+    nx.rid = g_arena.take<int>(nx.R);
+    nx.loc = g_arena.take<int>(nx.R);
+    nx.rok = g_arena.take<int>(nx.R);
+    nx.roff = g_arena.take<int>(nx.R);
+    int* ctl = g_ctl + 2 * (k + 1);
+    if (int rc = fill<int>(ctl, 2, 0, s)) return rc;
+    if (int rc = fill<int>(nx.rnext, nx.R, -2, s)) return rc;
+    if (int rc = fill<int>(nx.w, nx.R, 0, s)) return rc;
+    if (int rc = fill<int>(L.rid, L.R, -1, s)) return rc;
+    if (B + L.H > 0)
+    walk_w<<<nblocks(B + L.H, 256), 256, 0, s>>>(L.rnext, L.w, L.R, L.hb, L.H, B, log_d, tu.salt,
+                                                 hc, L.rid, L.loc, nx.rnext, nx.w, ctl, B + L.H,
+                                                 over);
+    PSI_LAUNCH_CHECK();
+    pl.lv.push_back(nx);
+    if (int rc = rank_level(pl, k + 1, s)) return rc;
+    Level& L2 = pl.lv[k];
+    Level& N2 = pl.lv[k + 1];
+    prop_kernel<<<nblocks(L2.R, 256), 256, 0, s>>>(L2.R, L2.rid, L2.loc, N2.rok, N2.roff, L2.rok,
+                                                   L2.roff);
+    PSI_LAUNCH_CHECK();
+    return PSI_OK;
+}
+
+static int wyllie0(const int* succ, int N, int* rank, hipStream_t s)
+{
+    size_t need = (((size_t)N * 4 + 255) & ~(size_t)255) * 2 + 2 * (((size_t)N * 8 + 255) & ~(size_t)255);
+    if (int rc = ensure(&g_arena.buf, &g_arena.cap, need)) return rc;
+    g_arena.used = 0;
+    int* pred = g_arena.take<int>(N);
+    int* rok = g_arena.take<int>(N);
+    int2* ps0 = g_arena.take<int2>(N);
+    int2* ps1 = g_arena.take<int2>(N);
+    int* active = g_ctl + kMaxLevels * 2;
+    int rounds = ceil_log2(N) + 1;
+    if (rounds > 64) rounds = 64;
+    if (int rc = fill<int>(pred, N, -1, s)) return rc;
+    if (int rc = fill<int>(active, 64, 0, s)) return rc;
+    wy_pred<<<nblocks(N, 256), 256, 0, s>>>(succ, N, pred);
+    wy_init<true><<<nblocks(N, 256), 256, 0, s>>>(pred, nullptr, N, -1, 0, ps0);
+    for (int r = 0; r < rounds; r++)
+        wy_round<<<nblocks(N, 256), 256, 0, s>>>((r & 1) ? ps1 : ps0, (r & 1) ? ps0 : ps1, N,
+                                                active, r);
+    wy_finish<<<nblocks(N, 256), 256, 0, s>>>(ps0, ps1, N, active, rounds, rok, rank);
+    PSI_LAUNCH_CHECK();
+    return PSI_OK;
+}
+
+static int hj0(const int* succ, int N, const int* heads, int H, int* rank, bool blocks, bool checked,
+               hipStream_t s)
+{
+    const Tuning& tu = g_tune;
+    Plan pl;
+    pl.N = N;
+    pl.H = H;
+    pl.blocks0 = blocks;
+    int log_d = tu.log_d0;
+    pl.B0 = blocks ? (int)((N + (1LL << log_d) - 1) >> log_d) : 0;
+    int LB = tu.local_bits;
+    long long R1;
+    for (;;) {
+        pl.over0 = (int)(N >> LB) + 1;
+        R1 = (long long)pl.B0 + H + pl.over0;
+        if (R1 < (1LL << (31 - LB))) break;
+        if (--LB < 4) return fail(PSI_ERR_INVALID, "list ranking: too many heads/sublists to pack");
+    }
+    pl.LB = LB;
+    // Worst-case arena: a geometric series of reduced levels.
+    size_t need = 0;
+    long long R = R1;
+    for (int k = 0; k < kMaxLevels && R > 0; k++) {
+        need += level_bytes(R) + 3 * (((size_t)R * 8 + 255) & ~(size_t)255) + 4096;
+        R = ((R + (1LL << tu.log_dk) - 1) >> tu.log_dk) + H + R / tu.hop_cap_k + 1;
+    }
+    if (int rc = ensure(&g_arena.buf, &g_arena.cap, need)) return rc;
+    g_arena.used = 0;
+    Level L1;
+    L1.R = (int)R1;
+    L1.hb = pl.B0;
+    L1.H = H;
+    L1.rnext = g_arena.take<int>(R1);
+    L1.w = g_arena.take<int>(R1);
+    L1.rid = g_arena.take<int>(R1);
+    L1.loc = g_arena.take<int>(R1);
+    L1.rok = g_arena.take<int>(R1);
+    L1.roff = g_arena.take<int>(R1);
+    pl.lv.push_back(L1);
+    int* ctl = g_ctl;
+    if (int rc = fill<int>(ctl, 2, 0, s)) return rc;
+    if (int rc = fill<int>(L1.rnext, R1, -2, s)) return rc;
+    if (int rc = fill<int>(L1.w, R1, 0, s)) return rc;
+    if (checked)
+        if (int rc = fill<int>(rank, N, -1, s)) return rc;
+    if (pl.B0 + H > 0) {
+        walk0_unit<<<nblocks((long long)pl.B0 + H, 256), 256, 0, s>>>(
+            succ, N, heads, H, pl.B0, log_d, tu.salt, LB, rank, L1.rnext, L1.w, ctl, pl.B0 + H, pl.over0);
+        PSI_LAUNCH_CHECK();
+    }
+    if (int rc = rank_level(pl, 0, s)) return rc;
+    final0<<<nblocks(N, 256), 256, 0, s>>>(N, LB, (int)R1, rank, pl.lv[0].rok, pl.lv[0].roff);
+    PSI_LAUNCH_CHECK();
+    return PSI_OK;
+}
+
+} // namespace lr
+} // namespace psi
+
+using namespace psi;
+
+extern "C" int psi_list_rank(const int* d_succ, int n, const int* d_heads, int n_heads, int* d_rank,
+                             int method)
+{
+    if (n < 0 || n_heads < 0) return fail(PSI_ERR_INVALID, "psi_list_rank: negative size");
+    if (n == 0) return PSI_OK;
+    if (int rc = lr::ensure_ctl()) return rc;
+    hipStream_t s = stream();
+    int flags = method & ~0xff;
+    method &= 0xff;
+    bool checked = (flags & PSI_LR_CHECKED) != 0;
+    if (int rc = lr::fill<int>(lr::g_ctl, lr::kMaxLevels * 2, 0, s)) return rc;
+    if (method == PSI_LR_AUTO) {
+        long long mean = n_heads ? (long long)n / n_heads : (long long)n;
+        method = mean <= lr::g_tune.walk_mean_max ? PSI_LR_WALK : PSI_LR_HJ;
+    }
+    switch (method) {
+    case PSI_LR_HJ: return lr::hj0(d_succ, n, d_heads, n_heads, d_rank, true, checked, s);
+    case PSI_LR_WALK: return lr::hj0(d_succ, n, d_heads, n_heads, d_rank, false, checked, s);
+    case PSI_LR_WYLLIE: return lr::wyllie0(d_succ, n, d_rank, s);
+    default: return fail(PSI_ERR_INVALID, "psi_list_rank: unknown method");
+    }
+}
+
+// Reads and clears the device error bits of the last list ranking.
+extern "C" int psi_list_rank_status(int* error_bits, int* overflow_used)
+{
+    if (int rc = lr::ensure_ctl()) return rc;
+    int h[2];
+    PSI_HIP(hipMemcpyAsync(h, lr::g_ctl, sizeof h, hipMemcpyDeviceToHost, stream()));
+    PSI_HIP(hipStreamSynchronize(stream()));
+    if (overflow_used) *overflow_used = h[0];
+    int bits = 0;
+    std::vector<int> all(lr::kMaxLevels * 2);
+    PSI_HIP(hipMemcpyAsync(all.data(), lr::g_ctl, all.size() * sizeof(int), hipMemcpyDeviceToHost, stream()));
+    PSI_HIP(hipStreamSynchronize(stream()));
+    for (int k = 0; k < lr::kMaxLevels; k++) bits |= all[2 * k + 1];
+    if (error_bits) *error_bits = bits;
+    return PSI_OK;
+}
+
+extern "C" int psi_list_rank_host(const int* succ, int n, const int* heads, int n_heads, int* rank,
+                                  int method)
+{
+    if (n <= 0) return n == 0 ? PSI_OK : fail(PSI_ERR_INVALID, "psi_list_rank_host: negative size");
+    hipStream_t s = stream();
+    int *ds = nullptr, *dh = nullptr, *dr = nullptr;
+    PSI_HIP(hipMalloc(&ds, (size_t)n * 4));
+    PSI_HIP(hipMalloc(&dh, (size_t)std::max(n_heads, 1) * 4));
+    PSI_HIP(hipMalloc(&dr, (size_t)n * 4));
+    PSI_HIP(hipMemcpyAsync(ds, succ, (size_t)n * 4, hipMemcpyHostToDevice, s));
+    if (n_heads) PSI_HIP(hipMemcpyAsync(dh, heads, (size_t)n_heads * 4, hipMemcpyHostToDevice, s));
+    int rc = psi_list_rank(ds, n, dh, n_heads, dr, method | PSI_LR_CHECKED);
+    if (rc == PSI_OK) {
+        PSI_HIP(hipMemcpyAsync(rank, dr, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+        PSI_HIP(hipStreamSynchronize(s));
+        int bits = 0;
+        rc = psi_list_rank_status(&bits, nullptr);
+        if (rc == PSI_OK && bits) rc = fail(PSI_ERR_INVALID, bits & 1 ? "list ranking: successor or head out of range"
+                                                                      : "list ranking: walk did not end (cycle reachable from a head?)");
+    }
+    (void)hipFree(ds);
+    (void)hipFree(dh);
+    (void)hipFree(dr);
+    return rc;
+}
+
+extern "C" int psi_list_rank_tune(int param, int value)
+{
+    auto& t = lr::g_tune;
+    switch (param) {
+    case PSI_LRT_LOG_D0: if (value < 1 || value > 16) break; t.log_d0 = value; return PSI_OK;
+    case PSI_LRT_LOCAL_BITS: if (value < 4 || value > 16) break; t.local_bits = value; return PSI_OK;
+    case PSI_LRT_LOG_DK: if (value < 1 || value > 16) break; t.log_dk = value; return PSI_OK;
+    case PSI_LRT_HOP_CAP_K: if (value < 16) break; t.hop_cap_k = value; return PSI_OK;
+    case PSI_LRT_WALK_MEAN_MAX: if (value < 1) break; t.walk_mean_max = value; return PSI_OK;
+    default: break;
+    }
+    return fail(PSI_ERR_INVALID, "psi_list_rank_tune: bad parameter or value");
+}

@@ -1,0 +1,241 @@
+"""ctypes binding of libpsi.so (the C ABI in include/psi.h).
+
+Host glue for tests and the benchmark: numpy arrays in, numpy arrays out.
+The library must be built (``make -C parallel-si_amd``); importing this module
+without it raises, so no caller can silently fall back to a CPU path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpsi.so")
+
+PSI_OK = 0
+PSI_ERR_INVALID = -1
+PSI_ERR_HIP = -2
+PSI_ERR_NO_DEVICE = -3
+PSI_ERR_CAP = -4
+PSI_ERR_BUDGET = -5
+
+LR_AUTO, LR_HJ, LR_WYLLIE, LR_WALK = 0, 1, 2, 3
+LR_CHECKED = 0x100
+LRT_LOG_D0, LRT_LOCAL_BITS, LRT_LOG_DK, LRT_HOP_CAP_K, LRT_WALK_MEAN_MAX = 1, 2, 3, 4, 5
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built (run make -C parallel-si_amd)")
+lib = C.CDLL(LIB_PATH)
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_sig("psi_last_error", C.c_char_p)
+_sig("psi_init", C.c_int)
+_sig("psi_device_count", C.c_int, C.POINTER(C.c_int))
+_sig("psi_synchronize", C.c_int)
+_sig("psi_malloc", C.c_int, C.POINTER(_vp), C.c_size_t)
+_sig("psi_free", C.c_int, _vp)
+_sig("psi_memcpy_h2d", C.c_int, _vp, _vp, C.c_size_t)
+_sig("psi_memcpy_d2h", C.c_int, _vp, _vp, C.c_size_t)
+_sig("psi_memcpy_d2d", C.c_int, _vp, _vp, C.c_size_t)
+_sig("psi_event_create", C.c_int, C.POINTER(_vp))
+_sig("psi_event_destroy", C.c_int, _vp)
+_sig("psi_event_record", C.c_int, _vp)
+_sig("psi_event_elapsed_ms", C.c_int, _vp, _vp, C.POINTER(C.c_float))
+_sig("psi_list_rank", C.c_int, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int)
+_sig("psi_list_rank_host", C.c_int, _i32p, C.c_int, _i32p, C.c_int, _i32p, C.c_int)
+_sig("psi_list_rank_status", C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int))
+_sig("psi_list_rank_tune", C.c_int, C.c_int, C.c_int)
+_sig("psi_gen_random_game", C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int,
+     _i32p, _i32p, _i32p, _i32p, C.c_int64, C.POINTER(C.c_int64))
+_sig("psi_gen_chain_game", C.c_int, C.c_uint64, C.c_int, _i32p, _i32p, _i32p, _i32p,
+     C.c_int64, C.POINTER(C.c_int64))
+_sig("psi_gen_sizes", C.c_int, C.c_uint64, C.c_int, C.c_double, C.c_double, _i32p)
+_sig("psi_gen_permutation", C.c_int, C.c_uint64, C.c_int, _i32p)
+_sig("psi_gen_list", C.c_int, C.c_uint64, C.c_int, _i32p, C.POINTER(C.c_int))
+_sig("psi_gen_forest", C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int, _i32p, _i32p, _vp)
+
+
+class PsiError(RuntimeError):
+    def __init__(self, rc, where):
+        msg = lib.psi_last_error().decode(errors="replace")
+        super().__init__(f"{where}: rc={rc}: {msg}")
+        self.rc = rc
+
+
+def check(rc, where):
+    if rc != PSI_OK:
+        raise PsiError(rc, where)
+    return rc
+
+
+def init():
+    check(lib.psi_init(), "psi_init")
+
+
+def synchronize():
+    check(lib.psi_synchronize(), "psi_synchronize")
+
+
+class Game:
+    """A game as int32 CSR arrays (the GPUGame layout, gpu_game.h:6-21)."""
+
+    def __init__(self, owner, priority, offsets, targets):
+        self.owner = np.ascontiguousarray(owner, dtype=np.int32)
+        self.priority = np.ascontiguousarray(priority, dtype=np.int32)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+        self.targets = np.ascontiguousarray(targets, dtype=np.int32)
+
+    @property
+    def n(self):
+        return int(self.owner.shape[0])
+
+    @property
+    def m(self):
+        return int(self.targets.shape[0])
+
+    @property
+    def max_pri(self):
+        return int(self.priority.max()) if self.n else 0
+
+    def to_gm(self):
+        """PGSolver text (the format game.cpp:51-107 parses)."""
+        lines = [f"parity {self.n - 1};"]
+        for v in range(self.n):
+            succ = ",".join(str(int(t)) for t in self.targets[self.offsets[v]:self.offsets[v + 1]])
+            lines.append(f"{v} {int(self.priority[v])} {int(self.owner[v])} {succ};")
+        return "\n".join(lines) + "\n"
+
+
+def gen_random_game(seed, n, pri_bound, deg_lo, deg_hi):
+    owner = np.empty(n, np.int32)
+    pri = np.empty(n, np.int32)
+    off = np.empty(n + 1, np.int32)
+    cap = n * min(deg_hi, n, 64)
+    tgt = np.empty(cap, np.int32)
+    m = C.c_int64()
+    check(lib.psi_gen_random_game(seed, n, pri_bound, deg_lo, deg_hi, owner, pri, off, tgt, cap,
+                                  C.byref(m)), "psi_gen_random_game")
+    return Game(owner, pri, off, tgt[: m.value].copy())
+
+
+def gen_chain_game(seed, n):
+    owner = np.empty(n, np.int32)
+    pri = np.empty(n, np.int32)
+    off = np.empty(n + 1, np.int32)
+    cap = 2 * n
+    tgt = np.empty(cap, np.int32)
+    m = C.c_int64()
+    check(lib.psi_gen_chain_game(seed, n, owner, pri, off, tgt, cap, C.byref(m)),
+          "psi_gen_chain_game")
+    return Game(owner, pri, off, tgt[: m.value].copy())
+
+
+def gen_sizes(seed, count, lo, hi):
+    out = np.empty(count, np.int32)
+    check(lib.psi_gen_sizes(seed, count, float(lo), float(hi), out), "psi_gen_sizes")
+    return out
+
+
+def gen_list(seed, n):
+    succ = np.empty(n, np.int32)
+    head = C.c_int()
+    check(lib.psi_gen_list(seed, n, succ, C.byref(head)), "psi_gen_list")
+    return succ, np.array([head.value], np.int32)
+
+
+def gen_forest(seed, n_nodes, n_lists, len_hi=4096):
+    succ = np.empty(n_nodes, np.int32)
+    heads = np.empty(n_lists, np.int32)
+    lens = np.empty(n_lists, np.int32)
+    check(lib.psi_gen_forest(seed, n_nodes, n_lists, len_hi, succ, heads,
+                             lens.ctypes.data_as(_vp)), "psi_gen_forest")
+    return succ, heads, lens
+
+
+class DeviceArray:
+    """A device allocation with numpy-typed copies in and out."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = _vp()
+        check(lib.psi_malloc(C.byref(self.ptr), self.nbytes), "psi_malloc")
+
+    @classmethod
+    def from_host(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        d = cls(max(arr.nbytes, 1))
+        if arr.nbytes:
+            check(lib.psi_memcpy_h2d(d.ptr, arr.ctypes.data_as(_vp), arr.nbytes), "h2d")
+        return d
+
+    def to_host(self, dtype, count):
+        out = np.empty(count, dtype)
+        if out.nbytes:
+            check(lib.psi_memcpy_d2h(out.ctypes.data_as(_vp), self.ptr, out.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib.psi_free(self.ptr)
+            self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Timer:
+    """HIP events on the library stream."""
+
+    def __init__(self):
+        self.a, self.b = _vp(), _vp()
+        check(lib.psi_event_create(C.byref(self.a)), "event")
+        check(lib.psi_event_create(C.byref(self.b)), "event")
+
+    def start(self):
+        check(lib.psi_event_record(self.a), "record")
+
+    def stop(self):
+        check(lib.psi_event_record(self.b), "record")
+
+    def ms(self):
+        f = C.c_float()
+        check(lib.psi_event_elapsed_ms(self.a, self.b, C.byref(f)), "elapsed")
+        return f.value
+
+
+def list_rank_host(succ, heads, method=LR_AUTO):
+    succ = np.ascontiguousarray(succ, np.int32)
+    heads = np.ascontiguousarray(heads, np.int32)
+    rank = np.empty_like(succ)
+    check(lib.psi_list_rank_host(succ, succ.shape[0], heads, heads.shape[0], rank, method),
+          "psi_list_rank_host")
+    return rank
+
+
+def list_rank_device(d_succ, n, d_heads, n_heads, d_rank, method=LR_AUTO):
+    check(lib.psi_list_rank(d_succ.ptr, n, d_heads.ptr, n_heads, d_rank.ptr, method),
+          "psi_list_rank")
+
+
+def list_rank_status():
+    bits, over = C.c_int(), C.c_int()
+    check(lib.psi_list_rank_status(C.byref(bits), C.byref(over)), "status")
+    return bits.value, over.value
+
+
+def list_rank_tune(param, value):
+    check(lib.psi_list_rank_tune(param, value), "psi_list_rank_tune")

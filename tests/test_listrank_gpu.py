@@ -1,0 +1,130 @@
+"""List ranking on the GPU (Helman-JaJa, head walks, Wyllie) against the
+oracle's brute-force walk: bit-exact ranks on every shape, including the
+16,777,216-node list and the 1,048,576-list forest of the benchmark."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+METHODS = ["AUTO", "HJ", "WALK", "WYLLIE"]
+
+
+def _m(psi, name):
+    return getattr(psi, "LR_" + name)
+
+
+def _check(psi, succ, heads, method, expect=None):
+    succ = np.ascontiguousarray(succ, np.int32)
+    heads = np.ascontiguousarray(heads, np.int32)
+    got = psi.list_rank_host(succ, heads, _m(psi, method))
+    if expect is None:
+        expect = oracle.list_rank(succ, heads)
+    bad = np.nonzero(got != expect)[0]
+    assert bad.size == 0, (method, succ.size, heads.size, bad[:10], got[bad[:10]], expect[bad[:10]])
+
+
+def perm_list(rng, n):
+    perm = rng.permutation(n).astype(np.int32)
+    succ = np.full(n, -1, np.int32)
+    succ[perm[:-1]] = perm[1:]
+    return succ, perm[:1].copy()
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 1000, 4097, 100000, 300001])
+def test_random_list(gpu, method, n):
+    print(f"list n={n} method={method}", flush=True)
+    succ, heads = perm_list(np.random.default_rng(n), n)
+    _check(gpu, succ, heads, method)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_structured_lists(gpu, method):
+    n = 200000
+    ident = np.arange(1, n + 1, dtype=np.int32)
+    ident[-1] = -1
+    print(f"structured lists method={method}", flush=True)
+    _check(gpu, ident, np.array([0], np.int32), method)
+    rev = np.arange(-1, n - 1, dtype=np.int32)
+    _check(gpu, rev, np.array([n - 1], np.int32), method)
+    # Euler-tour-like order: all odd ids ascending, then all even ids descending
+    order = np.concatenate([np.arange(1, n, 2), np.arange(n - 2, -1, -2)]).astype(np.int32)
+    succ = np.full(n, -1, np.int32)
+    succ[order[:-1]] = order[1:]
+    _check(gpu, succ, order[:1], method)
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("n_lists,n_nodes", [(1, 10), (10, 10), (1000, 50000), (40000, 60000),
+                                             (7, 200000), (5000, 2000000)])
+def test_forests(gpu, method, n_lists, n_nodes):
+    print(f"forest lists={n_lists} nodes={n_nodes} method={method}", flush=True)
+    import psi
+    succ, heads, _ = psi.gen_forest(n_lists * 31 + n_nodes, n_nodes, n_lists, 4096)
+    _check(gpu, succ, heads, method)
+
+
+@pytest.mark.parametrize("method", ["HJ", "WALK", "WYLLIE"])
+def test_unreachable_nodes_read_minus_one(gpu, method):
+    # list 0->1->2, a 3-cycle 3->4->5->3 and a list 6->7 whose head is not passed
+    succ = np.array([1, 2, -1, 4, 5, 3, 7, -1], np.int32)
+    heads = np.array([0], np.int32)
+    got = gpu.list_rank_host(succ, heads, _m(gpu, method))
+    exp = [0, 1, 2, -1, -1, -1, -1, -1]
+    if method == "WYLLIE":       # Wyllie treats every node without predecessor as a head
+        exp = [0, 1, 2, -1, -1, -1, 0, 1]
+    assert got.tolist() == exp
+
+
+def test_long_list_head_walk_overflows_hop_cap(gpu):
+    # one list of 50,000 nodes ranked by head walks only: the walk is cut every
+    # 2^local_bits hops into overflow sublists that the reduced level links up
+    succ, heads = perm_list(np.random.default_rng(9), 50000)
+    _check(gpu, succ, heads, "WALK")
+    bits, over = gpu.list_rank_status()
+    assert bits == 0 and over >= 50000 // 4096
+
+
+@pytest.mark.parametrize("log_d0,local_bits", [(3, 8), (6, 12), (8, 10)])
+def test_tuning_knobs_keep_results(gpu, log_d0, local_bits):
+    succ, heads = perm_list(np.random.default_rng(5), 1 << 20)
+    expect = oracle.list_rank(succ, heads)
+    gpu.list_rank_tune(gpu.LRT_LOG_D0, log_d0)
+    gpu.list_rank_tune(gpu.LRT_LOCAL_BITS, local_bits)
+    try:
+        _check(gpu, succ, heads, "HJ", expect)
+    finally:
+        gpu.list_rank_tune(gpu.LRT_LOG_D0, 5)
+        gpu.list_rank_tune(gpu.LRT_LOCAL_BITS, 12)
+
+
+def test_full_size_list_16m(gpu):
+    """Configuration 1(i): one list of 16,777,216 nodes (device API, unchecked)."""
+    import psi
+    n = 1 << 24
+    succ, heads = psi.gen_list(1, n)
+    expect = oracle.list_rank(succ, heads)
+    for method in ("HJ", "WYLLIE"):
+        print(f"16M list method={method}", flush=True)
+        ds = psi.DeviceArray.from_host(succ)
+        dh = psi.DeviceArray.from_host(heads)
+        dr = psi.DeviceArray(4 * n)
+        psi.list_rank_device(ds, n, dh, 1, dr, _m(psi, method))
+        psi.synchronize()
+        got = dr.to_host(np.int32, n)
+        assert np.array_equal(got, expect), method
+        # property: ranks are a permutation of 0..n-1 and follow succ
+        nz = succ >= 0
+        assert np.all(got[succ[nz]] == got[nz] + 1)
+
+
+def test_full_size_forest(gpu):
+    """Configuration 1(ii): 1,048,576 lists, 16,777,216 nodes."""
+    import psi
+    succ, heads, lens = psi.gen_forest(2, 1 << 24, 1 << 20, 4096)
+    expect = oracle.list_rank(succ, heads)
+    for method in ("AUTO", "HJ", "WYLLIE"):
+        print(f"forest 16M method={method}", flush=True)
+        _check(psi, succ, heads, method, expect)
