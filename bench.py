@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X strategy-improvement solver (one GPU).
+
+Headline: SI vertex-iterations/s on the 4,194,304-vertex random game
+(owner Bernoulli(1/2), priority U[0, 2^20), out-degree U[2, 8], seed 2):
+value = n * br / t_solve summed over the K timed solves, where br counts
+valuation rounds (gpumain.cpp:34) and t_solve is one full solve_si
+(init_strat to convergence) with the game already resident in HBM.
+
+One JSON line on stdout.  Besides the contract keys it carries
+  roofline     - the dominant kernel of the solve (pointer-jumping round),
+                 its algorithmic bytes per launch over its HIP-event time;
+  cpu_baseline - the oracle's sparse CPUParity restatement (1 core) on the
+                 first rounds of the same game (a bounded sample);
+  listrank     - 16,777,216-node list and 1,048,576-list forest rankings.
+Progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "parallel-si_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def si_bench(psi, n, steps, warmup, seed):
+    t0 = time.time()
+    g = psi.gen_random_game(seed, n, 1 << 20, 2, 8)
+    log(f"[bench] generated n={g.n} m={g.m} in {time.time() - t0:.1f}s")
+    s = psi.Solver(g)
+    for _ in range(warmup):
+        s.solve()
+    s.sync()
+    timer = psi.Timer()
+    brs = []
+    t_wall = time.perf_counter()
+    timer.start()
+    for _ in range(steps):
+        s.solve()
+        brs.append(s.stats()[1])
+    timer.stop()
+    s.sync()
+    wall = time.perf_counter() - t_wall
+    dev_ms = timer.ms()
+    it, br, capped = s.stats()
+    log(f"[bench] {steps} solves: iter={it} br={br} wall={wall:.3f}s device={dev_ms:.1f}ms")
+    prof = psi.solve_profiled(s)
+    log("[bench] per-kernel ms/launches: " + ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in prof.items()))
+    return g, s, it, br, wall, prof
+
+
+def listrank_bench(psi, reps=10):
+    out = {}
+    n = 1 << 24
+    succ, heads = psi.gen_list(1, n)
+    cases = [("list_16M", succ, heads, psi.LR_HJ)]
+    fs, fh, _ = psi.gen_forest(2, n, 1 << 20, 4096)
+    cases.append(("forest_1M_lists", fs, fh, psi.LR_AUTO))
+    for name, sc, hd, method in cases:
+        ds = psi.DeviceArray.from_host(sc)
+        dh = psi.DeviceArray.from_host(hd)
+        dr = psi.DeviceArray(4 * n)
+        for _ in range(3):
+            psi.list_rank_device(ds, n, dh, hd.size, dr, method)
+        psi.synchronize()
+        t = psi.Timer()
+        t.start()
+        for _ in range(reps):
+            psi.list_rank_device(ds, n, dh, hd.size, dr, method)
+        t.stop()
+        ms = t.ms() / reps
+        gbs = 8.0 * n / (ms * 1e-3) / 1e9
+        out[name] = {"nodes": n, "lists": int(hd.size), "us": round(ms * 1e3, 2),
+                     "nodes_per_s": round(n / (ms * 1e-3), 1), "alg_GBps": round(gbs, 1),
+                     "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"[bench] {name}: {ms * 1e3:.1f} us, {gbs:.0f} GB/s algorithmic")
+        for d in (ds, dh, dr):
+            d.free()
+    return out
+
+
+def cpu_baseline(g, max_rounds):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    r = oracle.solve(g, rep=oracle.SPARSE, max_rounds=max_rounds)
+    return {"value": round(g.n * r.br / r.seconds, 1), "unit": "vertex-iterations/s", "cores": 1,
+            "kind": "port",
+            "sample": f"first {r.br} valuation rounds of the same game (oracle sparse CPUParity "
+                      f"restatement, single thread, {r.seconds:.1f}s, {cpu_model()})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1 << 22)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-rounds", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-listrank", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              f"(independent games would shard by game across GPUs, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    import psi
+    psi.init()
+    g, s, it, br, wall, prof = si_bench(psi, a.n, a.steps, a.warmup, a.seed)
+    value = g.n * br * a.steps / wall
+    # Roofline of the dominant kernel: a pointer-jumping round reads its own
+    # 16-byte record, gathers the successor's record and writes the result.
+    jump_ms, jump_launches = prof["jump"]
+    jump_bytes = 48.0 * g.n
+    achieved = jump_bytes / (jump_ms / jump_launches * 1e-3) / 1e9 if jump_launches else 0.0
+    roof = {"bound": "hbm", "kernel": "k_jump", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "alg_bytes_per_launch": jump_bytes,
+            "avg_launch_us": round(jump_ms / max(jump_launches, 1) * 1e3, 2)}
+    line = {
+        "metric": "SI vertex-iterations/s on 4M-vertex game",
+        "value": round(value, 1),
+        "unit": "vertex-iterations/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded random parity game, exact integer solve)",
+        "config": {"workload": "plain SI solve, random game n=4194304, priority U[0,2^20), "
+                               "out-degree U[2,8], seed 2",
+                   "n": g.n, "m": g.m, "iter": it, "br": br, "parallelism": "single GPU"},
+        "roofline": roof,
+        "kernels_ms": {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in prof.items()},
+    }
+    s.close()
+    if not a.no_listrank:
+        line["listrank"] = listrank_bench(psi)
+    if not a.no_cpu:
+        log("[bench] cpu baseline ...")
+        line["cpu_baseline"] = cpu_baseline(g, a.cpu_rounds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

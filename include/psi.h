@@ -79,6 +79,85 @@ int psi_list_rank_status(int* error_bits, int* overflow_used);
 #define PSI_LRT_WALK_MEAN_MAX 5 /* AUTO threshold on mean list length            */
 int psi_list_rank_tune(int param, int value);
 
+/* ---- games --------------------------------------------------------------- */
+/* Replaces Game (game.h:9-34) as input and GPUGame (gpu_game.h:6-21,
+ * gpu_game.cpp:11-41) as the int32 CSR layout: offsets[n+1] (offsets[n] = m),
+ * targets[m] (file order = the switch tie-break order), owner[n] in {0,1},
+ * priority[n] in [0, 2^30).  Unlike the reference, successor ids are range
+ * checked and negative priorities are rejected (both are undefined behaviour
+ * there).  The arrays are copied. */
+typedef struct psi_game psi_game;
+int psi_game_create(int n, int m, const int* owner, const int* priority, const int* offsets,
+                    const int* targets, psi_game** out);
+int psi_game_destroy(psi_game* g);
+int psi_game_info(const psi_game* g, int* n, int* m, int* max_pri);
+
+/* ---- solvers --------------------------------------------------------------
+ * Replaces GPUParity(Game&, GPUGame&) (gpu_parity.h:8-38) and GPUList
+ * (gpu_list_ranking.h:11-41): allocates device state and uploads the CSR.
+ * Semantics are CPUParity's (cpu_parity.cpp), the reference's canonical CPU
+ * path, for every game: results are bit-exact with it.
+ * A batch solver holds n_games games back to back (per-game n, m; owner and
+ * priority concatenated; each game's n+1 offsets concatenated; targets in
+ * local ids concatenated) and solves each as if alone (per-game control). */
+#define PSI_SOLVER_TRACE 1   /* keep per-round hashes and valuation digests */
+typedef struct psi_solver psi_solver;
+int psi_solver_create(const psi_game* g, int flags, int trace_cap, psi_solver** out);
+int psi_solver_create_batch(int n_games, const int* n_per, const int64_t* m_per, const int* owner,
+                            const int* priority, const int* offsets, const int* targets, int flags,
+                            int trace_cap, psi_solver** out);
+int psi_solver_destroy(psi_solver* s);
+int psi_solver_info(psi_solver* s, int* n_games, int64_t* n_total, int64_t* m_total);
+int psi_solver_synchronize(psi_solver* s);
+
+/* StrategyImprovement (si.h:6-16), one call each, all games of the handle:
+ *   init_strat              gpu_parity.cu:91-96, cpu_parity.cpp:317-327
+ *   compute_valuation       gpu_parity.cu:191-227 (compute_first_val is the same)
+ *   mark_solved(player)     cpu_parity.cpp:306-315 (GPUParity's is a no-op)
+ *   switch_strategy(player) gpu_parity.cu:343-357: *switched = 1 if any vertex
+ *                           switched (the reference returns this 0/1 flag)
+ *   reset_opponent_strategy cpu_parity.cpp:341-348 */
+int psi_init_strat(psi_solver* s);
+int psi_compute_valuation(psi_solver* s);
+int psi_mark_solved(psi_solver* s, int player);
+int psi_switch_strategy(psi_solver* s, int player, int* switched);
+int psi_reset_opponent_strategy(psi_solver* s);
+
+/* The whole solve_si loop (gpumain.cpp:18-53) on the device, every game with
+ * its own convergence: reset = the cpu-reset mode (main.cpp:25-26);
+ * max_rounds > 0 caps valuation rounds per game (then psi_get_stats reports
+ * capped); steps_per_sync SI steps run per host check (a captured hipGraph). */
+int psi_solve(psi_solver* s, int reset, int max_rounds, int steps_per_sync);
+int psi_get_stats(psi_solver* s, int game, int* iter, int* br, int* capped);
+/* The same solve launched kernel by kernel with HIP events between launches
+ * (no graph): device milliseconds and launch counts per kernel class. */
+#define PSI_K_VAL_INIT 0
+#define PSI_K_JUMP 1      /* pointer-jumping rounds that ran               */
+#define PSI_K_JUMP_IDLE 2 /* rounds launched after convergence (no-ops)    */
+#define PSI_K_LABEL 3
+#define PSI_K_SWITCH1 4
+#define PSI_K_SWITCH0 5
+#define PSI_K_CTL 6
+#define PSI_K_TOTAL 7
+#define PSI_K_COUNT 8
+int psi_solve_profiled(psi_solver* s, int reset, int max_rounds, double* times_ms,
+                       int64_t* counts);
+
+/* Results of one game, local vertex ids.  get_winning: gpu_parity.cu:380-394. */
+int psi_get_winning(psi_solver* s, int game, int player, int* out, int* len);
+int psi_get_strategy(psi_solver* s, int game, int* out);   /* -1 = sink */
+int psi_get_labels(psi_solver* s, int game, int* out);     /* infinite[]: -1, 0, +1 */
+int psi_get_solved(psi_solver* s, int game, int* out);
+int psi_get_depths(psi_solver* s, int game, int* out);     /* path length to sink */
+/* Trace mode only.  Digest of a finite valuation: sum_p count[p]*splitmix64(p)
+ * mod 2^64 (0 for infinite vertices).  Trace: per valuation round, hashes of
+ * the strategy valued, solved flags, labels and digests (4 x uint64). */
+int psi_get_digests(psi_solver* s, int game, uint64_t* out);
+int psi_get_trace(psi_solver* s, int game, uint64_t* hashes, int cap, int* count);
+/* Loads strategy (-1 sink), solved flags and labels of solved vertices. */
+int psi_set_state(psi_solver* s, int game, const int* strat, const int* solved,
+                  const int* labels);
+
 /* ---- seeded generators (host only, no device calls) --------------------- */
 /* Random game: owner Bernoulli(1/2), priority U[0,pri_bound), out-degree
  * U[deg_lo,deg_hi] distinct uniform successors; CSR with offsets[n+1]. */

@@ -281,10 +281,29 @@ __global__ void wy_init(const int* __restrict__ pred, const int* __restrict__ w,
     ps[i] = v;
 }
 
-__global__ void wy_round(const int2* __restrict__ in, int2* __restrict__ out, int R,
-                         int* __restrict__ active, int r)
+// active: kWySlots counters per round (one atomic per block into slot
+// blockIdx % kWySlots), then active[64 * kWySlots] = last round that ran.
+constexpr int kWySlots = 64;
+constexpr int kWyRounds = 64;
+
+__global__ void __launch_bounds__(256) wy_round(const int2* __restrict__ in, int2* __restrict__ out,
+                                                int R, int* __restrict__ active, int r)
 {
-    if (r > 0 && active[r - 1] == 0) return;   // converged: nothing left to jump
+    __shared__ int run_s;
+    __shared__ unsigned int cnt;
+    if (threadIdx.x == 0) {
+        int run = 1;
+        if (r > 0) {
+            int a = 0;
+            for (int k = 0; k < kWySlots; k++) a |= active[(r - 1) * kWySlots + k];
+            run = a != 0;                         // converged: nothing left to jump
+        }
+        run_s = run;
+        cnt = 0;
+        if (run && blockIdx.x == 0) active[kWyRounds * kWySlots] = r;
+    }
+    __syncthreads();
+    if (!run_s) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool act = false;
     if (i < R) {
@@ -298,24 +317,20 @@ __global__ void wy_round(const int2* __restrict__ in, int2* __restrict__ out, in
         out[i] = v;
     }
     unsigned long long m = __ballot(act);
-    if (m && (threadIdx.x & (kWave - 1)) == (unsigned)(__ffsll((long long)m) - 1))
-        atomicAdd(&active[r], 1);
+    if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(&cnt, 1u);
+    __syncthreads();
+    if (threadIdx.x == 0 && cnt) atomicAdd(&active[r * kWySlots + (blockIdx.x & (kWySlots - 1))], 1);
 }
 
-// Copies the converged buffer forward when the last rounds exited early.
+// Reads the result from the buffer the last round that ran wrote.
 __global__ void wy_finish(const int2* __restrict__ a, const int2* __restrict__ b, int R,
-                          const int* __restrict__ active, int rounds, int* __restrict__ rok,
+                          const int* __restrict__ active, int* __restrict__ rok,
                           int* __restrict__ roff)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= R) return;
-    // Round r writes buffer (r+1)&1.  Find the last round that ran.
-    int last = -1;
-    for (int r = 0; r < rounds; r++) {
-        if (r > 0 && active[r - 1] == 0) break;
-        last = r;
-    }
-    const int2* src = last < 0 ? a : (((last + 1) & 1) ? b : a);
+    int last = active[kWyRounds * kWySlots];      // round r writes buffer (r+1)&1
+    const int2* src = ((last + 1) & 1) ? b : a;
     int2 v = src[i];
     rok[i] = v.x == -1 ? 1 : 0;
     roff[i] = v.x == -1 ? v.y : -1;
@@ -385,7 +400,7 @@ static size_t level_bytes(long long R) { return 6 * (((size_t)R * 4 + 255) & ~(s
 Arena g_arena;
 int* g_ctl = nullptr;    // 2 ints per level + 64 Wyllie round counters, device
 constexpr int kMaxLevels = 8;
-constexpr int kCtlInts = kMaxLevels * 2 + 64;
+constexpr int kCtlInts = kMaxLevels * 2 + 64 * 64 + 1;
 
 static int ensure_ctl()
 {
@@ -413,13 +428,13 @@ static int rank_level(Plan& pl, int k, hipStream_t s)
         int* active = g_ctl + kMaxLevels * 2;
         int rounds = ceil_log2(L.R) + 1;
         if (int rc = fill<int>(pred, L.R, -1, s)) return rc;
-        if (int rc = fill<int>(active, 64, 0, s)) return rc;
+        if (int rc = fill<int>(active, kWyRounds * kWySlots + 1, 0, s)) return rc;
         wy_pred<<<nblocks(L.R, 256), 256, 0, s>>>(L.rnext, L.R, pred);
         wy_init<false><<<nblocks(L.R, 256), 256, 0, s>>>(pred, L.w, L.R, L.hb, L.H, ps0);
         for (int r = 0; r < rounds; r++)
             wy_round<<<nblocks(L.R, 256), 256, 0, s>>>((r & 1) ? ps1 : ps0, (r & 1) ? ps0 : ps1,
                                                       L.R, active, r);
-        wy_finish<<<nblocks(L.R, 256), 256, 0, s>>>(ps0, ps1, L.R, active, rounds, L.rok, L.roff);
+        wy_finish<<<nblocks(L.R, 256), 256, 0, s>>>(ps0, ps1, L.R, active, L.rok, L.roff);
         PSI_LAUNCH_CHECK();
         return PSI_OK;
     }
@@ -471,13 +486,13 @@ static int wyllie0(const int* succ, int N, int* rank, hipStream_t s)
     int rounds = ceil_log2(N) + 1;
     if (rounds > 64) rounds = 64;
     if (int rc = fill<int>(pred, N, -1, s)) return rc;
-    if (int rc = fill<int>(active, 64, 0, s)) return rc;
+    if (int rc = fill<int>(active, kWyRounds * kWySlots + 1, 0, s)) return rc;
     wy_pred<<<nblocks(N, 256), 256, 0, s>>>(succ, N, pred);
     wy_init<true><<<nblocks(N, 256), 256, 0, s>>>(pred, nullptr, N, -1, 0, ps0);
     for (int r = 0; r < rounds; r++)
         wy_round<<<nblocks(N, 256), 256, 0, s>>>((r & 1) ? ps1 : ps0, (r & 1) ? ps0 : ps1, N,
                                                 active, r);
-    wy_finish<<<nblocks(N, 256), 256, 0, s>>>(ps0, ps1, N, active, rounds, rok, rank);
+    wy_finish<<<nblocks(N, 256), 256, 0, s>>>(ps0, ps1, N, active, rok, rank);
     PSI_LAUNCH_CHECK();
     return PSI_OK;
 }

@@ -1,0 +1,295 @@
+// C ABI of the solver: games, single-game and batched solvers, the
+// step-wise StrategyImprovement operations (si.h:6-16) and the
+// device-resident solve (solve_si, gpumain.cpp:18-53).
+#include "si_core.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+using namespace psi;
+
+struct psi_game {
+    int n = 0, m = 0, max_pri = 0;
+    std::vector<int> owner, priority, offsets, targets;
+};
+
+struct psi_solver {
+    Core core;
+    bool trace = false;
+};
+
+extern "C" int psi_game_create(int n, int m, const int* owner, const int* priority,
+                               const int* offsets, const int* targets, psi_game** out)
+{
+    *out = nullptr;
+    if (n <= 0 || m <= 0) return fail(PSI_ERR_INVALID, "psi_game_create: empty game");
+    if (offsets[0] != 0 || offsets[n] != m)
+        return fail(PSI_ERR_INVALID, "psi_game_create: offsets must run from 0 to m");
+    auto* g = new psi_game;
+    g->n = n;
+    g->m = m;
+    g->owner.assign(owner, owner + n);
+    g->priority.assign(priority, priority + n);
+    g->offsets.assign(offsets, offsets + n + 1);
+    g->targets.assign(targets, targets + m);
+    for (int v = 0; v < n; v++) {
+        if (offsets[v + 1] <= offsets[v]) { delete g; return fail(PSI_ERR_INVALID, "sinks are not allowed!"); }
+        if (owner[v] != 0 && owner[v] != 1) { delete g; return fail(PSI_ERR_INVALID, "invalid player"); }
+        if (priority[v] < 0 || priority[v] > 0x3FFFFFFF) {
+            delete g;
+            return fail(PSI_ERR_INVALID, "priority out of range [0, 2^30)");
+        }
+        g->max_pri = std::max(g->max_pri, priority[v]);
+    }
+    for (int e = 0; e < m; e++)
+        if (targets[e] < 0 || targets[e] >= n) { delete g; return fail(PSI_ERR_INVALID, "successor out of range"); }
+    *out = g;
+    return PSI_OK;
+}
+
+extern "C" int psi_game_destroy(psi_game* g)
+{
+    delete g;
+    return PSI_OK;
+}
+
+extern "C" int psi_game_info(const psi_game* g, int* n, int* m, int* max_pri)
+{
+    if (n) *n = g->n;
+    if (m) *m = g->m;
+    if (max_pri) *max_pri = g->max_pri;
+    return PSI_OK;
+}
+
+extern "C" int psi_solver_create_batch(int n_games, const int* n_per, const int64_t* m_per,
+                                       const int* owner, const int* priority, const int* offsets,
+                                       const int* targets, int flags, int trace_cap,
+                                       psi_solver** out)
+{
+    *out = nullptr;
+    if (int rc = psi_init()) return rc;
+    auto* s = new psi_solver;
+    s->trace = (flags & PSI_SOLVER_TRACE) != 0;
+    std::vector<long long> m(m_per, m_per + n_games);
+    int rc = core_create(s->core, n_games, n_per, m.data(), owner, priority, offsets, targets,
+                         s->trace, trace_cap, stream());
+    if (rc) {
+        core_destroy(s->core);
+        delete s;
+        return rc;
+    }
+    if ((rc = core_init_strat(s->core))) {
+        core_destroy(s->core);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return PSI_OK;
+}
+
+extern "C" int psi_solver_create(const psi_game* g, int flags, int trace_cap, psi_solver** out)
+{
+    int64_t m = g->m;
+    return psi_solver_create_batch(1, &g->n, &m, g->owner.data(), g->priority.data(),
+                                   g->offsets.data(), g->targets.data(), flags, trace_cap, out);
+}
+
+extern "C" int psi_solver_destroy(psi_solver* s)
+{
+    if (!s) return PSI_OK;
+    (void)hipStreamSynchronize(s->core.s);
+    core_destroy(s->core);
+    delete s;
+    return PSI_OK;
+}
+
+extern "C" int psi_solver_info(psi_solver* s, int* n_games, int64_t* n_total, int64_t* m_total)
+{
+    if (n_games) *n_games = s->core.G;
+    if (n_total) *n_total = s->core.N;
+    if (m_total) *m_total = s->core.M;
+    return PSI_OK;
+}
+
+extern "C" int psi_init_strat(psi_solver* s) { return core_init_strat(s->core); }
+
+extern "C" int psi_compute_valuation(psi_solver* s) { return core_valuation(s->core, false); }
+
+extern "C" int psi_mark_solved(psi_solver* s, int player)
+{
+    if (player != 0 && player != 1) return fail(PSI_ERR_INVALID, "player must be 0 or 1");
+    return core_mark_solved(s->core, player);
+}
+
+extern "C" int psi_switch_strategy(psi_solver* s, int player, int* switched)
+{
+    if (player != 0 && player != 1) return fail(PSI_ERR_INVALID, "player must be 0 or 1");
+    return core_switch(s->core, player, switched);
+}
+
+extern "C" int psi_reset_opponent_strategy(psi_solver* s) { return core_reset_opp(s->core); }
+
+extern "C" int psi_solve(psi_solver* s, int reset, int max_rounds, int steps_per_sync)
+{
+    Core& c = s->core;
+    if (int rc = core_init_strat(c)) return rc;
+    if (int rc = core_set_params(c, reset, max_rounds)) return rc;
+    if (steps_per_sync <= 0) steps_per_sync = 4;
+    return core_solve(c, steps_per_sync);
+}
+
+extern "C" int psi_solve_profiled(psi_solver* s, int reset, int max_rounds, double* times_ms,
+                                  int64_t* counts)
+{
+    Core& c = s->core;
+    if (int rc = core_init_strat(c)) return rc;
+    if (int rc = core_set_params(c, reset, max_rounds)) return rc;
+    std::vector<long long> cn(PSI_K_COUNT);
+    int rc = core_solve_profiled(c, times_ms, cn.data());
+    for (int k = 0; k < PSI_K_COUNT; k++) counts[k] = cn[k];
+    return rc;
+}
+
+static int check_game(psi_solver* s, int game)
+{
+    if (game < 0 || game >= s->core.G) return fail(PSI_ERR_INVALID, "game index out of range");
+    return PSI_OK;
+}
+
+template <class T> static int copy_game(psi_solver* s, int game, const T* dev, T* out)
+{
+    Core& c = s->core;
+    int vb = c.h_vbase[game], n = c.h_n[game];
+    PSI_HIP(hipMemcpyAsync(out, dev + vb, (size_t)n * sizeof(T), hipMemcpyDeviceToHost, c.s));
+    PSI_HIP(hipStreamSynchronize(c.s));
+    return PSI_OK;
+}
+
+extern "C" int psi_get_stats(psi_solver* s, int game, int* iter, int* br, int* capped)
+{
+    if (int rc = check_game(s, game)) return rc;
+    Core& c = s->core;
+    GameCtl gc;
+    PSI_HIP(hipMemcpyAsync(&gc, c.gctl + game, sizeof gc, hipMemcpyDeviceToHost, c.s));
+    PSI_HIP(hipStreamSynchronize(c.s));
+    if (iter) *iter = gc.iter;
+    if (br) *br = gc.br;
+    if (capped) *capped = gc.capped;
+    return PSI_OK;
+}
+
+// Strategy in the game's local vertex ids (-1 = sink).
+extern "C" int psi_get_strategy(psi_solver* s, int game, int* out)
+{
+    if (int rc = check_game(s, game)) return rc;
+    if (int rc = copy_game(s, game, s->core.strat, out)) return rc;
+    int vb = s->core.h_vbase[game], n = s->core.h_n[game];
+    for (int v = 0; v < n; v++)
+        if (out[v] >= 0) out[v] -= vb;
+    return PSI_OK;
+}
+
+// infinite[] labels: +1, 0, -1 (cpu_parity.h:15).
+extern "C" int psi_get_labels(psi_solver* s, int game, int* out)
+{
+    if (int rc = check_game(s, game)) return rc;
+    int n = s->core.h_n[game];
+    std::vector<int8_t> t(n);
+    if (int rc = copy_game(s, game, s->core.lab, t.data())) return rc;
+    for (int v = 0; v < n; v++) out[v] = t[v];
+    return PSI_OK;
+}
+
+extern "C" int psi_get_solved(psi_solver* s, int game, int* out)
+{
+    if (int rc = check_game(s, game)) return rc;
+    int n = s->core.h_n[game];
+    std::vector<int8_t> t(n);
+    if (int rc = copy_game(s, game, s->core.solved, t.data())) return rc;
+    for (int v = 0; v < n; v++) out[v] = t[v];
+    return PSI_OK;
+}
+
+// Path length to the sink of finite vertices after the last valuation (0 otherwise).
+extern "C" int psi_get_depths(psi_solver* s, int game, int* out)
+{
+    if (int rc = check_game(s, game)) return rc;
+    if (int rc = copy_game(s, game, s->core.dep, out)) return rc;
+    int n = s->core.h_n[game];
+    std::vector<int8_t> t(n);
+    if (int rc = copy_game(s, game, s->core.lab, t.data())) return rc;
+    for (int v = 0; v < n; v++)
+        if (t[v] != 0) out[v] = 0;
+    return PSI_OK;
+}
+
+// Valuation digests sum_p count_v[p] * splitmix64(p) of finite vertices (trace mode).
+extern "C" int psi_get_digests(psi_solver* s, int game, uint64_t* out)
+{
+    if (int rc = check_game(s, game)) return rc;
+    if (!s->trace) return fail(PSI_ERR_INVALID, "digests need a solver created with PSI_SOLVER_TRACE");
+    return copy_game(s, game, (const uint64_t*)s->core.dig, out);
+}
+
+// Per-round hashes (strategy valued, solved, labels, digests), 4 per round.
+extern "C" int psi_get_trace(psi_solver* s, int game, uint64_t* hashes, int cap, int* count)
+{
+    if (int rc = check_game(s, game)) return rc;
+    Core& c = s->core;
+    if (!s->trace) return fail(PSI_ERR_INVALID, "trace needs a solver created with PSI_SOLVER_TRACE");
+    GameCtl gc;
+    PSI_HIP(hipMemcpyAsync(&gc, c.gctl + game, sizeof gc, hipMemcpyDeviceToHost, c.s));
+    PSI_HIP(hipStreamSynchronize(c.s));
+    int k = std::min(std::min(gc.br, c.trace_cap), cap);
+    if (k > 0) {
+        PSI_HIP(hipMemcpyAsync(hashes, c.tr + (size_t)game * c.trace_cap * 4, (size_t)k * 4 * 8,
+                               hipMemcpyDeviceToHost, c.s));
+        PSI_HIP(hipStreamSynchronize(c.s));
+    }
+    *count = k;
+    return PSI_OK;
+}
+
+// get_winning(out, player) (cpu_parity.cpp:329-338): player 0 gets the +inf
+// vertices, player 1 every other vertex, ascending local ids.
+extern "C" int psi_get_winning(psi_solver* s, int game, int player, int* out, int* len)
+{
+    if (int rc = check_game(s, game)) return rc;
+    int n = s->core.h_n[game];
+    std::vector<int> lab(n);
+    if (int rc = psi_get_labels(s, game, lab.data())) return rc;
+    int k = 0;
+    for (int v = 0; v < n; v++)
+        if ((player == 0) == (lab[v] == 1)) out[k++] = v;
+    *len = k;
+    return PSI_OK;
+}
+
+// Loads a state (local ids) into game `game`: strategy, solved flags, labels.
+extern "C" int psi_set_state(psi_solver* s, int game, const int* strat, const int* solved,
+                             const int* labels)
+{
+    if (int rc = check_game(s, game)) return rc;
+    Core& c = s->core;
+    int vb = c.h_vbase[game], n = c.h_n[game];
+    std::vector<int> st(n);
+    std::vector<int8_t> so(n), la(n);
+    for (int v = 0; v < n; v++) {
+        if (strat[v] < -1 || strat[v] >= n) return fail(PSI_ERR_INVALID, "strategy out of range");
+        st[v] = strat[v] >= 0 ? strat[v] + vb : -1;
+        so[v] = solved ? (int8_t)(solved[v] != 0) : 0;
+        la[v] = labels ? (int8_t)labels[v] : 0;
+    }
+    PSI_HIP(hipMemcpyAsync(c.strat + vb, st.data(), n * 4, hipMemcpyHostToDevice, c.s));
+    PSI_HIP(hipMemcpyAsync(c.solved + vb, so.data(), n, hipMemcpyHostToDevice, c.s));
+    PSI_HIP(hipMemcpyAsync(c.lab + vb, la.data(), n, hipMemcpyHostToDevice, c.s));
+    PSI_HIP(hipStreamSynchronize(c.s));
+    return PSI_OK;
+}
+
+extern "C" int psi_solver_synchronize(psi_solver* s)
+{
+    PSI_HIP(hipStreamSynchronize(s->core.s));
+    return PSI_OK;
+}

@@ -1,0 +1,99 @@
+// Device-resident strategy-improvement engine shared by the single-game
+// solver (GPUParity / GPUList roles) and the batched and serving forms.
+// Many games are solved together as one disjoint union; every game keeps its
+// own control block, so the SI control flow (inner loop, outer loop,
+// convergence) of each game is exactly that of solve_si (gpumain.cpp:18-53)
+// run on that game alone.
+#pragma once
+
+#include "common.h"
+
+#include <vector>
+
+namespace psi {
+
+// Per-game control block (device memory).
+struct GameCtl {
+    int done;       // 1 once switch_strategy(0) switched nothing (or capped)
+    int br;         // valuation rounds (gpumain.cpp:34)
+    int iter;       // outer iterations (gpumain.cpp:45)
+    int sw1;        // a player-1 switch happened this step
+    int sw0;        // a player-0 switch happened this step
+    int outer;      // this step runs mark_solved(0) + switch_strategy(0)
+    int reset_pending;
+    int capped;
+    int max_rounds; // 0 = no cap
+    int reset;      // cpu-reset mode (main.cpp:25-26)
+    int vbase;      // first global vertex id of the game
+    int n;          // vertices of the game
+};
+
+// Jump-round bookkeeping (device memory), rewritten by every valuation.
+// Counts are spread over kSlots addresses per round (one atomic per block into
+// slot blockIdx % kSlots) so that no single address takes more than a few
+// hundred atomics per round.
+constexpr int kSlots = 64;
+constexpr int kMaxRounds = 40;
+struct JumpCnt {
+    unsigned long long slot[kMaxRounds][kSlots]; // (resolved << 32) | still-jumping
+    unsigned long long summary;                  // (stop round + 1) << 32 | cycle-bound count
+    int last;       // last round that ran
+    int err;        // 1: ran out of launched rounds
+};
+
+struct Core {
+    int G = 0;
+    long long N = 0, M = 0;
+    int lmax = 0;          // jump rounds launched per valuation
+    bool trace = false;
+    int trace_cap = 0;     // rounds of trace per game
+    int max_pri = 0;
+    // game (device)
+    int* vinfo = nullptr;  // priority | owner << 30
+    int* off = nullptr;    // N+1 global CSR offsets
+    int* tgt = nullptr;    // M global target ids
+    int* gid = nullptr;    // N game of each vertex
+    // state (device)
+    int* strat = nullptr;
+    int* par = nullptr;    // strategy snapshot of the last valuation (-1 sink)
+    int* dep = nullptr;    // path length to the sink of finite vertices
+    int* key = nullptr;    // (label+1) << 30 | max priority on the path
+    int8_t* solved = nullptr;
+    int8_t* lab = nullptr; // infinite[]: -1, 0, +1
+    int4* S[2] = {nullptr, nullptr};
+    unsigned long long* Z[2] = {nullptr, nullptr};
+    unsigned long long* dig = nullptr;
+    GameCtl* gctl = nullptr;
+    JumpCnt* jc = nullptr;
+    unsigned long long* tr = nullptr;  // G * trace_cap * 4
+    int* flag = nullptr;               // scratch flag for step-wise switch
+    hipStream_t s = nullptr;
+    // host copies of per-game sizes
+    std::vector<int> h_n, h_vbase;
+    std::vector<long long> h_m;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+};
+
+// Builds a core from host CSR arrays of G games laid out back to back
+// (local vertex ids per game).  Validates the input.
+int core_create(Core& c, int G, const int* n_per, const long long* m_per, const int* owner,
+                const int* priority, const int* offsets, const int* targets, bool trace,
+                int trace_cap, hipStream_t s);
+void core_destroy(Core& c);
+int core_init_strat(Core& c);
+int core_set_params(Core& c, int reset, int max_rounds);
+// One valuation (val_init, jump rounds, labels); fuse_mark1 also applies
+// mark_solved(1) and counts the round (the device-resident loop's form).
+int core_valuation(Core& c, bool loop_form);
+int core_mark_solved(Core& c, int player);
+int core_reset_opp(Core& c);
+int core_switch(Core& c, int player, int* switched_host);   // step-wise, syncs
+// Enqueues `steps` SI steps (device-resident loop body).
+int core_enqueue_steps(Core& c, int steps);
+// Runs the loop until every game is done; graph-captured step batches.
+int core_solve(Core& c, int steps_per_check);
+int core_all_done(Core& c, int* done);
+int core_solve_profiled(Core& c, double* times, long long* counts);
+
+} // namespace psi

@@ -1,0 +1,813 @@
+// Device-resident strategy improvement with exact compact valuations.
+//
+// Reference semantics: CPUParity (cpu_parity.cpp:95-348) driven by solve_si
+// (gpumain.cpp:18-53).  Reference GPU roles: GPUParity (gpu_parity.cu:56-394):
+// d_init_strat :78, the tree-Wyllie valuation d_init_val/d_val_step/
+// d_mark_infinite :101-227, compare_valuation :231-302, d_switch_strategy
+// :304-357, d_mark_solved :361-376, get_winning :380-394.
+//
+// Valuation.  A finite vertex's valuation is the multiset of priorities on its
+// strategy path to the sink.  Instead of the reference's dense count vector
+// per vertex (n * (max_pri+1) ints, gpu_parity.cu:66-73) each vertex keeps
+//   par  - its strategy successor at valuation time (the sink tree),
+//   dep  - the length of its path to the sink,
+//   key  - (label + 1) << 30 | the largest priority on that path,
+// computed by pointer jumping over the strategy graph: (next, max, count)
+// records combine associatively, so ceil(log2 d) + 1 rounds resolve every
+// vertex at distance d from the sink or from a solved vertex.  A round that
+// resolves nothing proves that every still-jumping vertex leads into a cycle;
+// from then on the rounds only have to make the jump window cover the longest
+// cycle, after which the window maximum at the cycle vertex gives the cycle's
+// largest priority and so the label (cpu_parity.cpp:76-93, :122-130).
+// Solved vertices are terminals with their frozen label (cpu_parity.cpp:157-168).
+//
+// Compare (total preorder of cpu_parity.cpp:195-277).  +inf > finite > -inf.
+// Two finite valuations whose path maxima differ are ordered by that maximum
+// alone: the larger one appears only on one side, so it is the maxdiff
+// priority.  Equal maxima (a few percent of compares on random games) take the
+// exact path: walk both paths to their lowest common ancestor with par/dep,
+// then compare the two private segments from the top priority down,
+// (max, count) level by level.  All integer, no approximation.
+//
+// Control.  Every game has a GameCtl block.  One SI step = valuation, br++,
+// mark_solved(1), switch_strategy(1), and for games whose player-1 switch was
+// empty: mark_solved(0), iter++, switch_strategy(0), done if empty.  Kernels
+// read the blocks and skip finished games, so steps can be enqueued (and
+// captured in a hipGraph) without any host round trip.
+#include "si_core.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace psi {
+
+constexpr int kBS = 256;
+constexpr int kPriMask = 0x3FFFFFFF;
+
+__device__ __forceinline__ int pri_of(int vi) { return vi & kPriMask; }
+__device__ __forceinline__ int owner_of(int vi) { return (vi >> 30) & 1; }
+
+__device__ __forceinline__ unsigned long long hmix_i32(int i, int v)
+{
+    return splitmix64(((unsigned long long)(uint32_t)i << 32) | (unsigned long long)(uint32_t)v);
+}
+
+__device__ __forceinline__ unsigned long long hmix_u64(int i, unsigned long long v)
+{
+    return splitmix64(v ^ splitmix64((unsigned long long)(uint32_t)i + 0x51ED27ull));
+}
+
+// Adds val to dst (per lane); one atomic per wave when the wave's lanes all
+// target the same address.  Must be called by every lane of the wave.
+__device__ __forceinline__ void wave_add_u64(unsigned long long* dst, unsigned long long val)
+{
+    unsigned long long d0 = __shfl((unsigned long long)dst, 0);
+    if (__all((unsigned long long)dst == d0)) {
+        for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o);
+        if ((threadIdx.x & (kWave - 1)) == 0 && dst) atomicAdd(dst, val);
+    } else if (dst) {
+        atomicAdd(dst, val);
+    }
+}
+
+// Phase bookkeeping for round r: stop = first round that resolved nothing
+// (-1 if none yet), cyc = vertices still jumping after it (all lead into
+// cycles).  jc->summary holds the state through round r-2 (written by block 0
+// of round r-1); round r-1 is read from its slots.
+__device__ __forceinline__ void round_state(int r, const JumpCnt* jc, int* stop, int* cyc)
+{
+    unsigned long long sm = *(volatile const unsigned long long*)&jc->summary;
+    int st = (int)(sm >> 32) - 1, cy = (int)(sm & 0xFFFFFFFFu);
+    if (r >= 1 && st < 0) {
+        unsigned long long res = 0, act = 0;
+        for (int k = 0; k < kSlots; k++) {
+            unsigned long long x = jc->slot[r - 1][k];
+            res += x >> 32;
+            act += x & 0xFFFFFFFFu;
+        }
+        if (res == 0) {
+            st = r - 1;
+            cy = (int)act;
+        }
+    }
+    *stop = st;
+    *cyc = cy;
+}
+
+__device__ __forceinline__ int decide_round(int r, const JumpCnt* jc, int maxn)
+{
+    if (r == 0) return 1;
+    if (r >= kMaxRounds) return 0;
+    int stop, cyc;
+    round_state(r, jc, &stop, &cyc);
+    if (stop < 0) return 1;
+    cyc = min(cyc, maxn);
+    return (r < 30 && (1 << r) < cyc) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// init_strat (gpu_parity.cu:78-96; cpu_parity.cpp:317-327)
+__global__ void k_init_strat(long long N, const int* __restrict__ vinfo, const int* __restrict__ off,
+                             const int* __restrict__ tgt, int* __restrict__ strat,
+                             int8_t* __restrict__ solved, int8_t* __restrict__ lab)
+{
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    strat[v] = owner_of(vinfo[v]) == 0 ? -1 : tgt[off[v]];
+    solved[v] = 0;
+    lab[v] = 0;
+}
+
+__global__ void k_reset_ctl(int G, GameCtl* gc)
+{
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    GameCtl c = gc[g];
+    c.done = 0;
+    c.br = 0;
+    c.iter = 0;
+    c.sw1 = 0;
+    c.sw0 = 0;
+    c.outer = 0;
+    c.reset_pending = 0;
+    c.capped = 0;
+    gc[g] = c;
+}
+
+// reset_opponent_strategy (cpu_parity.cpp:341-348)
+__global__ void k_reset_opp(long long N, const int* __restrict__ vinfo, const int* __restrict__ off,
+                            const int* __restrict__ tgt, int* __restrict__ strat)
+{
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    if (owner_of(vinfo[v]) == 1) strat[v] = tgt[off[v]];
+}
+
+// Level-0 jump records.  S = (next or terminal code, max priority, count, -).
+// Terminal codes: -1 sink, -2 solved +inf, -3 solved -inf, -4 finished game.
+__global__ void k_val_init(long long N, const int* __restrict__ vinfo, const int* __restrict__ off,
+                           const int* __restrict__ tgt, const int* __restrict__ gid,
+                           const GameCtl* __restrict__ gc, int* __restrict__ strat,
+                           const int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
+                           int* __restrict__ par, int4* __restrict__ S0,
+                           unsigned long long* __restrict__ Z0, JumpCnt* __restrict__ jc, int trace)
+{
+    if (blockIdx.x == 0) {
+        unsigned long long* sl = &jc->slot[0][0];
+        for (int i = threadIdx.x; i < kMaxRounds * kSlots; i += blockDim.x) sl[i] = 0;
+        if (threadIdx.x == 0) {
+            jc->summary = 0;
+            jc->last = -1;                         // jc->err stays sticky
+        }
+    }
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    int g = gid[v];
+    if (gc[g].done) {
+        S0[v] = make_int4(-4, -1, 0, 0);
+        return;
+    }
+    int vi = vinfo[v];
+    int s = strat[v];
+    if (gc[g].reset_pending && owner_of(vi) == 1) {   // main.cpp:25-26
+        s = tgt[off[v]];
+        strat[v] = s;
+    }
+    if (solved[v]) {
+        S0[v] = make_int4(lab[v] == 1 ? -2 : -3, -1, 0, 0);
+        par[v] = -1;
+        if (trace) Z0[v] = 0;
+        return;
+    }
+    par[v] = s;
+    S0[v] = make_int4(s, pri_of(vi), 1, 0);
+    if (trace) Z0[v] = splitmix64((unsigned long long)(uint32_t)pri_of(vi));
+}
+
+__global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn,
+                                              const int4* __restrict__ Sin, int4* __restrict__ Sout,
+                                              const unsigned long long* __restrict__ Zin,
+                                              unsigned long long* __restrict__ Zout,
+                                              JumpCnt* __restrict__ jc, int trace)
+{
+    __shared__ int run_s;
+    __shared__ unsigned int cnt_res, cnt_act;
+    if (threadIdx.x == 0) {
+        int stop, cyc;
+        round_state(r, jc, &stop, &cyc);
+        int run = r == 0 || stop < 0 || (r < 30 && (1 << r) < min(cyc, maxn));
+        if (r >= kMaxRounds) run = 0;
+        run_s = run;
+        cnt_res = 0;
+        cnt_act = 0;
+        if (blockIdx.x == 0) {
+            jc->summary = ((unsigned long long)(stop + 1) << 32) | (unsigned)cyc;
+            if (run) jc->last = r;
+        }
+    }
+    __syncthreads();
+    if (!run_s) return;
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool res = false, act = false;
+    if (v < N) {
+        int4 s = Sin[v];
+        if (s.x >= 0) {
+            int4 t = Sin[s.x];
+            if (trace) Zout[v] = Zin[v] + Zin[s.x];
+            s.x = t.x;
+            s.y = max(s.y, t.y);
+            s.z += t.z;
+            res = s.x < 0;
+            act = !res;
+        } else if (trace) {
+            Zout[v] = Zin[v];
+        }
+        Sout[v] = s;
+    }
+    unsigned long long mr = __ballot(res), ma = __ballot(act);
+    if ((threadIdx.x & (kWave - 1)) == 0 && (mr | ma)) {
+        atomicAdd(&cnt_res, (unsigned)__popcll(mr));
+        atomicAdd(&cnt_act, (unsigned)__popcll(ma));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && (cnt_res | cnt_act))
+        atomicAdd(&jc->slot[r][blockIdx.x & (kSlots - 1)],
+                  ((unsigned long long)cnt_res << 32) | cnt_act);
+}
+
+// Labels (d_mark_infinite role), compare keys, trace hashes; in the loop form
+// also mark_solved(1) (cpu_parity.cpp:306-315), which solve_si calls right
+// after every valuation.
+__global__ void __launch_bounds__(kBS) k_label(
+    long long N, int maxn, int lmax, const int4* __restrict__ S0, const int4* __restrict__ S1,
+    const unsigned long long* __restrict__ Z0, const unsigned long long* __restrict__ Z1,
+    const int* __restrict__ gid, const GameCtl* __restrict__ gc, const int* __restrict__ strat,
+    int8_t* __restrict__ solved, int8_t* __restrict__ lab, int* __restrict__ dep,
+    int* __restrict__ key, unsigned long long* __restrict__ dig, unsigned long long* __restrict__ tr,
+    int trace_cap, int loop_form, int trace, JumpCnt* __restrict__ jc)
+{
+    const int last = jc->last;
+    const int F = last + 1;
+    const int4* SF = (F & 1) ? S1 : S0;
+    const unsigned long long* ZF = (F & 1) ? Z1 : Z0;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && decide_round(lmax, jc, maxn)) jc->err = 1;
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int g = -1, l = 0, sol = 0, loc = 0;
+    unsigned long long d = 0;
+    bool live = false;
+    if (v < N) {
+        g = gid[v];
+        live = !gc[g].done;
+    }
+    if (live) {
+        sol = solved[v];
+        loc = (int)(v - gc[g].vbase);
+        if (sol) {
+            l = lab[v];
+            key[v] = (l + 1) << 30;
+            dep[v] = 0;
+        } else {
+            int4 s = SF[v];
+            if (s.x == -1) {
+                l = 0;
+                dep[v] = s.z;
+                key[v] = (1 << 30) | s.y;
+                if (trace) d = ZF[v];
+            } else {
+                if (s.x == -2) l = 1;
+                else if (s.x == -3) l = -1;
+                else l = (SF[s.x].y & 1) ? -1 : 1;   // largest priority on the cycle
+                key[v] = (l + 1) << 30;
+                dep[v] = 0;
+            }
+            lab[v] = (int8_t)l;
+            if (loop_form && l == -1) solved[v] = 1;
+        }
+        if (trace) dig[v] = d;
+    }
+    if (trace) {
+        int br = live ? gc[g].br : 0;
+        bool rec = live && br < trace_cap;
+        unsigned long long* base = rec ? tr + ((size_t)g * trace_cap + br) * 4 : nullptr;
+        wave_add_u64(rec ? base + 0 : nullptr, rec ? hmix_i32(loc, strat[v]) : 0);
+        wave_add_u64(rec ? base + 1 : nullptr, rec ? hmix_i32(loc, sol) : 0);
+        wave_add_u64(rec ? base + 2 : nullptr, rec ? hmix_i32(loc, l) : 0);
+        wave_add_u64(rec ? base + 3 : nullptr, rec ? hmix_u64(loc, d) : 0);
+    }
+}
+
+// Exact compare of two finite vertices whose path maxima are equal.
+__device__ int cmp_exact(int u, int w, const int* __restrict__ par, const int* __restrict__ dep,
+                         const int* __restrict__ vinfo)
+{
+    int a = u, b = w, da = dep[u], db = dep[w];
+    while (da > db) { a = par[a]; da--; }
+    while (db > da) { b = par[b]; db--; }
+    while (a != b) { a = par[a]; b = par[b]; da--; }
+    const int la = dep[u] - da, lb = dep[w] - da;
+    int thr = 0x7FFFFFFF;
+    for (;;) {
+        int ma = -1, ca = 0, mb = -1, cb = 0;
+        int x = u;
+        for (int i = 0; i < la; i++) {
+            int p = pri_of(vinfo[x]);
+            if (p < thr) {
+                if (p > ma) { ma = p; ca = 1; }
+                else if (p == ma) ca++;
+            }
+            x = par[x];
+        }
+        x = w;
+        for (int i = 0; i < lb; i++) {
+            int p = pri_of(vinfo[x]);
+            if (p < thr) {
+                if (p > mb) { mb = p; cb = 1; }
+                else if (p == mb) cb++;
+            }
+            x = par[x];
+        }
+        if (ma != mb) {
+            if (ma > mb) return (ma & 1) ? -1 : 1;
+            return (mb & 1) ? 1 : -1;
+        }
+        if (ma < 0) return 0;
+        if (ca != cb) {
+            if ((ma & 1) == 0) return ca > cb ? 1 : -1;
+            return ca < cb ? 1 : -1;
+        }
+        thr = ma;
+    }
+}
+
+// Sign of compare_valuation(u, w) (cpu_parity.cpp:235-277); -1 is the sink.
+__device__ __forceinline__ int cmp_val(int u, int w, const int* __restrict__ key,
+                                       const int* __restrict__ par, const int* __restrict__ dep,
+                                       const int* __restrict__ vinfo)
+{
+    if (u == w) return 0;
+    int ku = u < 0 ? (1 << 30) : key[u];
+    int kw = w < 0 ? (1 << 30) : key[w];
+    int cu = ku >> 30, cw = kw >> 30;
+    if (cu != cw) return cu > cw ? 1 : -1;
+    if (cu != 1) return 0;
+    int pu = u < 0 ? -1 : (ku & kPriMask);
+    int pw = w < 0 ? -1 : (kw & kPriMask);
+    if (pu != pw) {
+        if (pu > pw) return (pu & 1) ? -1 : 1;
+        return (pw & 1) ? 1 : -1;
+    }
+    return cmp_exact(u, w, par, dep, vinfo);
+}
+
+// switch_strategy(player) (cpu_parity.cpp:280-304, gpu_parity.cu:304-357):
+// keep the current successor unless one is strictly better; the first
+// strictly-better-than-current edge in CSR order wins each step of the scan.
+// Loop form with player 0 also applies mark_solved(0) first.
+__global__ void __launch_bounds__(kBS) k_switch(
+    int player, long long N, const int* __restrict__ vinfo, const int* __restrict__ off,
+    const int* __restrict__ tgt, const int* __restrict__ gid, GameCtl* __restrict__ gc,
+    int* __restrict__ strat, int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
+    const int* __restrict__ key, const int* __restrict__ dep, const int* __restrict__ par,
+    int loop_form, int* __restrict__ flag)
+{
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool sw = false;
+    int g = -1;
+    if (v < N) {
+        g = gid[v];
+        bool live = !gc[g].done && (!loop_form || player == 1 || gc[g].outer);
+        if (live) {
+            if (loop_form && player == 0 && lab[v] == 1) solved[v] = 1;
+            int vi = vinfo[v];
+            if (owner_of(vi) == player) {
+                int cur = strat[v];
+                int e1 = off[v + 1];
+                for (int e = off[v]; e < e1; e++) {
+                    int u = tgt[e];
+                    int c = cmp_val(u, cur, key, par, dep, vinfo);
+                    if (player == 0 ? c > 0 : c < 0) {
+                        cur = u;
+                        sw = true;
+                    }
+                }
+                if (sw) strat[v] = cur;
+            }
+        }
+    }
+    // one store per wave and game: the flag only has to become nonzero
+    unsigned long long m = __ballot(sw);
+    if (m) {
+        int g0 = __shfl(g, __ffsll((long long)m) - 1);
+        bool uni = __all(!sw || g == g0);
+        if (uni) {
+            if ((threadIdx.x & (kWave - 1)) == (unsigned)(__ffsll((long long)m) - 1)) {
+                if (loop_form) {
+                    if (player == 1) gc[g0].sw1 = 1; else gc[g0].sw0 = 1;
+                } else {
+                    *flag = 1;
+                }
+            }
+        } else if (sw) {
+            if (loop_form) {
+                if (player == 1) gc[g].sw1 = 1; else gc[g].sw0 = 1;
+            } else {
+                *flag = 1;
+            }
+        }
+    }
+}
+
+// mark_solved(player) as a separate step (cpu_parity.cpp:306-315).
+__global__ void k_mark_solved(int player, long long N, const int8_t* __restrict__ lab,
+                              int8_t* __restrict__ solved)
+{
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    if ((player == 0 && lab[v] == 1) || (player == 1 && lab[v] == -1)) solved[v] = 1;
+}
+
+// After switch_strategy(1): count the round; the inner loop ends when no
+// player-1 vertex switched (gpumain.cpp:34-38).
+__global__ void k_ctl1(int G, GameCtl* gc)
+{
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    GameCtl c = gc[g];
+    if (c.done) return;
+    c.br++;
+    c.outer = c.sw1 == 0;
+    c.sw1 = 0;
+    c.reset_pending = 0;
+    gc[g] = c;
+}
+
+// After switch_strategy(0) (gpumain.cpp:44-50).
+__global__ void k_ctl0(int G, GameCtl* gc)
+{
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    GameCtl c = gc[g];
+    if (c.done) return;
+    if (c.outer) {
+        c.iter++;
+        if (c.sw0 == 0) c.done = 1;
+        else if (c.reset) c.reset_pending = 1;
+    }
+    c.sw0 = 0;
+    c.outer = 0;
+    if (!c.done && c.max_rounds > 0 && c.br >= c.max_rounds) {
+        c.capped = 1;
+        c.done = 1;
+    }
+    gc[g] = c;
+}
+
+__global__ void k_count_live(int G, const GameCtl* gc, int* out)
+{
+    __shared__ int acc;
+    if (threadIdx.x == 0) acc = 0;
+    __syncthreads();
+    int live = 0;
+    for (int g = threadIdx.x; g < G; g += blockDim.x) live += gc[g].done ? 0 : 1;
+    atomicAdd(&acc, live);
+    __syncthreads();
+    if (threadIdx.x == 0) *out = acc;
+}
+
+template <class T> __global__ void k_fill(T* p, long long n, T v)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+static int validate(int G, const int* n_per, const long long* m_per, const int* owner,
+                    const int* priority, const int* offsets, const int* targets, long long* N,
+                    long long* M, int* maxn, int* max_pri)
+{
+    long long vb = 0, eb = 0;
+    *maxn = 0;
+    *max_pri = 0;
+    for (int g = 0; g < G; g++) {
+        int n = n_per[g];
+        long long m = m_per[g];
+        if (n <= 0) return fail(PSI_ERR_INVALID, "game with no vertices");
+        const int* off = offsets + vb + g;   // each game has n+1 offsets
+        if (off[0] != 0 || (long long)off[n] != m)
+            return fail(PSI_ERR_INVALID, "offsets must start at 0 and end at the edge count");
+        for (int v = 0; v < n; v++) {
+            if (off[v + 1] <= off[v]) return fail(PSI_ERR_INVALID, "sinks are not allowed!");
+            int o = owner[vb + v], p = priority[vb + v];
+            if (o != 0 && o != 1) return fail(PSI_ERR_INVALID, "invalid player");
+            if (p < 0 || p > kPriMask) return fail(PSI_ERR_INVALID, "priority out of range [0, 2^30)");
+            if (p > *max_pri) *max_pri = p;
+        }
+        for (long long e = 0; e < m; e++) {
+            int t = targets[eb + e];
+            if (t < 0 || t >= n) return fail(PSI_ERR_INVALID, "successor out of range");
+        }
+        if (n > *maxn) *maxn = n;
+        vb += n;
+        eb += m;
+    }
+    if (vb >= (1LL << 31) - 1 || eb >= (1LL << 31) - 1)
+        return fail(PSI_ERR_INVALID, "total vertices and edges must fit in int32");
+    *N = vb;
+    *M = eb;
+    return PSI_OK;
+}
+
+template <class T> static int dalloc(T** p, long long n)
+{
+    PSI_HIP(hipMalloc((void**)p, (size_t)std::max(n, 1LL) * sizeof(T)));
+    return PSI_OK;
+}
+
+int core_create(Core& c, int G, const int* n_per, const long long* m_per, const int* owner,
+                const int* priority, const int* offsets, const int* targets, bool trace,
+                int trace_cap, hipStream_t s)
+{
+    long long N, M;
+    int maxn, max_pri;
+    if (G <= 0) return fail(PSI_ERR_INVALID, "no games");
+    if (int rc = validate(G, n_per, m_per, owner, priority, offsets, targets, &N, &M, &maxn, &max_pri))
+        return rc;
+    c.G = G;
+    c.N = N;
+    c.M = M;
+    c.s = s;
+    c.trace = trace;
+    c.trace_cap = trace ? std::max(trace_cap, 1) : 0;
+    c.max_pri = max_pri;
+    c.lmax = ceil_log2(maxn) + 2;
+    c.h_n.assign(n_per, n_per + G);
+    c.h_m.assign(m_per, m_per + G);
+    c.h_vbase.resize(G);
+    // Global CSR with global target ids.
+    std::vector<int> vinfo(N), off(N + 1), tgt(M), gid(N);
+    std::vector<GameCtl> gc(G);
+    long long vb = 0, eb = 0;
+    for (int g = 0; g < G; g++) {
+        int n = n_per[g];
+        const int* o = offsets + vb + g;
+        for (int v = 0; v < n; v++) {
+            vinfo[vb + v] = priority[vb + v] | (owner[vb + v] << 30);
+            off[vb + v] = (int)(eb + o[v]);
+            gid[vb + v] = g;
+        }
+        for (long long e = 0; e < m_per[g]; e++) tgt[eb + e] = (int)(targets[eb + e] + vb);
+        std::memset(&gc[g], 0, sizeof(GameCtl));
+        gc[g].vbase = (int)vb;
+        gc[g].n = n;
+        c.h_vbase[g] = (int)vb;
+        vb += n;
+        eb += m_per[g];
+    }
+    off[N] = (int)M;
+    if (int rc = dalloc(&c.vinfo, N)) return rc;
+    if (int rc = dalloc(&c.off, N + 1)) return rc;
+    if (int rc = dalloc(&c.tgt, M)) return rc;
+    if (int rc = dalloc(&c.gid, N)) return rc;
+    if (int rc = dalloc(&c.strat, N)) return rc;
+    if (int rc = dalloc(&c.par, N)) return rc;
+    if (int rc = dalloc(&c.dep, N)) return rc;
+    if (int rc = dalloc(&c.key, N)) return rc;
+    if (int rc = dalloc(&c.solved, N)) return rc;
+    if (int rc = dalloc(&c.lab, N)) return rc;
+    if (int rc = dalloc(&c.S[0], N)) return rc;
+    if (int rc = dalloc(&c.S[1], N)) return rc;
+    if (trace) {
+        if (int rc = dalloc(&c.Z[0], N)) return rc;
+        if (int rc = dalloc(&c.Z[1], N)) return rc;
+        if (int rc = dalloc(&c.dig, N)) return rc;
+        if (int rc = dalloc(&c.tr, (long long)G * c.trace_cap * 4)) return rc;
+    }
+    if (int rc = dalloc(&c.gctl, G)) return rc;
+    if (int rc = dalloc(&c.jc, 1)) return rc;
+    if (int rc = dalloc(&c.flag, 4)) return rc;
+    PSI_HIP(hipMemcpyAsync(c.vinfo, vinfo.data(), N * 4, hipMemcpyHostToDevice, s));
+    PSI_HIP(hipMemcpyAsync(c.off, off.data(), (N + 1) * 4, hipMemcpyHostToDevice, s));
+    PSI_HIP(hipMemcpyAsync(c.tgt, tgt.data(), M * 4, hipMemcpyHostToDevice, s));
+    PSI_HIP(hipMemcpyAsync(c.gid, gid.data(), N * 4, hipMemcpyHostToDevice, s));
+    PSI_HIP(hipMemcpyAsync(c.gctl, gc.data(), G * sizeof(GameCtl), hipMemcpyHostToDevice, s));
+    PSI_HIP(hipStreamSynchronize(s));
+    return PSI_OK;
+}
+
+void core_destroy(Core& c)
+{
+    if (c.graph) (void)hipGraphExecDestroy(c.graph);
+    void* ptrs[] = {c.vinfo, c.off, c.tgt, c.gid, c.strat, c.par, c.dep, c.key, c.solved, c.lab,
+                    c.S[0], c.S[1], c.Z[0], c.Z[1], c.dig, c.tr, c.gctl, c.jc, c.flag};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    c = Core();
+}
+
+#define GRID(n) nblocks((n), kBS), kBS, 0, c.s
+
+int core_set_params(Core& c, int reset, int max_rounds)
+{
+    std::vector<GameCtl> gc(c.G);
+    PSI_HIP(hipMemcpyAsync(gc.data(), c.gctl, c.G * sizeof(GameCtl), hipMemcpyDeviceToHost, c.s));
+    PSI_HIP(hipStreamSynchronize(c.s));
+    for (auto& x : gc) {
+        x.reset = reset;
+        x.max_rounds = max_rounds;
+    }
+    PSI_HIP(hipMemcpyAsync(c.gctl, gc.data(), c.G * sizeof(GameCtl), hipMemcpyHostToDevice, c.s));
+    PSI_HIP(hipStreamSynchronize(c.s));
+    return PSI_OK;
+}
+
+int core_init_strat(Core& c)
+{
+    k_init_strat<<<GRID(c.N)>>>(c.N, c.vinfo, c.off, c.tgt, c.strat, c.solved, c.lab);
+    k_reset_ctl<<<GRID(c.G)>>>(c.G, c.gctl);
+    k_fill<int><<<1, 256, 0, c.s>>>((int*)c.jc, (long long)(sizeof(JumpCnt) / 4), 0);
+    if (c.trace) k_fill<unsigned long long><<<GRID((long long)c.G * c.trace_cap * 4)>>>(
+        c.tr, (long long)c.G * c.trace_cap * 4, 0ull);
+    PSI_LAUNCH_CHECK();
+    return PSI_OK;
+}
+
+int core_valuation(Core& c, bool loop_form)
+{
+    int maxn = 0;
+    for (int n : c.h_n) maxn = std::max(maxn, n);
+    k_val_init<<<GRID(c.N)>>>(c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
+                              c.par, c.S[0], c.Z[0], c.jc, c.trace);
+    for (int r = 0; r < c.lmax; r++)
+        k_jump<<<GRID(c.N)>>>(r, c.N, maxn, c.S[r & 1], c.S[(r + 1) & 1], c.Z[r & 1],
+                              c.Z[(r + 1) & 1], c.jc, c.trace);
+    k_label<<<GRID(c.N)>>>(c.N, maxn, c.lmax, c.S[0], c.S[1], c.Z[0], c.Z[1], c.gid, c.gctl,
+                           c.strat, c.solved, c.lab, c.dep, c.key, c.dig, c.tr, c.trace_cap,
+                           loop_form, c.trace, c.jc);
+    PSI_LAUNCH_CHECK();
+    return PSI_OK;
+}
+
+int core_reset_opp(Core& c)
+{
+    k_reset_opp<<<GRID(c.N)>>>(c.N, c.vinfo, c.off, c.tgt, c.strat);
+    PSI_LAUNCH_CHECK();
+    return PSI_OK;
+}
+
+int core_mark_solved(Core& c, int player)
+{
+    k_mark_solved<<<GRID(c.N)>>>(player, c.N, c.lab, c.solved);
+    PSI_LAUNCH_CHECK();
+    return PSI_OK;
+}
+
+int core_switch(Core& c, int player, int* switched_host)
+{
+    k_fill<int><<<1, 64, 0, c.s>>>(c.flag, 1, 0);
+    k_switch<<<GRID(c.N)>>>(player, c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved,
+                            c.lab, c.key, c.dep, c.par, 0, c.flag);
+    PSI_LAUNCH_CHECK();
+    int h = 0;
+    PSI_HIP(hipMemcpyAsync(&h, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.s));
+    PSI_HIP(hipStreamSynchronize(c.s));
+    *switched_host = h;
+    return PSI_OK;
+}
+
+int core_enqueue_steps(Core& c, int steps)
+{
+    for (int k = 0; k < steps; k++) {
+        if (int rc = core_valuation(c, true)) return rc;
+        k_switch<<<GRID(c.N)>>>(1, c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved,
+                                c.lab, c.key, c.dep, c.par, 1, c.flag);
+        k_ctl1<<<GRID(c.G)>>>(c.G, c.gctl);
+        k_switch<<<GRID(c.N)>>>(0, c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved,
+                                c.lab, c.key, c.dep, c.par, 1, c.flag);
+        k_ctl0<<<GRID(c.G)>>>(c.G, c.gctl);
+        PSI_LAUNCH_CHECK();
+    }
+    return PSI_OK;
+}
+
+// The same loop launched kernel by kernel with HIP events between launches, to
+// attribute device time per kernel.  times[k], counts[k] for k in
+// PSI_K_VAL_INIT..PSI_K_CTL; a jump launch that found nothing to do is counted
+// under PSI_K_JUMP_IDLE.
+int core_solve_profiled(Core& c, double* times, long long* counts)
+{
+    for (int k = 0; k < PSI_K_COUNT; k++) {
+        times[k] = 0;
+        counts[k] = 0;
+    }
+    int maxn = 0;
+    for (int n : c.h_n) maxn = std::max(maxn, n);
+    const int nev = c.lmax + 8;
+    std::vector<hipEvent_t> ev(nev);
+    for (auto& e : ev) PSI_HIP(hipEventCreate(&e));
+    auto span = [&](int a, int b) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev[a], ev[b]);
+        return (double)ms;
+    };
+    int rc = PSI_OK;
+    for (int step = 0; step < (1 << 20); step++) {
+        int k = 0;
+        PSI_HIP(hipEventRecord(ev[k++], c.s));
+        k_val_init<<<GRID(c.N)>>>(c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved,
+                                  c.lab, c.par, c.S[0], c.Z[0], c.jc, c.trace);
+        PSI_HIP(hipEventRecord(ev[k++], c.s));
+        for (int r = 0; r < c.lmax; r++) {
+            k_jump<<<GRID(c.N)>>>(r, c.N, maxn, c.S[r & 1], c.S[(r + 1) & 1], c.Z[r & 1],
+                                  c.Z[(r + 1) & 1], c.jc, c.trace);
+            PSI_HIP(hipEventRecord(ev[k++], c.s));
+        }
+        k_label<<<GRID(c.N)>>>(c.N, maxn, c.lmax, c.S[0], c.S[1], c.Z[0], c.Z[1], c.gid, c.gctl,
+                               c.strat, c.solved, c.lab, c.dep, c.key, c.dig, c.tr, c.trace_cap, 1,
+                               c.trace, c.jc);
+        PSI_HIP(hipEventRecord(ev[k++], c.s));
+        k_switch<<<GRID(c.N)>>>(1, c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved,
+                                c.lab, c.key, c.dep, c.par, 1, c.flag);
+        PSI_HIP(hipEventRecord(ev[k++], c.s));
+        k_ctl1<<<GRID(c.G)>>>(c.G, c.gctl);
+        PSI_HIP(hipEventRecord(ev[k++], c.s));
+        k_switch<<<GRID(c.N)>>>(0, c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved,
+                                c.lab, c.key, c.dep, c.par, 1, c.flag);
+        PSI_HIP(hipEventRecord(ev[k++], c.s));
+        k_ctl0<<<GRID(c.G)>>>(c.G, c.gctl);
+        PSI_HIP(hipEventRecord(ev[k++], c.s));
+        PSI_LAUNCH_CHECK();
+        PSI_HIP(hipStreamSynchronize(c.s));
+        JumpCnt jc;
+        PSI_HIP(hipMemcpy(&jc, c.jc, sizeof jc, hipMemcpyDeviceToHost));
+        int e = 0;
+        times[PSI_K_VAL_INIT] += span(e, e + 1);
+        counts[PSI_K_VAL_INIT]++;
+        e++;
+        for (int r = 0; r < c.lmax; r++, e++) {
+            int cls = r <= jc.last ? PSI_K_JUMP : PSI_K_JUMP_IDLE;
+            times[cls] += span(e, e + 1);
+            counts[cls]++;
+        }
+        times[PSI_K_LABEL] += span(e, e + 1); counts[PSI_K_LABEL]++; e++;
+        times[PSI_K_SWITCH1] += span(e, e + 1); counts[PSI_K_SWITCH1]++; e++;
+        times[PSI_K_CTL] += span(e, e + 1); counts[PSI_K_CTL]++; e++;
+        times[PSI_K_SWITCH0] += span(e, e + 1); counts[PSI_K_SWITCH0]++; e++;
+        times[PSI_K_CTL] += span(e, e + 1); counts[PSI_K_CTL]++; e++;
+        times[PSI_K_TOTAL] += span(0, e);
+        counts[PSI_K_TOTAL]++;
+        int done = 0;
+        if ((rc = core_all_done(c, &done))) break;
+        if (done) break;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int core_all_done(Core& c, int* done)
+{
+    k_count_live<<<1, 256, 0, c.s>>>(c.G, c.gctl, c.flag + 1);
+    PSI_LAUNCH_CHECK();
+    int live = 0;
+    PSI_HIP(hipMemcpyAsync(&live, c.flag + 1, sizeof(int), hipMemcpyDeviceToHost, c.s));
+    PSI_HIP(hipStreamSynchronize(c.s));
+    *done = live == 0;
+    return PSI_OK;
+}
+
+// Captures `steps` SI steps once into a graph and replays it until every game
+// is done.  No host round trip inside a batch of steps.
+int core_solve(Core& c, int steps_per_check)
+{
+    if (steps_per_check < 1) steps_per_check = 1;
+    if (!c.graph || c.graph_steps != steps_per_check) {
+        if (c.graph) {
+            (void)hipGraphExecDestroy(c.graph);
+            c.graph = nullptr;
+        }
+        hipGraph_t gr;
+        PSI_HIP(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
+        int rc = core_enqueue_steps(c, steps_per_check);
+        hipError_t e = hipStreamEndCapture(c.s, &gr);
+        if (rc) return rc;
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        e = hipGraphInstantiate(&c.graph, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+        c.graph_steps = steps_per_check;
+    }
+    for (;;) {
+        PSI_HIP(hipGraphLaunch(c.graph, c.s));
+        int done = 0;
+        if (int rc = core_all_done(c, &done)) return rc;
+        if (done) break;
+    }
+    JumpCnt jc;
+    PSI_HIP(hipMemcpyAsync(&jc, c.jc, sizeof jc, hipMemcpyDeviceToHost, c.s));
+    PSI_HIP(hipStreamSynchronize(c.s));
+    if (jc.err) return fail(PSI_ERR_INVALID, "internal: jump rounds exhausted");
+    return PSI_OK;
+}
+
+} // namespace psi

@@ -239,3 +239,145 @@ def list_rank_status():
 
 def list_rank_tune(param, value):
     check(lib.psi_list_rank_tune(param, value), "psi_list_rank_tune")
+
+
+# ---- solvers ---------------------------------------------------------------
+SOLVER_TRACE = 1
+_pp = C.POINTER(_vp)
+_sig("psi_game_create", C.c_int, C.c_int, C.c_int, _i32p, _i32p, _i32p, _i32p, _pp)
+_sig("psi_game_destroy", C.c_int, _vp)
+_sig("psi_game_info", C.c_int, _vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int))
+_sig("psi_solver_create", C.c_int, _vp, C.c_int, C.c_int, _pp)
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_sig("psi_solver_create_batch", C.c_int, C.c_int, _i32p, _i64p, _i32p, _i32p, _i32p, _i32p,
+     C.c_int, C.c_int, _pp)
+_sig("psi_solver_destroy", C.c_int, _vp)
+_sig("psi_solver_info", C.c_int, _vp, C.POINTER(C.c_int), C.POINTER(C.c_int64),
+     C.POINTER(C.c_int64))
+_sig("psi_solver_synchronize", C.c_int, _vp)
+_sig("psi_init_strat", C.c_int, _vp)
+_sig("psi_compute_valuation", C.c_int, _vp)
+_sig("psi_mark_solved", C.c_int, _vp, C.c_int)
+_sig("psi_switch_strategy", C.c_int, _vp, C.c_int, C.POINTER(C.c_int))
+_sig("psi_reset_opponent_strategy", C.c_int, _vp)
+_sig("psi_solve", C.c_int, _vp, C.c_int, C.c_int, C.c_int)
+_sig("psi_get_stats", C.c_int, _vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+     C.POINTER(C.c_int))
+_sig("psi_get_winning", C.c_int, _vp, C.c_int, C.c_int, _i32p, C.POINTER(C.c_int))
+for _f in ("psi_get_strategy", "psi_get_labels", "psi_get_solved", "psi_get_depths"):
+    _sig(_f, C.c_int, _vp, C.c_int, _i32p)
+_sig("psi_get_digests", C.c_int, _vp, C.c_int, _u64p)
+_sig("psi_get_trace", C.c_int, _vp, C.c_int, _u64p, C.c_int, C.POINTER(C.c_int))
+_sig("psi_set_state", C.c_int, _vp, C.c_int, _i32p, _i32p, _i32p)
+
+
+class Solver:
+    """One psi_solver handle: a single game, or a batch of games solved each
+    as if alone."""
+
+    def __init__(self, games, trace=False, trace_cap=4096):
+        if isinstance(games, Game):
+            games = [games]
+        self.games = list(games)
+        n_per = np.array([g.n for g in self.games], np.int32)
+        m_per = np.array([g.m for g in self.games], np.int64)
+        owner = np.concatenate([g.owner for g in self.games])
+        pri = np.concatenate([g.priority for g in self.games])
+        off = np.concatenate([g.offsets for g in self.games])
+        tgt = np.concatenate([g.targets for g in self.games])
+        self.h = _vp()
+        self.trace = trace
+        self.trace_cap = trace_cap
+        check(lib.psi_solver_create_batch(len(self.games), n_per, m_per, owner, pri, off, tgt,
+                                          SOLVER_TRACE if trace else 0, trace_cap,
+                                          C.byref(self.h)), "psi_solver_create_batch")
+
+    def close(self):
+        if self.h:
+            lib.psi_solver_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, reset=False, max_rounds=0, steps_per_sync=4):
+        check(lib.psi_solve(self.h, int(reset), max_rounds, steps_per_sync), "psi_solve")
+
+    def sync(self):
+        check(lib.psi_solver_synchronize(self.h), "sync")
+
+    # step-wise si.h operations
+    def init_strat(self):
+        check(lib.psi_init_strat(self.h), "init_strat")
+
+    def compute_valuation(self):
+        check(lib.psi_compute_valuation(self.h), "compute_valuation")
+
+    def mark_solved(self, player):
+        check(lib.psi_mark_solved(self.h, player), "mark_solved")
+
+    def switch_strategy(self, player):
+        s = C.c_int()
+        check(lib.psi_switch_strategy(self.h, player, C.byref(s)), "switch_strategy")
+        return s.value
+
+    def reset_opponent_strategy(self):
+        check(lib.psi_reset_opponent_strategy(self.h), "reset_opponent_strategy")
+
+    def set_state(self, strat, solved=None, labels=None, game=0):
+        n = self.games[game].n
+        strat = np.ascontiguousarray(strat, np.int32)
+        solved = np.ascontiguousarray(solved if solved is not None else np.zeros(n), np.int32)
+        labels = np.ascontiguousarray(labels if labels is not None else np.zeros(n), np.int32)
+        check(lib.psi_set_state(self.h, game, strat, solved, labels), "set_state")
+
+    def stats(self, game=0):
+        it, br, cap = C.c_int(), C.c_int(), C.c_int()
+        check(lib.psi_get_stats(self.h, game, C.byref(it), C.byref(br), C.byref(cap)), "stats")
+        return it.value, br.value, bool(cap.value)
+
+    def _get(self, fn, game, dtype=np.int32):
+        out = np.empty(self.games[game].n, dtype)
+        check(getattr(lib, fn)(self.h, game, out), fn)
+        return out
+
+    def strategy(self, game=0):
+        return self._get("psi_get_strategy", game)
+
+    def labels(self, game=0):
+        return self._get("psi_get_labels", game)
+
+    def solved(self, game=0):
+        return self._get("psi_get_solved", game)
+
+    def depths(self, game=0):
+        return self._get("psi_get_depths", game)
+
+    def digests(self, game=0):
+        return self._get("psi_get_digests", game, np.uint64)
+
+    def winning(self, player, game=0):
+        out = np.empty(self.games[game].n, np.int32)
+        k = C.c_int()
+        check(lib.psi_get_winning(self.h, game, player, out, C.byref(k)), "get_winning")
+        return out[: k.value]
+
+    def trace_hashes(self, game=0):
+        buf = np.zeros(4 * self.trace_cap, np.uint64)
+        k = C.c_int()
+        check(lib.psi_get_trace(self.h, game, buf, self.trace_cap, C.byref(k)), "get_trace")
+        return buf[: 4 * k.value].reshape(-1, 4)
+
+K_NAMES = ["val_init", "jump", "jump_idle", "label", "switch1", "switch0", "ctl", "total"]
+_sig("psi_solve_profiled", C.c_int, _vp, C.c_int, C.c_int,
+     np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS"), _i64p)
+
+
+def solve_profiled(solver, reset=False, max_rounds=0):
+    t = np.zeros(8, np.float64)
+    c = np.zeros(8, np.int64)
+    check(lib.psi_solve_profiled(solver.h, int(reset), max_rounds, t, c), "psi_solve_profiled")
+    return {k: (float(t[i]), int(c[i])) for i, k in enumerate(K_NAMES)}

@@ -1,0 +1,189 @@
+"""SI solver on the GPU against the oracle and the reference's golden vectors:
+bit-exact per-round strategies, solved sets, labels and valuation digests,
+final strategies, winning regions and iteration / valuation-round counts."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import fixture_game, load_fixtures
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = load_fixtures()
+CASES = [(fx["name"], mode) for fx in FIXTURES for mode in ("cpu", "cpu-reset")
+         if mode in fx["modes"]]
+
+
+def _fx(name):
+    return next(f for f in FIXTURES if f["name"] == name)
+
+
+@pytest.mark.parametrize("name,mode", CASES)
+def test_golden(gpu, name, mode):
+    print(f"golden {name} {mode}", flush=True)
+    fx = _fx(name)
+    g = fixture_game(fx)
+    exp = fx["modes"][mode]
+    fin = exp["final"]
+    s = gpu.Solver(g, trace=True)
+    s.solve(reset=(mode == "cpu-reset"))
+    it, br, capped = s.stats()
+    assert not capped
+    assert (it, br) == (fin["iter"], fin["br"])
+    assert s.strategy().tolist() == fin["strat"]
+    assert s.labels().tolist() == fin["inf"]
+    assert s.winning(0).tolist() == fin["w0"]
+    assert s.winning(1).tolist() == fin["w1"]
+    got = [[str(int(x)) for x in row] for row in s.trace_hashes()]
+    assert got == exp["round_hashes"]
+    s.close()
+
+
+def _solve_both(gpu, g, reset=False):
+    r = oracle.solve(g, rep=oracle.SPARSE, reset=reset, trace=True)
+    s = gpu.Solver(g, trace=True, trace_cap=max(r.br + 1, 16))
+    s.solve(reset=reset)
+    return r, s
+
+
+def _assert_same(r, s):
+    it, br, capped = s.stats()
+    assert (it, br) == (r.iter, r.br)
+    assert np.array_equal(s.strategy(), r.strat)
+    assert np.array_equal(s.labels(), r.inf)
+    assert np.array_equal(s.solved(), r.solved)
+    assert np.array_equal(s.trace_hashes(), r.hashes)
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_random_small(gpu, seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 400))
+    g = gpu.gen_random_game(9000 + seed, n, int(rng.integers(1, 3 * n)), 1, int(rng.integers(1, 7)))
+    print(f"random n={n} seed={seed}", flush=True)
+    r, s = _solve_both(gpu, g, reset=bool(seed % 3 == 0))
+    _assert_same(r, s)
+
+
+@pytest.mark.parametrize("n,pri,dlo,dhi", [(20000, 100, 1, 4), (65536, 1 << 14, 2, 8),
+                                            (100000, 100000, 1, 6), (200000, 16, 1, 3)])
+def test_random_mid(gpu, n, pri, dlo, dhi):
+    print(f"random mid n={n}", flush=True)
+    g = gpu.gen_random_game(n + pri, n, pri, dlo, dhi)
+    r, s = _solve_both(gpu, g)
+    _assert_same(r, s)
+
+
+@pytest.mark.parametrize("n", [50, 1000, 20000])
+def test_chain_games(gpu, n):
+    print(f"chain n={n}", flush=True)
+    g = gpu.gen_chain_game(77 + n, n)
+    r, s = _solve_both(gpu, g)
+    _assert_same(r, s)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_valuation_of_given_state(gpu, seed):
+    """One valuation of an arbitrary strategy with an arbitrary solved set:
+    labels (cycle parity, inheritance from solved vertices), path lengths and
+    digests equal the oracle's (pins the jump kernels alone)."""
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(5, 3000))
+    g = gpu.gen_random_game(400 + seed, n, int(rng.integers(1, 50)), 1, 4)
+    strat = np.array([g.targets[rng.integers(g.offsets[v], g.offsets[v + 1])] for v in range(n)],
+                     np.int32)
+    strat[(g.owner == 0) & (rng.random(n) < 0.3)] = -1
+    solved = (rng.random(n) < 0.1).astype(np.int32)
+    lab_in = np.where(solved == 1, np.where(rng.random(n) < 0.5, 1, -1), 0).astype(np.int32)
+    inf, dig, depth = oracle.valuate(g, strat, solved, lab_in)
+    s = gpu.Solver(g, trace=True)
+    s.set_state(strat, solved, lab_in)
+    s.compute_valuation()
+    assert np.array_equal(s.labels(), inf)
+    assert np.array_equal(s.depths(), depth)
+    assert np.array_equal(s.digests(), dig)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_switch_of_given_state(gpu, seed):
+    rng = np.random.default_rng(200 + seed)
+    n = int(rng.integers(5, 3000))
+    g = gpu.gen_random_game(600 + seed, n, int(rng.integers(1, 20)), 1, 5)
+    strat = np.array([g.targets[rng.integers(g.offsets[v], g.offsets[v + 1])] for v in range(n)],
+                     np.int32)
+    strat[(g.owner == 0) & (rng.random(n) < 0.5)] = -1
+    solved = np.zeros(n, np.int32)
+    for player in (0, 1):
+        cnt, exp = oracle.switch(g, strat, solved, np.zeros(n, np.int32), player)
+        s = gpu.Solver(g)
+        s.set_state(strat, solved, None)
+        s.compute_valuation()
+        sw = s.switch_strategy(player)
+        assert sw == (1 if cnt else 0)
+        assert np.array_equal(s.strategy(), exp)
+
+
+def test_stepwise_api_matches_solve(gpu):
+    """Driving init_strat / compute_valuation / mark_solved / switch_strategy
+    from the host (the reference's solve_si loop) gives the same result."""
+    g = gpu.gen_random_game(31337, 5000, 300, 1, 5)
+    r = oracle.solve(g)
+    s = gpu.Solver(g)
+    s.init_strat()
+    br = it = 0
+    while True:
+        while True:
+            s.compute_valuation()
+            br += 1
+            s.mark_solved(1)
+            if s.switch_strategy(1) == 0:
+                break
+        s.mark_solved(0)
+        it += 1
+        if s.switch_strategy(0) == 0:
+            break
+    assert (it, br) == (r.iter, r.br)
+    assert np.array_equal(s.strategy(), r.strat)
+    assert np.array_equal(s.labels(), r.inf)
+
+
+def test_batch_equals_alone(gpu):
+    """Many games of uneven size in one handle: each result equals its solve
+    alone (per-game control blocks)."""
+    sizes = [3, 17, 400, 2500, 10, 9000, 1, 64]
+    games = []
+    for i, n in enumerate(sizes):
+        n = max(n, 2)
+        if i % 3 == 2:
+            games.append(gpu.gen_chain_game(i, n))
+        else:
+            games.append(gpu.gen_random_game(50 + i, n, max(1, n // 3), 1, 4))
+    b = gpu.Solver(games, trace=True)
+    b.solve()
+    for k, g in enumerate(games):
+        r = oracle.solve(g, trace=True)
+        it, br, capped = b.stats(k)
+        assert (it, br) == (r.iter, r.br), k
+        assert np.array_equal(b.strategy(k), r.strat), k
+        assert np.array_equal(b.labels(k), r.inf), k
+        assert np.array_equal(b.trace_hashes(k), r.hashes), k
+
+
+def test_round_cap(gpu):
+    g = gpu.gen_random_game(5, 3000, 100, 1, 4)
+    r = oracle.solve(g, max_rounds=3)
+    assert r.capped
+    s = gpu.Solver(g)
+    s.solve(max_rounds=3)
+    it, br, capped = s.stats()
+    assert capped and br == 3 and it == r.iter
+    assert np.array_equal(s.strategy(), r.strat)
+
+
+def test_invalid_games_rejected(gpu):
+    G = gpu.Game([0, 1], [1, 2], [0, 1, 2], [1, 5])
+    with pytest.raises(gpu.PsiError, match="successor out of range"):
+        gpu.Solver(G)
+    G = gpu.Game([0, 1], [-1, 2], [0, 1, 2], [1, 0])
+    with pytest.raises(gpu.PsiError, match="priority"):
+        gpu.Solver(G)
