@@ -154,6 +154,10 @@ int psi_get_depths(psi_solver* s, int game, int* out);     /* path length to sin
  * the strategy valued, solved flags, labels and digests (4 x uint64). */
 int psi_get_digests(psi_solver* s, int game, uint64_t* out);
 int psi_get_trace(psi_solver* s, int game, uint64_t* hashes, int cap, int* count);
+/* Pointer-jumping statistics of the last valuation: out[2r] vertices that
+ * reached a terminal in round r, out[2r+1] still jumping after it; *last =
+ * last round that ran; *err = 1 if the launched rounds were not enough. */
+int psi_get_jump_stats(psi_solver* s, int* out, int rounds, int* last, int* err);
 /* Loads strategy (-1 sink), solved flags and labels of solved vertices. */
 int psi_set_state(psi_solver* s, int game, const int* strat, const int* solved,
                   const int* labels);

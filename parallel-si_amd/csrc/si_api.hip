@@ -293,3 +293,26 @@ extern "C" int psi_solver_synchronize(psi_solver* s)
     PSI_HIP(hipStreamSynchronize(s->core.s));
     return PSI_OK;
 }
+
+// Jump-round counters of the last valuation: out[2r] = vertices resolved in
+// round r, out[2r+1] = vertices still jumping after round r (r < rounds);
+// *last = last round that ran, *err = 1 if the launched rounds ran out.
+extern "C" int psi_get_jump_stats(psi_solver* s, int* out, int rounds, int* last, int* err)
+{
+    Core& c = s->core;
+    JumpCnt jc;
+    PSI_HIP(hipMemcpyAsync(&jc, c.jc, sizeof jc, hipMemcpyDeviceToHost, c.s));
+    PSI_HIP(hipStreamSynchronize(c.s));
+    for (int r = 0; r < rounds && r < kMaxRounds; r++) {
+        unsigned long long res = 0, act = 0;
+        for (int k = 0; k < kSlots; k++) {
+            res += jc.slot[r][k] >> 32;
+            act += jc.slot[r][k] & 0xFFFFFFFFu;
+        }
+        out[2 * r] = (int)res;
+        out[2 * r + 1] = (int)act;
+    }
+    *last = jc.last;
+    *err = jc.err;
+    return PSI_OK;
+}

@@ -36,7 +36,7 @@ constexpr int kSlots = 64;
 constexpr int kMaxRounds = 40;
 struct JumpCnt {
     unsigned long long slot[kMaxRounds][kSlots]; // (resolved << 32) | still-jumping
-    unsigned long long summary;                  // (stop round + 1) << 32 | cycle-bound count
+    unsigned long long summary[2];               // by round parity: (stop + 1) << 32 | cycle-bound count
     int last;       // last round that ran
     int err;        // 1: ran out of launched rounds
 };

@@ -72,11 +72,13 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, unsigned l
 
 // Phase bookkeeping for round r: stop = first round that resolved nothing
 // (-1 if none yet), cyc = vertices still jumping after it (all lead into
-// cycles).  jc->summary holds the state through round r-2 (written by block 0
-// of round r-1); round r-1 is read from its slots.
+// cycles).  The state through round r-2 comes from the summary written by
+// round r-1; round r-1 itself is read from its slots.  No kernel reads a word
+// that its own blocks write.
 __device__ __forceinline__ void round_state(int r, const JumpCnt* jc, int* stop, int* cyc)
 {
-    unsigned long long sm = *(volatile const unsigned long long*)&jc->summary;
+    // summary[(r-1)&1] was written by block 0 of round r-1 (a previous kernel)
+    unsigned long long sm = r >= 1 ? jc->summary[(r - 1) & 1] : 0ull;
     int st = (int)(sm >> 32) - 1, cy = (int)(sm & 0xFFFFFFFFu);
     if (r >= 1 && st < 0) {
         unsigned long long res = 0, act = 0;
@@ -156,7 +158,7 @@ __global__ void k_val_init(long long N, const int* __restrict__ vinfo, const int
         unsigned long long* sl = &jc->slot[0][0];
         for (int i = threadIdx.x; i < kMaxRounds * kSlots; i += blockDim.x) sl[i] = 0;
         if (threadIdx.x == 0) {
-            jc->summary = 0;
+            jc->summary[0] = jc->summary[1] = 0;
             jc->last = -1;                         // jc->err stays sticky
         }
     }
@@ -201,7 +203,7 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn,
         cnt_res = 0;
         cnt_act = 0;
         if (blockIdx.x == 0) {
-            jc->summary = ((unsigned long long)(stop + 1) << 32) | (unsigned)cyc;
+            jc->summary[r & 1] = ((unsigned long long)(stop + 1) << 32) | (unsigned)cyc;
             if (run) jc->last = r;
         }
     }
@@ -347,7 +349,7 @@ __device__ __forceinline__ int cmp_val(int u, int w, const int* __restrict__ key
     if (u == w) return 0;
     int ku = u < 0 ? (1 << 30) : key[u];
     int kw = w < 0 ? (1 << 30) : key[w];
-    int cu = ku >> 30, cw = kw >> 30;
+    int cu = (int)((unsigned)ku >> 30), cw = (int)((unsigned)kw >> 30);   // +inf keys are negative ints
     if (cu != cw) return cu > cw ? 1 : -1;
     if (cu != 1) return 0;
     int pu = u < 0 ? -1 : (ku & kPriMask);
@@ -626,7 +628,7 @@ int core_init_strat(Core& c)
 {
     k_init_strat<<<GRID(c.N)>>>(c.N, c.vinfo, c.off, c.tgt, c.strat, c.solved, c.lab);
     k_reset_ctl<<<GRID(c.G)>>>(c.G, c.gctl);
-    k_fill<int><<<1, 256, 0, c.s>>>((int*)c.jc, (long long)(sizeof(JumpCnt) / 4), 0);
+    k_fill<int><<<GRID((long long)(sizeof(JumpCnt) / 4))>>>((int*)c.jc, (long long)(sizeof(JumpCnt) / 4), 0);
     if (c.trace) k_fill<unsigned long long><<<GRID((long long)c.G * c.trace_cap * 4)>>>(
         c.tr, (long long)c.G * c.trace_cap * 4, 0ull);
     PSI_LAUNCH_CHECK();

@@ -381,3 +381,12 @@ def solve_profiled(solver, reset=False, max_rounds=0):
     c = np.zeros(8, np.int64)
     check(lib.psi_solve_profiled(solver.h, int(reset), max_rounds, t, c), "psi_solve_profiled")
     return {k: (float(t[i]), int(c[i])) for i, k in enumerate(K_NAMES)}
+
+_sig("psi_get_jump_stats", C.c_int, _vp, _i32p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int))
+
+
+def jump_stats(solver, rounds=40):
+    out = np.zeros(2 * rounds, np.int32)
+    last, err = C.c_int(), C.c_int()
+    check(lib.psi_get_jump_stats(solver.h, out, rounds, C.byref(last), C.byref(err)), "jump_stats")
+    return out.reshape(-1, 2), last.value, err.value
