@@ -291,7 +291,9 @@ __global__ void __launch_bounds__(kBS) k_label(
         int br = live ? gc[g].br : 0;
         bool rec = live && br < trace_cap;
         unsigned long long* base = rec ? tr + ((size_t)g * trace_cap + br) * 4 : nullptr;
-        wave_add_u64(rec ? base + 0 : nullptr, rec ? hmix_i32(loc, strat[v]) : 0);
+        int sv = rec ? strat[v] : -1;
+        if (sv >= 0) sv -= gc[g].vbase;                  // local ids, as a game alone
+        wave_add_u64(rec ? base + 0 : nullptr, rec ? hmix_i32(loc, sv) : 0);
         wave_add_u64(rec ? base + 1 : nullptr, rec ? hmix_i32(loc, sol) : 0);
         wave_add_u64(rec ? base + 2 : nullptr, rec ? hmix_i32(loc, l) : 0);
         wave_add_u64(rec ? base + 3 : nullptr, rec ? hmix_u64(loc, d) : 0);
