@@ -120,6 +120,8 @@ def main():
     ap.add_argument("--cpu-rounds", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-listrank", action="store_true")
+    ap.add_argument("--only", choices=["si", "listrank"], default=None,
+                    help="run one workload only (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -128,17 +130,25 @@ def main():
         sys.exit(2)
     import psi
     psi.init()
+    if a.only == "listrank":
+        print(json.dumps({"listrank": listrank_bench(psi)}), flush=True)
+        return
+    if a.only == "si":
+        a.no_listrank = a.no_cpu = True
     g, s, it, br, wall, prof = si_bench(psi, a.n, a.steps, a.warmup, a.seed)
     value = g.n * br * a.steps / wall
-    # Roofline of the dominant kernel: a pointer-jumping round reads its own
-    # 16-byte record, gathers the successor's record and writes the result.
+    # Roofline of the dominant kernel: a pointer-jumping round reads and
+    # writes every vertex's 16-byte record (next, max, count) and gathers the
+    # 16-byte record of each still-jumping vertex's successor.
     jump_ms, jump_launches = prof["jump"]
-    jump_bytes = 48.0 * g.n
+    gathers = prof["gathers"][1]
+    jump_bytes = (32.0 * g.n * jump_launches + 16.0 * gathers) / max(jump_launches, 1)
     achieved = jump_bytes / (jump_ms / jump_launches * 1e-3) / 1e9 if jump_launches else 0.0
     roof = {"bound": "hbm", "kernel": "k_jump", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None, "alg_bytes_per_launch": jump_bytes,
-            "avg_launch_us": round(jump_ms / max(jump_launches, 1) * 1e3, 2)}
+            "traffic": None, "alg_bytes_per_launch": round(jump_bytes),
+            "avg_launch_us": round(jump_ms / max(jump_launches, 1) * 1e3, 2),
+            "launches": jump_launches}
     line = {
         "metric": "SI vertex-iterations/s on 4M-vertex game",
         "value": round(value, 1),
@@ -156,7 +166,8 @@ def main():
                                "out-degree U[2,8], seed 2",
                    "n": g.n, "m": g.m, "iter": it, "br": br, "parallelism": "single GPU"},
         "roofline": roof,
-        "kernels_ms": {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in prof.items()},
+        "kernels_ms": {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in prof.items()
+                       if k != "gathers"},
     }
     s.close()
     if not a.no_listrank:

@@ -139,9 +139,10 @@ int psi_get_stats(psi_solver* s, int game, int* iter, int* br, int* capped);
 #define PSI_K_SWITCH0 5
 #define PSI_K_CTL 6
 #define PSI_K_TOTAL 7
-#define PSI_K_COUNT 8
+#define PSI_K_GATHERS 8   /* counts only: successor records gathered by jump rounds */
+#define PSI_K_COUNT 9
 int psi_solve_profiled(psi_solver* s, int reset, int max_rounds, double* times_ms,
-                       int64_t* counts);
+                       int64_t* counts);  /* both arrays have PSI_K_COUNT entries */
 
 /* Results of one game, local vertex ids.  get_winning: gpu_parity.cu:380-394. */
 int psi_get_winning(psi_solver* s, int game, int player, int* out, int* len);

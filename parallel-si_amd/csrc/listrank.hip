@@ -250,7 +250,7 @@ __global__ void __launch_bounds__(kTopThreads) top_lds(int R, const int* __restr
     }
     for (int i = tid; i < R; i += kTopThreads) {
         rok[i] = P[i] == -1 ? 1 : 0;
-        roff[i] = S[i];
+        roff[i] = P[i] == -1 ? S[i] : -1;          // -1: unreachable
     }
 }
 
@@ -343,13 +343,9 @@ __global__ void prop_kernel(int R, const int* __restrict__ rid, const int* __res
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     int id = rid[r];
-    if (id < 0 || !rok_up[id]) {
-        rok[r] = 0;
-        roff[r] = 0;
-    } else {
-        rok[r] = 1;
-        roff[r] = roff_up[id] + loc[r];
-    }
+    int o = id < 0 ? -1 : roff_up[id];
+    rok[r] = o >= 0;
+    roff[r] = o >= 0 ? o + loc[r] : -1;
 }
 
 __global__ void final0(int N, int LB, int R1, int* __restrict__ rank, const int* __restrict__ rok1,
@@ -360,8 +356,8 @@ __global__ void final0(int N, int LB, int R1, int* __restrict__ rank, const int*
     int p = rank[x];
     if (p < 0) return;                             // never visited: stays -1
     int id = p >> LB;
-    int loc = p & ((1 << LB) - 1);
-    rank[x] = (id < R1 && rok1[id]) ? roff1[id] + loc : -1;
+    int o = id < R1 ? roff1[id] : -1;             // one gather: -1 marks unreachable
+    rank[x] = o >= 0 ? o + (p & ((1 << LB) - 1)) : -1;
 }
 
 // ---------------------------------------------------------------------------

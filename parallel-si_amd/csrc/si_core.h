@@ -60,7 +60,10 @@ struct Core {
     int* key = nullptr;    // (label+1) << 30 | max priority on the path
     int8_t* solved = nullptr;
     int8_t* lab = nullptr; // infinite[]: -1, 0, +1
-    int4* S[2] = {nullptr, nullptr};
+    int4* Sl = nullptr;    // jump records of every level: level j at Sl + j*N
+    int levels = 0;        // lmax + 1
+    int* own[2] = {nullptr, nullptr};   // vertices owned by each player
+    long long own_n[2] = {0, 0};
     unsigned long long* Z[2] = {nullptr, nullptr};
     unsigned long long* dig = nullptr;
     GameCtl* gctl = nullptr;

@@ -75,12 +75,13 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, unsigned l
 // cycles).  The state through round r-2 comes from the summary written by
 // round r-1; round r-1 itself is read from its slots.  No kernel reads a word
 // that its own blocks write.
-__device__ __forceinline__ void round_state(int r, const JumpCnt* jc, int* stop, int* cyc)
+__device__ __forceinline__ void round_state(int r, const JumpCnt* jc, int* stop, int* cyc, int* halt)
 {
     // summary[(r-1)&1] was written by block 0 of round r-1 (a previous kernel)
     unsigned long long sm = r >= 1 ? jc->summary[(r - 1) & 1] : 0ull;
-    int st = (int)(sm >> 32) - 1, cy = (int)(sm & 0xFFFFFFFFu);
-    if (r >= 1 && st < 0) {
+    *halt = (int)(sm >> 63);
+    int st = (int)((sm >> 32) & 0x7FFFFFFFu) - 1, cy = (int)(sm & 0xFFFFFFFFu);
+    if (r >= 1 && st < 0 && !*halt) {
         unsigned long long res = 0, act = 0;
         for (int k = 0; k < kSlots; k++) {
             unsigned long long x = jc->slot[r - 1][k];
@@ -96,15 +97,16 @@ __device__ __forceinline__ void round_state(int r, const JumpCnt* jc, int* stop,
     *cyc = cy;
 }
 
-__device__ __forceinline__ int decide_round(int r, const JumpCnt* jc, int maxn)
+// Whether round r runs: always while some vertex still reaches a terminal;
+// afterwards until the window 2^r covers the cycle-bound vertices.  Once a
+// round does not run, no later round does (halt).
+__device__ __forceinline__ int decide_round(int r, const JumpCnt* jc, int maxn, int* stop,
+                                            int* cyc, int* halt)
 {
-    if (r == 0) return 1;
-    if (r >= kMaxRounds) return 0;
-    int stop, cyc;
-    round_state(r, jc, &stop, &cyc);
-    if (stop < 0) return 1;
-    cyc = min(cyc, maxn);
-    return (r < 30 && (1 << r) < cyc) ? 1 : 0;
+    round_state(r, jc, stop, cyc, halt);
+    if (*halt || r >= kMaxRounds) return 0;
+    if (r == 0 || *stop < 0) return 1;
+    return (r < 30 && (1 << r) < min(*cyc, maxn)) ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -186,6 +188,8 @@ __global__ void k_val_init(long long N, const int* __restrict__ vinfo, const int
     if (trace) Z0[v] = splitmix64((unsigned long long)(uint32_t)pri_of(vi));
 }
 
+// One pointer-jumping round; grid-stride so that launches after convergence
+// cost a few thousand one-word reads.
 __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn,
                                               const int4* __restrict__ Sin, int4* __restrict__ Sout,
                                               const unsigned long long* __restrict__ Zin,
@@ -195,41 +199,46 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn,
     __shared__ int run_s;
     __shared__ unsigned int cnt_res, cnt_act;
     if (threadIdx.x == 0) {
-        int stop, cyc;
-        round_state(r, jc, &stop, &cyc);
-        int run = r == 0 || stop < 0 || (r < 30 && (1 << r) < min(cyc, maxn));
-        if (r >= kMaxRounds) run = 0;
+        int stop, cyc, halt;
+        int run = decide_round(r, jc, maxn, &stop, &cyc, &halt);
         run_s = run;
         cnt_res = 0;
         cnt_act = 0;
         if (blockIdx.x == 0) {
-            jc->summary[r & 1] = ((unsigned long long)(stop + 1) << 32) | (unsigned)cyc;
+            jc->summary[r & 1] = ((unsigned long long)(run ? 0 : 1) << 63) |
+                                 ((unsigned long long)(stop + 1) << 32) | (unsigned)cyc;
             if (run) jc->last = r;
         }
     }
     __syncthreads();
     if (!run_s) return;
-    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    bool res = false, act = false;
-    if (v < N) {
-        int4 s = Sin[v];
-        if (s.x >= 0) {
-            int4 t = Sin[s.x];
-            if (trace) Zout[v] = Zin[v] + Zin[s.x];
-            s.x = t.x;
-            s.y = max(s.y, t.y);
-            s.z += t.z;
-            res = s.x < 0;
-            act = !res;
-        } else if (trace) {
-            Zout[v] = Zin[v];
+    unsigned my_res = 0, my_act = 0;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < N; base += stride) {
+        long long v = base + threadIdx.x;
+        bool res = false, act = false;
+        if (v < N) {
+            int4 s = Sin[v];
+            if (s.x >= 0) {
+                int4 t = Sin[s.x];
+                if (trace) Zout[v] = Zin[v] + Zin[s.x];
+                s.x = t.x;
+                s.y = max(s.y, t.y);
+                s.z += t.z;
+                res = s.x < 0;
+                act = !res;
+            } else if (trace) {
+                Zout[v] = Zin[v];
+            }
+            Sout[v] = s;
         }
-        Sout[v] = s;
+        unsigned long long mr = __ballot(res), ma = __ballot(act);
+        my_res += (unsigned)__popcll(mr);
+        my_act += (unsigned)__popcll(ma);
     }
-    unsigned long long mr = __ballot(res), ma = __ballot(act);
-    if ((threadIdx.x & (kWave - 1)) == 0 && (mr | ma)) {
-        atomicAdd(&cnt_res, (unsigned)__popcll(mr));
-        atomicAdd(&cnt_act, (unsigned)__popcll(ma));
+    if ((threadIdx.x & (kWave - 1)) == 0 && (my_res | my_act)) {
+        atomicAdd(&cnt_res, my_res);
+        atomicAdd(&cnt_act, my_act);
     }
     __syncthreads();
     if (threadIdx.x == 0 && (cnt_res | cnt_act))
@@ -241,7 +250,7 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn,
 // also mark_solved(1) (cpu_parity.cpp:306-315), which solve_si calls right
 // after every valuation.
 __global__ void __launch_bounds__(kBS) k_label(
-    long long N, int maxn, int lmax, const int4* __restrict__ S0, const int4* __restrict__ S1,
+    long long N, int maxn, int lmax, const int4* __restrict__ Sl,
     const unsigned long long* __restrict__ Z0, const unsigned long long* __restrict__ Z1,
     const int* __restrict__ gid, const GameCtl* __restrict__ gc, const int* __restrict__ strat,
     int8_t* __restrict__ solved, int8_t* __restrict__ lab, int* __restrict__ dep,
@@ -250,9 +259,12 @@ __global__ void __launch_bounds__(kBS) k_label(
 {
     const int last = jc->last;
     const int F = last + 1;
-    const int4* SF = (F & 1) ? S1 : S0;
+    const int4* SF = Sl + (size_t)F * N;
     const unsigned long long* ZF = (F & 1) ? Z1 : Z0;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && decide_round(lmax, jc, maxn)) jc->err = 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        int stop, cyc, halt;
+        if (decide_round(lmax, jc, maxn, &stop, &cyc, &halt)) jc->err = 1;
+    }
     long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int g = -1, l = 0, sol = 0, loc = 0;
     unsigned long long d = 0;
@@ -300,15 +312,56 @@ __global__ void __launch_bounds__(kBS) k_label(
     }
 }
 
-// Exact compare of two finite vertices whose path maxima are equal.
+// Exact compare of two finite vertices whose path maxima are equal.  The jump
+// records of every level are binary-lifting tables (level j: 2^j-th
+// successor and the largest priority over those 2^j vertices), so the lowest
+// common ancestor and the largest priority of each private segment take
+// O(log depth) gathers.  Only when the two segment maxima are equal too does
+// the exact scan walk the segments level by level.
 __device__ int cmp_exact(int u, int w, const int* __restrict__ par, const int* __restrict__ dep,
-                         const int* __restrict__ vinfo)
+                         const int* __restrict__ vinfo, const int4* __restrict__ Sl, long long N,
+                         int F)
 {
     int a = u, b = w, da = dep[u], db = dep[w];
-    while (da > db) { a = par[a]; da--; }
-    while (db > da) { b = par[b]; db--; }
-    while (a != b) { a = par[a]; b = par[b]; da--; }
-    const int la = dep[u] - da, lb = dep[w] - da;
+    int mA = -1, mB = -1;
+    for (int j = F; j >= 0; j--) {
+        if (da - db >= (1 << j)) {
+            int4 t = Sl[(size_t)j * N + a];
+            mA = max(mA, t.y);
+            a = t.x;
+            da -= 1 << j;
+        }
+        if (db - da >= (1 << j)) {
+            int4 t = Sl[(size_t)j * N + b];
+            mB = max(mB, t.y);
+            b = t.x;
+            db -= 1 << j;
+        }
+    }
+    int dl;                                           // depth of the common ancestor
+    if (a == b) {
+        dl = da;
+    } else {
+        for (int j = F; j >= 0; j--) {
+            int4 ta = Sl[(size_t)j * N + a], tb = Sl[(size_t)j * N + b];
+            if (ta.x != tb.x) {
+                mA = max(mA, ta.y);
+                mB = max(mB, tb.y);
+                a = ta.x;
+                b = tb.x;
+                da -= 1 << j;
+            }
+        }
+        mA = max(mA, pri_of(vinfo[a]));
+        mB = max(mB, pri_of(vinfo[b]));
+        dl = da - 1;
+    }
+    if (mA != mB) {
+        if (mA > mB) return (mA & 1) ? -1 : 1;
+        return (mB & 1) ? 1 : -1;
+    }
+    // Equal segment maxima: compare the two multisets from the top down.
+    const int la = dep[u] - dl, lb = dep[w] - dl;
     int thr = 0x7FFFFFFF;
     for (;;) {
         int ma = -1, ca = 0, mb = -1, cb = 0;
@@ -346,7 +399,8 @@ __device__ int cmp_exact(int u, int w, const int* __restrict__ par, const int* _
 // Sign of compare_valuation(u, w) (cpu_parity.cpp:235-277); -1 is the sink.
 __device__ __forceinline__ int cmp_val(int u, int w, const int* __restrict__ key,
                                        const int* __restrict__ par, const int* __restrict__ dep,
-                                       const int* __restrict__ vinfo)
+                                       const int* __restrict__ vinfo, const int4* __restrict__ Sl,
+                                       long long N, int F)
 {
     if (u == w) return 0;
     int ku = u < 0 ? (1 << 30) : key[u];
@@ -360,7 +414,7 @@ __device__ __forceinline__ int cmp_val(int u, int w, const int* __restrict__ key
         if (pu > pw) return (pu & 1) ? -1 : 1;
         return (pw & 1) ? 1 : -1;
     }
-    return cmp_exact(u, w, par, dep, vinfo);
+    return cmp_exact(u, w, par, dep, vinfo, Sl, N, F);
 }
 
 // switch_strategy(player) (cpu_parity.cpp:280-304, gpu_parity.cu:304-357):
@@ -368,34 +422,34 @@ __device__ __forceinline__ int cmp_val(int u, int w, const int* __restrict__ key
 // strictly-better-than-current edge in CSR order wins each step of the scan.
 // Loop form with player 0 also applies mark_solved(0) first.
 __global__ void __launch_bounds__(kBS) k_switch(
-    int player, long long N, const int* __restrict__ vinfo, const int* __restrict__ off,
+    int player, long long N, const int* __restrict__ own, long long own_n,
+    const int* __restrict__ vinfo, const int* __restrict__ off,
     const int* __restrict__ tgt, const int* __restrict__ gid, GameCtl* __restrict__ gc,
     int* __restrict__ strat, int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
     const int* __restrict__ key, const int* __restrict__ dep, const int* __restrict__ par,
-    int loop_form, int* __restrict__ flag)
+    const int4* __restrict__ Sl, const JumpCnt* __restrict__ jc, int loop_form,
+    int* __restrict__ flag)
 {
-    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int F = jc->last + 1;                   // levels of the last valuation
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     bool sw = false;
     int g = -1;
-    if (v < N) {
+    if (i < own_n) {
+        int v = own[i];
         g = gid[v];
         bool live = !gc[g].done && (!loop_form || player == 1 || gc[g].outer);
         if (live) {
-            if (loop_form && player == 0 && lab[v] == 1) solved[v] = 1;
-            int vi = vinfo[v];
-            if (owner_of(vi) == player) {
-                int cur = strat[v];
-                int e1 = off[v + 1];
-                for (int e = off[v]; e < e1; e++) {
-                    int u = tgt[e];
-                    int c = cmp_val(u, cur, key, par, dep, vinfo);
-                    if (player == 0 ? c > 0 : c < 0) {
-                        cur = u;
-                        sw = true;
-                    }
+            int cur = strat[v];
+            int e1 = off[v + 1];
+            for (int e = off[v]; e < e1; e++) {
+                int u = tgt[e];
+                int c = cmp_val(u, cur, key, par, dep, vinfo, Sl, N, F);
+                if (player == 0 ? c > 0 : c < 0) {
+                    cur = u;
+                    sw = true;
                 }
-                if (sw) strat[v] = cur;
             }
+            if (sw) strat[v] = cur;
         }
     }
     // one store per wave and game: the flag only has to become nonzero
@@ -419,6 +473,17 @@ __global__ void __launch_bounds__(kBS) k_switch(
             }
         }
     }
+}
+
+// mark_solved(0) of the games that take the outer step (fused form of
+// cpu_parity.cpp:306-315 as solve_si calls it, gpumain.cpp:44).
+__global__ void k_mark0_outer(long long N, const int* __restrict__ gid, const GameCtl* __restrict__ gc,
+                              const int8_t* __restrict__ lab, int8_t* __restrict__ solved)
+{
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    int g = gid[v];
+    if (!gc[g].done && gc[g].outer && lab[v] == 1) solved[v] = 1;
 }
 
 // mark_solved(player) as a separate step (cpu_parity.cpp:306-315).
@@ -546,11 +611,12 @@ int core_create(Core& c, int G, const int* n_per, const long long* m_per, const 
     c.trace_cap = trace ? std::max(trace_cap, 1) : 0;
     c.max_pri = max_pri;
     c.lmax = ceil_log2(maxn) + 2;
+    c.levels = c.lmax + 1;
     c.h_n.assign(n_per, n_per + G);
     c.h_m.assign(m_per, m_per + G);
     c.h_vbase.resize(G);
     // Global CSR with global target ids.
-    std::vector<int> vinfo(N), off(N + 1), tgt(M), gid(N);
+    std::vector<int> vinfo(N), off(N + 1), tgt(M), gid(N), own0, own1;
     std::vector<GameCtl> gc(G);
     long long vb = 0, eb = 0;
     for (int g = 0; g < G; g++) {
@@ -558,6 +624,7 @@ int core_create(Core& c, int G, const int* n_per, const long long* m_per, const 
         const int* o = offsets + vb + g;
         for (int v = 0; v < n; v++) {
             vinfo[vb + v] = priority[vb + v] | (owner[vb + v] << 30);
+            (owner[vb + v] ? own1 : own0).push_back((int)(vb + v));
             off[vb + v] = (int)(eb + o[v]);
             gid[vb + v] = g;
         }
@@ -580,8 +647,13 @@ int core_create(Core& c, int G, const int* n_per, const long long* m_per, const 
     if (int rc = dalloc(&c.key, N)) return rc;
     if (int rc = dalloc(&c.solved, N)) return rc;
     if (int rc = dalloc(&c.lab, N)) return rc;
-    if (int rc = dalloc(&c.S[0], N)) return rc;
-    if (int rc = dalloc(&c.S[1], N)) return rc;
+    if (int rc = dalloc(&c.Sl, N * c.levels)) return rc;
+    c.own_n[0] = (long long)own0.size();
+    c.own_n[1] = (long long)own1.size();
+    if (int rc = dalloc(&c.own[0], c.own_n[0])) return rc;
+    if (int rc = dalloc(&c.own[1], c.own_n[1])) return rc;
+    if (c.own_n[0]) PSI_HIP(hipMemcpyAsync(c.own[0], own0.data(), c.own_n[0] * 4, hipMemcpyHostToDevice, s));
+    if (c.own_n[1]) PSI_HIP(hipMemcpyAsync(c.own[1], own1.data(), c.own_n[1] * 4, hipMemcpyHostToDevice, s));
     if (trace) {
         if (int rc = dalloc(&c.Z[0], N)) return rc;
         if (int rc = dalloc(&c.Z[1], N)) return rc;
@@ -604,13 +676,50 @@ void core_destroy(Core& c)
 {
     if (c.graph) (void)hipGraphExecDestroy(c.graph);
     void* ptrs[] = {c.vinfo, c.off, c.tgt, c.gid, c.strat, c.par, c.dep, c.key, c.solved, c.lab,
-                    c.S[0], c.S[1], c.Z[0], c.Z[1], c.dig, c.tr, c.gctl, c.jc, c.flag};
+                    c.Sl, c.own[0], c.own[1], c.Z[0], c.Z[1], c.dig, c.tr, c.gctl, c.jc, c.flag};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c = Core();
 }
 
 #define GRID(n) nblocks((n), kBS), kBS, 0, c.s
+// Jump rounds: capped grid (grid-stride inside) so idle launches stay cheap.
+#define JGRID(n) std::min(nblocks((n), kBS), 4096u), kBS, 0, c.s
+
+static int maxn_of(const Core& c)
+{
+    int m = 0;
+    for (int n : c.h_n) m = std::max(m, n);
+    return m;
+}
+
+static void launch_val_init(Core& c)
+{
+    k_val_init<<<GRID(c.N)>>>(c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
+                              c.par, c.Sl, c.Z[0], c.jc, c.trace);
+}
+
+static void launch_jump(Core& c, int r, int maxn)
+{
+    k_jump<<<JGRID(c.N)>>>(r, c.N, maxn, c.Sl + (size_t)r * c.N, c.Sl + (size_t)(r + 1) * c.N,
+                           c.Z[r & 1], c.Z[(r + 1) & 1], c.jc, c.trace);
+}
+
+static void launch_label(Core& c, int maxn, bool loop_form)
+{
+    k_label<<<GRID(c.N)>>>(c.N, maxn, c.lmax, c.Sl, c.Z[0], c.Z[1], c.gid, c.gctl, c.strat,
+                           c.solved, c.lab, c.dep, c.key, c.dig, c.tr, c.trace_cap, loop_form,
+                           c.trace, c.jc);
+}
+
+static void launch_switch(Core& c, int player, bool loop_form)
+{
+    if (loop_form && player == 0) k_mark0_outer<<<GRID(c.N)>>>(c.N, c.gid, c.gctl, c.lab, c.solved);
+    if (c.own_n[player] == 0) return;
+    k_switch<<<GRID(c.own_n[player])>>>(player, c.N, c.own[player], c.own_n[player], c.vinfo,
+                                        c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
+                                        c.key, c.dep, c.par, c.Sl, c.jc, loop_form, c.flag);
+}
 
 int core_set_params(Core& c, int reset, int max_rounds)
 {
@@ -639,16 +748,10 @@ int core_init_strat(Core& c)
 
 int core_valuation(Core& c, bool loop_form)
 {
-    int maxn = 0;
-    for (int n : c.h_n) maxn = std::max(maxn, n);
-    k_val_init<<<GRID(c.N)>>>(c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
-                              c.par, c.S[0], c.Z[0], c.jc, c.trace);
-    for (int r = 0; r < c.lmax; r++)
-        k_jump<<<GRID(c.N)>>>(r, c.N, maxn, c.S[r & 1], c.S[(r + 1) & 1], c.Z[r & 1],
-                              c.Z[(r + 1) & 1], c.jc, c.trace);
-    k_label<<<GRID(c.N)>>>(c.N, maxn, c.lmax, c.S[0], c.S[1], c.Z[0], c.Z[1], c.gid, c.gctl,
-                           c.strat, c.solved, c.lab, c.dep, c.key, c.dig, c.tr, c.trace_cap,
-                           loop_form, c.trace, c.jc);
+    int maxn = maxn_of(c);
+    launch_val_init(c);
+    for (int r = 0; r < c.lmax; r++) launch_jump(c, r, maxn);
+    launch_label(c, maxn, loop_form);
     PSI_LAUNCH_CHECK();
     return PSI_OK;
 }
@@ -670,8 +773,7 @@ int core_mark_solved(Core& c, int player)
 int core_switch(Core& c, int player, int* switched_host)
 {
     k_fill<int><<<1, 64, 0, c.s>>>(c.flag, 1, 0);
-    k_switch<<<GRID(c.N)>>>(player, c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved,
-                            c.lab, c.key, c.dep, c.par, 0, c.flag);
+    launch_switch(c, player, false);
     PSI_LAUNCH_CHECK();
     int h = 0;
     PSI_HIP(hipMemcpyAsync(&h, c.flag, sizeof(int), hipMemcpyDeviceToHost, c.s));
@@ -684,11 +786,9 @@ int core_enqueue_steps(Core& c, int steps)
 {
     for (int k = 0; k < steps; k++) {
         if (int rc = core_valuation(c, true)) return rc;
-        k_switch<<<GRID(c.N)>>>(1, c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved,
-                                c.lab, c.key, c.dep, c.par, 1, c.flag);
+        launch_switch(c, 1, true);
         k_ctl1<<<GRID(c.G)>>>(c.G, c.gctl);
-        k_switch<<<GRID(c.N)>>>(0, c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved,
-                                c.lab, c.key, c.dep, c.par, 1, c.flag);
+        launch_switch(c, 0, true);
         k_ctl0<<<GRID(c.G)>>>(c.G, c.gctl);
         PSI_LAUNCH_CHECK();
     }
@@ -698,15 +798,14 @@ int core_enqueue_steps(Core& c, int steps)
 // The same loop launched kernel by kernel with HIP events between launches, to
 // attribute device time per kernel.  times[k], counts[k] for k in
 // PSI_K_VAL_INIT..PSI_K_CTL; a jump launch that found nothing to do is counted
-// under PSI_K_JUMP_IDLE.
+// under PSI_K_JUMP_IDLE; counts[PSI_K_GATHERS] = successor records gathered.
 int core_solve_profiled(Core& c, double* times, long long* counts)
 {
     for (int k = 0; k < PSI_K_COUNT; k++) {
         times[k] = 0;
         counts[k] = 0;
     }
-    int maxn = 0;
-    for (int n : c.h_n) maxn = std::max(maxn, n);
+    int maxn = maxn_of(c);
     const int nev = c.lmax + 8;
     std::vector<hipEvent_t> ev(nev);
     for (auto& e : ev) PSI_HIP(hipEventCreate(&e));
@@ -719,25 +818,19 @@ int core_solve_profiled(Core& c, double* times, long long* counts)
     for (int step = 0; step < (1 << 20); step++) {
         int k = 0;
         PSI_HIP(hipEventRecord(ev[k++], c.s));
-        k_val_init<<<GRID(c.N)>>>(c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved,
-                                  c.lab, c.par, c.S[0], c.Z[0], c.jc, c.trace);
+        launch_val_init(c);
         PSI_HIP(hipEventRecord(ev[k++], c.s));
         for (int r = 0; r < c.lmax; r++) {
-            k_jump<<<GRID(c.N)>>>(r, c.N, maxn, c.S[r & 1], c.S[(r + 1) & 1], c.Z[r & 1],
-                                  c.Z[(r + 1) & 1], c.jc, c.trace);
+            launch_jump(c, r, maxn);
             PSI_HIP(hipEventRecord(ev[k++], c.s));
         }
-        k_label<<<GRID(c.N)>>>(c.N, maxn, c.lmax, c.S[0], c.S[1], c.Z[0], c.Z[1], c.gid, c.gctl,
-                               c.strat, c.solved, c.lab, c.dep, c.key, c.dig, c.tr, c.trace_cap, 1,
-                               c.trace, c.jc);
+        launch_label(c, maxn, true);
         PSI_HIP(hipEventRecord(ev[k++], c.s));
-        k_switch<<<GRID(c.N)>>>(1, c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved,
-                                c.lab, c.key, c.dep, c.par, 1, c.flag);
+        launch_switch(c, 1, true);
         PSI_HIP(hipEventRecord(ev[k++], c.s));
         k_ctl1<<<GRID(c.G)>>>(c.G, c.gctl);
         PSI_HIP(hipEventRecord(ev[k++], c.s));
-        k_switch<<<GRID(c.N)>>>(0, c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved,
-                                c.lab, c.key, c.dep, c.par, 1, c.flag);
+        launch_switch(c, 0, true);
         PSI_HIP(hipEventRecord(ev[k++], c.s));
         k_ctl0<<<GRID(c.G)>>>(c.G, c.gctl);
         PSI_HIP(hipEventRecord(ev[k++], c.s));
@@ -749,10 +842,19 @@ int core_solve_profiled(Core& c, double* times, long long* counts)
         times[PSI_K_VAL_INIT] += span(e, e + 1);
         counts[PSI_K_VAL_INIT]++;
         e++;
+        unsigned long long prev_act = 0;
         for (int r = 0; r < c.lmax; r++, e++) {
             int cls = r <= jc.last ? PSI_K_JUMP : PSI_K_JUMP_IDLE;
             times[cls] += span(e, e + 1);
             counts[cls]++;
+            unsigned long long res = 0, act = 0;
+            if (r < kMaxRounds)
+                for (int q = 0; q < kSlots; q++) {
+                    res += jc.slot[r][q] >> 32;
+                    act += jc.slot[r][q] & 0xFFFFFFFFu;
+                }
+            if (r <= jc.last) counts[PSI_K_GATHERS] += (long long)(r == 0 ? res + act : prev_act);
+            prev_act = act;
         }
         times[PSI_K_LABEL] += span(e, e + 1); counts[PSI_K_LABEL]++; e++;
         times[PSI_K_SWITCH1] += span(e, e + 1); counts[PSI_K_SWITCH1]++; e++;
