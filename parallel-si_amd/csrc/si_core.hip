@@ -318,7 +318,7 @@ __global__ void __launch_bounds__(kBS) k_label(
 // common ancestor and the largest priority of each private segment take
 // O(log depth) gathers.  Only when the two segment maxima are equal too does
 // the exact scan walk the segments level by level.
-__device__ int cmp_exact(int u, int w, const int* __restrict__ par, const int* __restrict__ dep,
+__device__ __noinline__ int cmp_exact(int u, int w, const int* __restrict__ par, const int* __restrict__ dep,
                          const int* __restrict__ vinfo, const int4* __restrict__ Sl, long long N,
                          int F)
 {
@@ -396,15 +396,13 @@ __device__ int cmp_exact(int u, int w, const int* __restrict__ par, const int* _
     }
 }
 
-// Sign of compare_valuation(u, w) (cpu_parity.cpp:235-277); -1 is the sink.
-__device__ __forceinline__ int cmp_val(int u, int w, const int* __restrict__ key,
-                                       const int* __restrict__ par, const int* __restrict__ dep,
-                                       const int* __restrict__ vinfo, const int4* __restrict__ Sl,
-                                       long long N, int F)
+// Sign of compare_valuation(u, w) (cpu_parity.cpp:235-277) from the two
+// keys; -1 is the sink (finite, empty path).
+__device__ __forceinline__ int cmp_keys(int u, int ku, int w, int kw, const int* __restrict__ par,
+                                        const int* __restrict__ dep, const int* __restrict__ vinfo,
+                                        const int4* __restrict__ Sl, long long N, int F)
 {
     if (u == w) return 0;
-    int ku = u < 0 ? (1 << 30) : key[u];
-    int kw = w < 0 ? (1 << 30) : key[w];
     int cu = (int)((unsigned)ku >> 30), cw = (int)((unsigned)kw >> 30);   // +inf keys are negative ints
     if (cu != cw) return cu > cw ? 1 : -1;
     if (cu != 1) return 0;
@@ -415,6 +413,11 @@ __device__ __forceinline__ int cmp_val(int u, int w, const int* __restrict__ key
         return (pw & 1) ? 1 : -1;
     }
     return cmp_exact(u, w, par, dep, vinfo, Sl, N, F);
+}
+
+__device__ __forceinline__ int key_of(const int* __restrict__ key, int u)
+{
+    return u < 0 ? (1 << 30) : key[u];
 }
 
 // switch_strategy(player) (cpu_parity.cpp:280-304, gpu_parity.cu:304-357):
@@ -440,13 +443,28 @@ __global__ void __launch_bounds__(kBS) k_switch(
         bool live = !gc[g].done && (!loop_form || player == 1 || gc[g].outer);
         if (live) {
             int cur = strat[v];
-            int e1 = off[v + 1];
-            for (int e = off[v]; e < e1; e++) {
-                int u = tgt[e];
-                int c = cmp_val(u, cur, key, par, dep, vinfo, Sl, N, F);
-                if (player == 0 ? c > 0 : c < 0) {
-                    cur = u;
-                    sw = true;
+            int kcur = key_of(key, cur);
+            const int e1 = off[v + 1];
+            // Successors in batches of 8: the target and key loads of a batch are
+            // independent, so they are in flight together; the scan itself is
+            // sequential in CSR order (the reference's tie-break).
+            for (int e0 = off[v]; e0 < e1; e0 += 8) {
+                const int cnt = min(8, e1 - e0);
+                int us[8], ks[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) us[k] = k < cnt ? tgt[e0 + k] : -1;
+#pragma unroll
+                for (int k = 0; k < 8; k++) ks[k] = k < cnt ? key[us[k]] : 0;
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    if (k < cnt) {
+                        int c = cmp_keys(us[k], ks[k], cur, kcur, par, dep, vinfo, Sl, N, F);
+                        if (player == 0 ? c > 0 : c < 0) {
+                            cur = us[k];
+                            kcur = ks[k];
+                            sw = true;
+                        }
+                    }
                 }
             }
             if (sw) strat[v] = cur;
