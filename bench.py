@@ -100,6 +100,63 @@ def listrank_bench(psi, reps=10):
     return out
 
 
+def gen_games_parallel(psi, specs, threads=16):
+    """specs: list of (kind, seed, n, pri_bound, deg_lo, deg_hi); generated on a
+    thread pool (the generator releases the GIL inside the C call)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(sp):
+        kind, seed, n, pb, dlo, dhi = sp
+        if kind == "chain":
+            return psi.gen_chain_game(seed, n)
+        return psi.gen_random_game(seed, n, pb, dlo, dhi)
+
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(one, specs))
+
+
+def batch_bench(psi, n_games=1024, seed=3, lo=1000, hi=1000000, check_alone=True):
+    """Configuration 3: n_games random games, sizes log-uniform in [lo, hi],
+    priorities U[0, n), out-degree U[1, 6], per-game seed = seed * 100000 + g.
+    games/s = games / wall time including CSR upload and result download,
+    excluding generation.  One-at-a-time = the plain solver per game in turn."""
+    sizes = psi.gen_sizes(seed, n_games, lo, hi)
+    t0 = time.time()
+    games = gen_games_parallel(psi, [("random", seed * 100000 + g, int(n), int(n), 1, 6)
+                                     for g, n in enumerate(sizes)])
+    log(f"[bench] batch: generated {n_games} games, sum n={int(sizes.sum())} in "
+        f"{time.time() - t0:.1f}s")
+    # batched: one handle, every game with its own convergence
+    t = time.perf_counter()
+    b = psi.Solver(games)
+    b.solve(steps_per_sync=4)
+    res_b = [(b.stats(k), b.strategy(k), b.labels(k)) for k in range(n_games)]
+    b.close()
+    t_batch = time.perf_counter() - t
+    log(f"[bench] batch: batched solve {t_batch:.2f}s")
+    # one at a time
+    t = time.perf_counter()
+    res_a = []
+    for g in games:
+        s = psi.Solver(g)
+        s.solve(steps_per_sync=4)
+        res_a.append((s.stats(), s.strategy(), s.labels()))
+        s.close()
+    t_alone = time.perf_counter() - t
+    log(f"[bench] batch: one-at-a-time {t_alone:.2f}s")
+    same = all(x[0] == y[0] and np.array_equal(x[1], y[1]) and np.array_equal(x[2], y[2])
+               for x, y in zip(res_b, res_a))
+    n_tot = int(sizes.sum())
+    br = [x[0][1] for x in res_b]
+    return {"games": n_games, "sum_n": n_tot, "sum_m": int(sum(g.m for g in games)),
+            "batched_s": round(t_batch, 3), "one_at_a_time_s": round(t_alone, 3),
+            "games_per_s_batched": round(n_games / t_batch, 1),
+            "games_per_s_one_at_a_time": round(n_games / t_alone, 1),
+            "speedup": round(t_alone / t_batch, 2), "br_max": int(max(br)),
+            "vertex_iterations": int(sum(int(sz) * b_ for sz, b_ in zip(sizes, br))),
+            "batched_equals_alone": bool(same)}
+
+
 def cpu_baseline(g, max_rounds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -120,7 +177,9 @@ def main():
     ap.add_argument("--cpu-rounds", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-listrank", action="store_true")
-    ap.add_argument("--only", choices=["si", "listrank"], default=None,
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-games", type=int, default=1024)
+    ap.add_argument("--only", choices=["si", "listrank", "batch"], default=None,
                     help="run one workload only (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -133,8 +192,11 @@ def main():
     if a.only == "listrank":
         print(json.dumps({"listrank": listrank_bench(psi)}), flush=True)
         return
+    if a.only == "batch":
+        print(json.dumps({"batch": batch_bench(psi, a.batch_games)}), flush=True)
+        return
     if a.only == "si":
-        a.no_listrank = a.no_cpu = True
+        a.no_listrank = a.no_cpu = a.no_batch = True
     g, s, it, br, wall, prof = si_bench(psi, a.n, a.steps, a.warmup, a.seed)
     value = g.n * br * a.steps / wall
     # Roofline of the dominant kernel: a pointer-jumping round reads and
@@ -172,6 +234,8 @@ def main():
     s.close()
     if not a.no_listrank:
         line["listrank"] = listrank_bench(psi)
+    if not a.no_batch:
+        line["batch"] = batch_bench(psi, a.batch_games)
     if not a.no_cpu:
         log("[bench] cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(g, a.cpu_rounds)

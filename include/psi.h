@@ -106,6 +106,11 @@ int psi_solver_create(const psi_game* g, int flags, int trace_cap, psi_solver** 
 int psi_solver_create_batch(int n_games, const int* n_per, const int64_t* m_per, const int* owner,
                             const int* priority, const int* offsets, const int* targets, int flags,
                             int trace_cap, psi_solver** out);
+/* The same with each game in its own arrays (no concatenation by the caller). */
+int psi_solver_create_games(int n_games, const int* n_per, const int64_t* m_per,
+                            const int* const* owner, const int* const* priority,
+                            const int* const* offsets, const int* const* targets, int flags,
+                            int trace_cap, psi_solver** out);
 int psi_solver_destroy(psi_solver* s);
 int psi_solver_info(psi_solver* s, int* n_games, int64_t* n_total, int64_t* m_total);
 int psi_solver_synchronize(psi_solver* s);

@@ -62,30 +62,50 @@ extern "C" int psi_game_info(const psi_game* g, int* n, int* m, int* max_pri)
     return PSI_OK;
 }
 
-extern "C" int psi_solver_create_batch(int n_games, const int* n_per, const int64_t* m_per,
-                                       const int* owner, const int* priority, const int* offsets,
-                                       const int* targets, int flags, int trace_cap,
-                                       psi_solver** out)
+static int create_from_views(int n_games, const GameView* views, int flags, int trace_cap,
+                             psi_solver** out)
 {
     *out = nullptr;
+    if (n_games <= 0) return fail(PSI_ERR_INVALID, "no games");
     if (int rc = psi_init()) return rc;
     auto* s = new psi_solver;
     s->trace = (flags & PSI_SOLVER_TRACE) != 0;
-    std::vector<long long> m(m_per, m_per + n_games);
-    int rc = core_create(s->core, n_games, n_per, m.data(), owner, priority, offsets, targets,
-                         s->trace, trace_cap, stream());
+    int rc = core_create(s->core, n_games, views, s->trace, trace_cap, stream());
+    if (!rc) rc = core_init_strat(s->core);
     if (rc) {
-        core_destroy(s->core);
-        delete s;
-        return rc;
-    }
-    if ((rc = core_init_strat(s->core))) {
         core_destroy(s->core);
         delete s;
         return rc;
     }
     *out = s;
     return PSI_OK;
+}
+
+extern "C" int psi_solver_create_batch(int n_games, const int* n_per, const int64_t* m_per,
+                                       const int* owner, const int* priority, const int* offsets,
+                                       const int* targets, int flags, int trace_cap,
+                                       psi_solver** out)
+{
+    std::vector<GameView> v(std::max(n_games, 0));
+    long long vb = 0, eb = 0;
+    for (int g = 0; g < n_games; g++) {
+        v[g] = GameView{n_per[g], (long long)m_per[g], owner + vb, priority + vb, offsets + vb + g,
+                        targets + eb};
+        vb += n_per[g];
+        eb += m_per[g];
+    }
+    return create_from_views(n_games, v.data(), flags, trace_cap, out);
+}
+
+extern "C" int psi_solver_create_games(int n_games, const int* n_per, const int64_t* m_per,
+                                       const int* const* owner, const int* const* priority,
+                                       const int* const* offsets, const int* const* targets,
+                                       int flags, int trace_cap, psi_solver** out)
+{
+    std::vector<GameView> v(std::max(n_games, 0));
+    for (int g = 0; g < n_games; g++)
+        v[g] = GameView{n_per[g], (long long)m_per[g], owner[g], priority[g], offsets[g], targets[g]};
+    return create_from_views(n_games, v.data(), flags, trace_cap, out);
 }
 
 extern "C" int psi_solver_create(const psi_game* g, int flags, int trace_cap, psi_solver** out)

@@ -60,6 +60,7 @@ struct Core {
     int* key = nullptr;    // (label+1) << 30 | max priority on the path
     int8_t* solved = nullptr;
     int8_t* lab = nullptr; // infinite[]: -1, 0, +1
+    uint8_t* blive = nullptr; // per 256-vertex block: holds a vertex of an unfinished game
     int4* Sl = nullptr;    // jump records of every level: level j at Sl + j*N
     int levels = 0;        // lmax + 1
     int* own[2] = {nullptr, nullptr};   // vertices owned by each player
@@ -78,11 +79,15 @@ struct Core {
     int graph_steps = 0;
 };
 
-// Builds a core from host CSR arrays of G games laid out back to back
-// (local vertex ids per game).  Validates the input.
-int core_create(Core& c, int G, const int* n_per, const long long* m_per, const int* owner,
-                const int* priority, const int* offsets, const int* targets, bool trace,
-                int trace_cap, hipStream_t s);
+// One game's host CSR arrays (local vertex ids).
+struct GameView {
+    int n;
+    long long m;
+    const int *owner, *pri, *off, *tgt;
+};
+
+// Builds a core from G games (validated; packed on host threads).
+int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap, hipStream_t s);
 void core_destroy(Core& c);
 int core_init_strat(Core& c);
 int core_set_params(Core& c, int reset, int max_rounds);

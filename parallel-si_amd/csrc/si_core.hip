@@ -37,7 +37,9 @@
 #include "si_core.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <thread>
 
 namespace psi {
 
@@ -149,12 +151,12 @@ __global__ void k_reset_opp(long long N, const int* __restrict__ vinfo, const in
 
 // Level-0 jump records.  S = (next or terminal code, max priority, count, -).
 // Terminal codes: -1 sink, -2 solved +inf, -3 solved -inf, -4 finished game.
-__global__ void k_val_init(long long N, const int* __restrict__ vinfo, const int* __restrict__ off,
-                           const int* __restrict__ tgt, const int* __restrict__ gid,
-                           const GameCtl* __restrict__ gc, int* __restrict__ strat,
-                           const int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
-                           int* __restrict__ par, int4* __restrict__ S0,
-                           unsigned long long* __restrict__ Z0, JumpCnt* __restrict__ jc, int trace)
+__global__ void __launch_bounds__(kBS) k_val_init(
+    long long N, const int* __restrict__ vinfo, const int* __restrict__ off,
+    const int* __restrict__ tgt, const int* __restrict__ gid, const GameCtl* __restrict__ gc,
+    int* __restrict__ strat, const int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
+    int* __restrict__ par, int4* __restrict__ S0, unsigned long long* __restrict__ Z0,
+    JumpCnt* __restrict__ jc, uint8_t* __restrict__ blive, int trace)
 {
     if (blockIdx.x == 0) {
         unsigned long long* sl = &jc->slot[0][0];
@@ -165,27 +167,32 @@ __global__ void k_val_init(long long N, const int* __restrict__ vinfo, const int
         }
     }
     long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= N) return;
-    int g = gid[v];
-    if (gc[g].done) {
-        S0[v] = make_int4(-4, -1, 0, 0);
-        return;
+    int live = 0;
+    if (v < N) {
+        int g = gid[v];
+        live = !gc[g].done;
+        if (live) {
+            int vi = vinfo[v];
+            int s = strat[v];
+            if (gc[g].reset_pending && owner_of(vi) == 1) {   // main.cpp:25-26
+                s = tgt[off[v]];
+                strat[v] = s;
+            }
+            if (solved[v]) {
+                S0[v] = make_int4(lab[v] == 1 ? -2 : -3, -1, 0, 0);
+                par[v] = -1;
+                if (trace) Z0[v] = 0;
+            } else {
+                par[v] = s;
+                S0[v] = make_int4(s, pri_of(vi), 1, 0);
+                if (trace) Z0[v] = splitmix64((unsigned long long)(uint32_t)pri_of(vi));
+            }
+        }
     }
-    int vi = vinfo[v];
-    int s = strat[v];
-    if (gc[g].reset_pending && owner_of(vi) == 1) {   // main.cpp:25-26
-        s = tgt[off[v]];
-        strat[v] = s;
-    }
-    if (solved[v]) {
-        S0[v] = make_int4(lab[v] == 1 ? -2 : -3, -1, 0, 0);
-        par[v] = -1;
-        if (trace) Z0[v] = 0;
-        return;
-    }
-    par[v] = s;
-    S0[v] = make_int4(s, pri_of(vi), 1, 0);
-    if (trace) Z0[v] = splitmix64((unsigned long long)(uint32_t)pri_of(vi));
+    // Blocks that hold no vertex of an unfinished game are skipped by the
+    // jump and label kernels of this valuation.
+    int any = __syncthreads_or(live);
+    if (threadIdx.x == 0) blive[blockIdx.x] = (uint8_t)any;
 }
 
 // One pointer-jumping round; grid-stride so that launches after convergence
@@ -194,7 +201,8 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn,
                                               const int4* __restrict__ Sin, int4* __restrict__ Sout,
                                               const unsigned long long* __restrict__ Zin,
                                               unsigned long long* __restrict__ Zout,
-                                              JumpCnt* __restrict__ jc, int trace)
+                                              JumpCnt* __restrict__ jc,
+                                              const uint8_t* __restrict__ blive, int trace)
 {
     __shared__ int run_s;
     __shared__ unsigned int cnt_res, cnt_act;
@@ -215,6 +223,7 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn,
     unsigned my_res = 0, my_act = 0;
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long base = (long long)blockIdx.x * blockDim.x; base < N; base += stride) {
+        if (!blive[base / kBS]) continue;         // whole chunk belongs to finished games
         long long v = base + threadIdx.x;
         bool res = false, act = false;
         if (v < N) {
@@ -255,7 +264,8 @@ __global__ void __launch_bounds__(kBS) k_label(
     const int* __restrict__ gid, const GameCtl* __restrict__ gc, const int* __restrict__ strat,
     int8_t* __restrict__ solved, int8_t* __restrict__ lab, int* __restrict__ dep,
     int* __restrict__ key, unsigned long long* __restrict__ dig, unsigned long long* __restrict__ tr,
-    int trace_cap, int loop_form, int trace, JumpCnt* __restrict__ jc)
+    int trace_cap, int loop_form, int trace, JumpCnt* __restrict__ jc,
+    const uint8_t* __restrict__ blive)
 {
     const int last = jc->last;
     const int F = last + 1;
@@ -265,6 +275,7 @@ __global__ void __launch_bounds__(kBS) k_label(
         int stop, cyc, halt;
         if (decide_round(lmax, jc, maxn, &stop, &cyc, &halt)) jc->err = 1;
     }
+    if (!blive[blockIdx.x]) return;               // uniform: the whole block is finished
     long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int g = -1, l = 0, sol = 0, loc = 0;
     unsigned long long d = 0;
@@ -570,91 +581,118 @@ template <class T> __global__ void k_fill(T* p, long long n, T v)
 // ---------------------------------------------------------------------------
 // Host side.
 
-static int validate(int G, const int* n_per, const long long* m_per, const int* owner,
-                    const int* priority, const int* offsets, const int* targets, long long* N,
-                    long long* M, int* maxn, int* max_pri)
-{
-    long long vb = 0, eb = 0;
-    *maxn = 0;
-    *max_pri = 0;
-    for (int g = 0; g < G; g++) {
-        int n = n_per[g];
-        long long m = m_per[g];
-        if (n <= 0) return fail(PSI_ERR_INVALID, "game with no vertices");
-        const int* off = offsets + vb + g;   // each game has n+1 offsets
-        if (off[0] != 0 || (long long)off[n] != m)
-            return fail(PSI_ERR_INVALID, "offsets must start at 0 and end at the edge count");
-        for (int v = 0; v < n; v++) {
-            if (off[v + 1] <= off[v]) return fail(PSI_ERR_INVALID, "sinks are not allowed!");
-            int o = owner[vb + v], p = priority[vb + v];
-            if (o != 0 && o != 1) return fail(PSI_ERR_INVALID, "invalid player");
-            if (p < 0 || p > kPriMask) return fail(PSI_ERR_INVALID, "priority out of range [0, 2^30)");
-            if (p > *max_pri) *max_pri = p;
-        }
-        for (long long e = 0; e < m; e++) {
-            int t = targets[eb + e];
-            if (t < 0 || t >= n) return fail(PSI_ERR_INVALID, "successor out of range");
-        }
-        if (n > *maxn) *maxn = n;
-        vb += n;
-        eb += m;
-    }
-    if (vb >= (1LL << 31) - 1 || eb >= (1LL << 31) - 1)
-        return fail(PSI_ERR_INVALID, "total vertices and edges must fit in int32");
-    *N = vb;
-    *M = eb;
-    return PSI_OK;
-}
-
 template <class T> static int dalloc(T** p, long long n)
 {
     PSI_HIP(hipMalloc((void**)p, (size_t)std::max(n, 1LL) * sizeof(T)));
     return PSI_OK;
 }
 
-int core_create(Core& c, int G, const int* n_per, const long long* m_per, const int* owner,
-                const int* priority, const int* offsets, const int* targets, bool trace,
-                int trace_cap, hipStream_t s)
+// Runs f(g) for g in [0, G) on up to 16 host threads.
+template <class F> static void parallel_games(int G, F f)
 {
-    long long N, M;
-    int maxn, max_pri;
+    int T = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    T = std::min(T, G);
+    if (T <= 1) {
+        for (int g = 0; g < G; g++) f(g);
+        return;
+    }
+    std::atomic<int> next(0);
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; t++)
+        th.emplace_back([&] {
+            for (int g; (g = next.fetch_add(1)) < G;) f(g);
+        });
+    for (auto& x : th) x.join();
+}
+
+static const char* validate_game(const GameView& gv, int* max_pri)
+{
+    const int n = gv.n;
+    const long long m = gv.m;
+    if (n <= 0) return "game with no vertices";
+    if (gv.off[0] != 0 || (long long)gv.off[n] != m)
+        return "offsets must start at 0 and end at the edge count";
+    int mp = 0;
+    for (int v = 0; v < n; v++) {
+        if (gv.off[v + 1] <= gv.off[v]) return "sinks are not allowed!";
+        int o = gv.owner[v], p = gv.pri[v];
+        if (o != 0 && o != 1) return "invalid player";
+        if (p < 0 || p > kPriMask) return "priority out of range [0, 2^30)";
+        mp = std::max(mp, p);
+    }
+    for (long long e = 0; e < m; e++)
+        if ((unsigned)gv.tgt[e] >= (unsigned)n) return "successor out of range";
+    *max_pri = mp;
+    return nullptr;
+}
+
+int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap, hipStream_t s)
+{
     if (G <= 0) return fail(PSI_ERR_INVALID, "no games");
-    if (int rc = validate(G, n_per, m_per, owner, priority, offsets, targets, &N, &M, &maxn, &max_pri))
-        return rc;
+    std::vector<long long> vbase(G + 1, 0), ebase(G + 1, 0);
+    int maxn = 0;
+    for (int g = 0; g < G; g++) {
+        vbase[g + 1] = vbase[g] + std::max(games[g].n, 0);
+        ebase[g + 1] = ebase[g] + std::max(games[g].m, 0LL);
+        maxn = std::max(maxn, games[g].n);
+    }
+    const long long N = vbase[G], M = ebase[G];
+    if (N >= (1LL << 31) - 1 || M >= (1LL << 31) - 1)
+        return fail(PSI_ERR_INVALID, "total vertices and edges must fit in int32");
+    std::vector<const char*> errs(G, nullptr);
+    std::vector<int> mp(G, 0);
+    parallel_games(G, [&](int g) { errs[g] = validate_game(games[g], &mp[g]); });
+    for (int g = 0; g < G; g++)
+        if (errs[g]) return fail(PSI_ERR_INVALID, errs[g]);
     c.G = G;
     c.N = N;
     c.M = M;
     c.s = s;
     c.trace = trace;
     c.trace_cap = trace ? std::max(trace_cap, 1) : 0;
-    c.max_pri = max_pri;
+    c.max_pri = *std::max_element(mp.begin(), mp.end());
     c.lmax = ceil_log2(maxn) + 2;
     c.levels = c.lmax + 1;
-    c.h_n.assign(n_per, n_per + G);
-    c.h_m.assign(m_per, m_per + G);
+    c.h_n.resize(G);
+    c.h_m.resize(G);
     c.h_vbase.resize(G);
-    // Global CSR with global target ids.
-    std::vector<int> vinfo(N), off(N + 1), tgt(M), gid(N), own0, own1;
+    // Global CSR with global target ids; per-player vertex lists.
+    std::vector<int> vinfo(N), off(N + 1), tgt(M), gid(N);
     std::vector<GameCtl> gc(G);
-    long long vb = 0, eb = 0;
-    for (int g = 0; g < G; g++) {
-        int n = n_per[g];
-        const int* o = offsets + vb + g;
-        for (int v = 0; v < n; v++) {
-            vinfo[vb + v] = priority[vb + v] | (owner[vb + v] << 30);
-            (owner[vb + v] ? own1 : own0).push_back((int)(vb + v));
-            off[vb + v] = (int)(eb + o[v]);
+    std::vector<long long> own1_per(G + 1, 0);
+    parallel_games(G, [&](int g) {
+        const GameView& gv = games[g];
+        const long long vb = vbase[g], eb = ebase[g];
+        long long o1 = 0;
+        for (int v = 0; v < gv.n; v++) {
+            vinfo[vb + v] = gv.pri[v] | (gv.owner[v] << 30);
+            off[vb + v] = (int)(eb + gv.off[v]);
             gid[vb + v] = g;
+            o1 += gv.owner[v];
         }
-        for (long long e = 0; e < m_per[g]; e++) tgt[eb + e] = (int)(targets[eb + e] + vb);
+        for (long long e = 0; e < gv.m; e++) tgt[eb + e] = (int)(gv.tgt[e] + vb);
+        own1_per[g + 1] = o1;
         std::memset(&gc[g], 0, sizeof(GameCtl));
         gc[g].vbase = (int)vb;
-        gc[g].n = n;
-        c.h_vbase[g] = (int)vb;
-        vb += n;
-        eb += m_per[g];
-    }
+        gc[g].n = gv.n;
+    });
     off[N] = (int)M;
+    for (int g = 0; g < G; g++) {
+        own1_per[g + 1] += own1_per[g];
+        c.h_n[g] = games[g].n;
+        c.h_m[g] = games[g].m;
+        c.h_vbase[g] = (int)vbase[g];
+    }
+    c.own_n[1] = own1_per[G];
+    c.own_n[0] = N - c.own_n[1];
+    std::vector<int> own0(c.own_n[0]), own1(c.own_n[1]);
+    parallel_games(G, [&](int g) {
+        long long k1 = own1_per[g], k0 = vbase[g] - own1_per[g];
+        for (long long v = vbase[g]; v < vbase[g + 1]; v++) {
+            if (vinfo[v] >> 30) own1[k1++] = (int)v;
+            else own0[k0++] = (int)v;
+        }
+    });
     if (int rc = dalloc(&c.vinfo, N)) return rc;
     if (int rc = dalloc(&c.off, N + 1)) return rc;
     if (int rc = dalloc(&c.tgt, M)) return rc;
@@ -665,13 +703,10 @@ int core_create(Core& c, int G, const int* n_per, const long long* m_per, const 
     if (int rc = dalloc(&c.key, N)) return rc;
     if (int rc = dalloc(&c.solved, N)) return rc;
     if (int rc = dalloc(&c.lab, N)) return rc;
+    if (int rc = dalloc(&c.blive, nblocks(N, kBS))) return rc;
     if (int rc = dalloc(&c.Sl, N * c.levels)) return rc;
-    c.own_n[0] = (long long)own0.size();
-    c.own_n[1] = (long long)own1.size();
     if (int rc = dalloc(&c.own[0], c.own_n[0])) return rc;
     if (int rc = dalloc(&c.own[1], c.own_n[1])) return rc;
-    if (c.own_n[0]) PSI_HIP(hipMemcpyAsync(c.own[0], own0.data(), c.own_n[0] * 4, hipMemcpyHostToDevice, s));
-    if (c.own_n[1]) PSI_HIP(hipMemcpyAsync(c.own[1], own1.data(), c.own_n[1] * 4, hipMemcpyHostToDevice, s));
     if (trace) {
         if (int rc = dalloc(&c.Z[0], N)) return rc;
         if (int rc = dalloc(&c.Z[1], N)) return rc;
@@ -685,6 +720,8 @@ int core_create(Core& c, int G, const int* n_per, const long long* m_per, const 
     PSI_HIP(hipMemcpyAsync(c.off, off.data(), (N + 1) * 4, hipMemcpyHostToDevice, s));
     PSI_HIP(hipMemcpyAsync(c.tgt, tgt.data(), M * 4, hipMemcpyHostToDevice, s));
     PSI_HIP(hipMemcpyAsync(c.gid, gid.data(), N * 4, hipMemcpyHostToDevice, s));
+    if (c.own_n[0]) PSI_HIP(hipMemcpyAsync(c.own[0], own0.data(), c.own_n[0] * 4, hipMemcpyHostToDevice, s));
+    if (c.own_n[1]) PSI_HIP(hipMemcpyAsync(c.own[1], own1.data(), c.own_n[1] * 4, hipMemcpyHostToDevice, s));
     PSI_HIP(hipMemcpyAsync(c.gctl, gc.data(), G * sizeof(GameCtl), hipMemcpyHostToDevice, s));
     PSI_HIP(hipStreamSynchronize(s));
     return PSI_OK;
@@ -693,7 +730,7 @@ int core_create(Core& c, int G, const int* n_per, const long long* m_per, const 
 void core_destroy(Core& c)
 {
     if (c.graph) (void)hipGraphExecDestroy(c.graph);
-    void* ptrs[] = {c.vinfo, c.off, c.tgt, c.gid, c.strat, c.par, c.dep, c.key, c.solved, c.lab,
+    void* ptrs[] = {c.vinfo, c.off, c.tgt, c.gid, c.strat, c.par, c.dep, c.key, c.solved, c.lab, c.blive,
                     c.Sl, c.own[0], c.own[1], c.Z[0], c.Z[1], c.dig, c.tr, c.gctl, c.jc, c.flag};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -714,20 +751,20 @@ static int maxn_of(const Core& c)
 static void launch_val_init(Core& c)
 {
     k_val_init<<<GRID(c.N)>>>(c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
-                              c.par, c.Sl, c.Z[0], c.jc, c.trace);
+                              c.par, c.Sl, c.Z[0], c.jc, c.blive, c.trace);
 }
 
 static void launch_jump(Core& c, int r, int maxn)
 {
     k_jump<<<JGRID(c.N)>>>(r, c.N, maxn, c.Sl + (size_t)r * c.N, c.Sl + (size_t)(r + 1) * c.N,
-                           c.Z[r & 1], c.Z[(r + 1) & 1], c.jc, c.trace);
+                           c.Z[r & 1], c.Z[(r + 1) & 1], c.jc, c.blive, c.trace);
 }
 
 static void launch_label(Core& c, int maxn, bool loop_form)
 {
     k_label<<<GRID(c.N)>>>(c.N, maxn, c.lmax, c.Sl, c.Z[0], c.Z[1], c.gid, c.gctl, c.strat,
                            c.solved, c.lab, c.dep, c.key, c.dig, c.tr, c.trace_cap, loop_form,
-                           c.trace, c.jc);
+                           c.trace, c.jc, c.blive);
 }
 
 static void launch_switch(Core& c, int player, bool loop_form)

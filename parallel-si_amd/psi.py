@@ -251,6 +251,8 @@ _sig("psi_solver_create", C.c_int, _vp, C.c_int, C.c_int, _pp)
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _sig("psi_solver_create_batch", C.c_int, C.c_int, _i32p, _i64p, _i32p, _i32p, _i32p, _i32p,
      C.c_int, C.c_int, _pp)
+_sig("psi_solver_create_games", C.c_int, C.c_int, _i32p, _i64p, C.POINTER(_vp), C.POINTER(_vp),
+     C.POINTER(_vp), C.POINTER(_vp), C.c_int, C.c_int, _pp)
 _sig("psi_solver_destroy", C.c_int, _vp)
 _sig("psi_solver_info", C.c_int, _vp, C.POINTER(C.c_int), C.POINTER(C.c_int64),
      C.POINTER(C.c_int64))
@@ -281,16 +283,14 @@ class Solver:
         self.games = list(games)
         n_per = np.array([g.n for g in self.games], np.int32)
         m_per = np.array([g.m for g in self.games], np.int64)
-        owner = np.concatenate([g.owner for g in self.games])
-        pri = np.concatenate([g.priority for g in self.games])
-        off = np.concatenate([g.offsets for g in self.games])
-        tgt = np.concatenate([g.targets for g in self.games])
+        G = len(self.games)
+        ptrs = [(_vp * G)(*[getattr(g, a).ctypes.data for g in self.games])
+                for a in ("owner", "priority", "offsets", "targets")]
         self.h = _vp()
         self.trace = trace
         self.trace_cap = trace_cap
-        check(lib.psi_solver_create_batch(len(self.games), n_per, m_per, owner, pri, off, tgt,
-                                          SOLVER_TRACE if trace else 0, trace_cap,
-                                          C.byref(self.h)), "psi_solver_create_batch")
+        check(lib.psi_solver_create_games(G, n_per, m_per, *ptrs, SOLVER_TRACE if trace else 0,
+                                          trace_cap, C.byref(self.h)), "psi_solver_create_games")
 
     def close(self):
         if self.h:
