@@ -129,11 +129,15 @@ def batch_bench(psi, n_games=1024, seed=3, lo=1000, hi=1000000, check_alone=True
     # batched: one handle, every game with its own convergence
     t = time.perf_counter()
     b = psi.Solver(games)
+    t1 = time.perf_counter()
     b.solve(steps_per_sync=4)
-    res_b = [(b.stats(k), b.strategy(k), b.labels(k)) for k in range(n_games)]
+    t2 = time.perf_counter()
+    res_b = b.results()
+    t3 = time.perf_counter()
     b.close()
     t_batch = time.perf_counter() - t
-    log(f"[bench] batch: batched solve {t_batch:.2f}s")
+    log(f"[bench] batch: batched {t_batch:.2f}s (create+upload {t1 - t:.2f}, solve {t2 - t1:.2f}, "
+        f"download {t3 - t2:.2f}, free {t_batch - (t3 - t):.2f})")
     # one at a time
     t = time.perf_counter()
     res_a = []

@@ -155,6 +155,9 @@ int psi_get_strategy(psi_solver* s, int game, int* out);   /* -1 = sink */
 int psi_get_labels(psi_solver* s, int game, int* out);     /* infinite[]: -1, 0, +1 */
 int psi_get_solved(psi_solver* s, int game, int* out);
 int psi_get_depths(psi_solver* s, int game, int* out);     /* path length to sink */
+/* Every game at once, concatenated in game order (strategies in local ids,
+ * labels) and stats_all[3g..3g+2] = (iter, br, capped).  Pointers may be null. */
+int psi_get_results(psi_solver* s, int* strat_all, int* labels_all, int* stats_all);
 /* Trace mode only.  Digest of a finite valuation: sum_p count[p]*splitmix64(p)
  * mod 2^64 (0 for infinite vertices).  Trace: per valuation round, hashes of
  * the strategy valued, solved flags, labels and digests (4 x uint64). */

@@ -336,3 +336,38 @@ extern "C" int psi_get_jump_stats(psi_solver* s, int* out, int rounds, int* last
     *err = jc.err;
     return PSI_OK;
 }
+
+// All games at once, concatenated in game order: strategies (local ids, -1
+// sink), labels, and per game (iter, br, capped).  Any pointer may be null.
+extern "C" int psi_get_results(psi_solver* s, int* strat_all, int* labels_all, int* stats_all)
+{
+    Core& c = s->core;
+    if (strat_all) {
+        PSI_HIP(hipMemcpyAsync(strat_all, c.strat, (size_t)c.N * 4, hipMemcpyDeviceToHost, c.s));
+    }
+    std::vector<int8_t> lab;
+    if (labels_all) {
+        lab.resize(c.N);
+        PSI_HIP(hipMemcpyAsync(lab.data(), c.lab, (size_t)c.N, hipMemcpyDeviceToHost, c.s));
+    }
+    std::vector<GameCtl> gc;
+    if (stats_all) {
+        gc.resize(c.G);
+        PSI_HIP(hipMemcpyAsync(gc.data(), c.gctl, c.G * sizeof(GameCtl), hipMemcpyDeviceToHost, c.s));
+    }
+    PSI_HIP(hipStreamSynchronize(c.s));
+    for (int g = 0; g < c.G; g++) {
+        long long vb = c.h_vbase[g], n = c.h_n[g];
+        if (strat_all)
+            for (long long v = vb; v < vb + n; v++)
+                if (strat_all[v] >= 0) strat_all[v] -= (int)vb;
+        if (labels_all)
+            for (long long v = vb; v < vb + n; v++) labels_all[v] = lab[v];
+        if (stats_all) {
+            stats_all[3 * g] = gc[g].iter;
+            stats_all[3 * g + 1] = gc[g].br;
+            stats_all[3 * g + 2] = gc[g].capped;
+        }
+    }
+    return PSI_OK;
+}

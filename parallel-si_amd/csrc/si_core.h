@@ -61,8 +61,8 @@ struct Core {
     int8_t* solved = nullptr;
     int8_t* lab = nullptr; // infinite[]: -1, 0, +1
     uint8_t* blive = nullptr; // per 256-vertex block: holds a vertex of an unfinished game
-    int4* Sl = nullptr;    // jump records of every level: level j at Sl + j*N
-    int levels = 0;        // lmax + 1
+    int4* Sl = nullptr;    // jump records: level j at Sl + level_slot(j)*N
+    int levels = 0;        // buffers allocated
     int* own[2] = {nullptr, nullptr};   // vertices owned by each player
     long long own_n[2] = {0, 0};
     unsigned long long* Z[2] = {nullptr, nullptr};

@@ -45,6 +45,13 @@ namespace psi {
 
 constexpr int kBS = 256;
 constexpr int kPriMask = 0x3FFFFFFF;
+// Jump levels 0..kLiftCap are kept as binary-lifting tables; later rounds
+// (only needed to cover long cycles) rotate through two more buffers.
+constexpr int kLiftCap = 12;
+__host__ __device__ __forceinline__ int level_slot(int j)
+{
+    return j <= kLiftCap ? j : kLiftCap + 1 + (j & 1);
+}
 
 __device__ __forceinline__ int pri_of(int vi) { return vi & kPriMask; }
 __device__ __forceinline__ int owner_of(int vi) { return (vi >> 30) & 1; }
@@ -269,7 +276,7 @@ __global__ void __launch_bounds__(kBS) k_label(
 {
     const int last = jc->last;
     const int F = last + 1;
-    const int4* SF = Sl + (size_t)F * N;
+    const int4* SF = Sl + (size_t)level_slot(F) * N;
     const unsigned long long* ZF = (F & 1) ? Z1 : Z0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         int stop, cyc, halt;
@@ -333,34 +340,33 @@ __device__ __noinline__ int cmp_exact(int u, int w, const int* __restrict__ par,
                          const int* __restrict__ vinfo, const int4* __restrict__ Sl, long long N,
                          int F)
 {
+    const int T = min(F, kLiftCap);                   // highest stored table level
+    auto rec = [&](int j, int x) { return Sl[(size_t)j * N + x]; };
     int a = u, b = w, da = dep[u], db = dep[w];
     int mA = -1, mB = -1;
-    for (int j = F; j >= 0; j--) {
-        if (da - db >= (1 << j)) {
-            int4 t = Sl[(size_t)j * N + a];
-            mA = max(mA, t.y);
-            a = t.x;
-            da -= 1 << j;
-        }
-        if (db - da >= (1 << j)) {
-            int4 t = Sl[(size_t)j * N + b];
-            mB = max(mB, t.y);
-            b = t.x;
-            db -= 1 << j;
-        }
+    // Bring the deeper one up to the other's depth.
+    while (da - db >= (1 << T)) { int4 t = rec(T, a); mA = max(mA, t.y); a = t.x; da -= 1 << T; }
+    while (db - da >= (1 << T)) { int4 t = rec(T, b); mB = max(mB, t.y); b = t.x; db -= 1 << T; }
+    for (int j = T - 1; j >= 0; j--) {
+        if (da - db >= (1 << j)) { int4 t = rec(j, a); mA = max(mA, t.y); a = t.x; da -= 1 << j; }
+        if (db - da >= (1 << j)) { int4 t = rec(j, b); mB = max(mB, t.y); b = t.x; db -= 1 << j; }
     }
     int dl;                                           // depth of the common ancestor
     if (a == b) {
         dl = da;
     } else {
-        for (int j = F; j >= 0; j--) {
-            int4 ta = Sl[(size_t)j * N + a], tb = Sl[(size_t)j * N + b];
+        // Jump both while their ancestors differ (only levels shorter than the
+        // remaining path are read: longer windows end at the sink for both).
+        while ((1 << T) < da) {
+            int4 ta = rec(T, a), tb = rec(T, b);
+            if (ta.x == tb.x) break;
+            mA = max(mA, ta.y); mB = max(mB, tb.y); a = ta.x; b = tb.x; da -= 1 << T;
+        }
+        for (int j = T - 1; j >= 0; j--) {
+            if ((1 << j) >= da) continue;
+            int4 ta = rec(j, a), tb = rec(j, b);
             if (ta.x != tb.x) {
-                mA = max(mA, ta.y);
-                mB = max(mB, tb.y);
-                a = ta.x;
-                b = tb.x;
-                da -= 1 << j;
+                mA = max(mA, ta.y); mB = max(mB, tb.y); a = ta.x; b = tb.x; da -= 1 << j;
             }
         }
         mA = max(mA, pri_of(vinfo[a]));
@@ -652,7 +658,7 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
     c.trace_cap = trace ? std::max(trace_cap, 1) : 0;
     c.max_pri = *std::max_element(mp.begin(), mp.end());
     c.lmax = ceil_log2(maxn) + 2;
-    c.levels = c.lmax + 1;
+    c.levels = std::min(c.lmax + 1, kLiftCap + 3);
     c.h_n.resize(G);
     c.h_m.resize(G);
     c.h_vbase.resize(G);
@@ -756,7 +762,8 @@ static void launch_val_init(Core& c)
 
 static void launch_jump(Core& c, int r, int maxn)
 {
-    k_jump<<<JGRID(c.N)>>>(r, c.N, maxn, c.Sl + (size_t)r * c.N, c.Sl + (size_t)(r + 1) * c.N,
+    k_jump<<<JGRID(c.N)>>>(r, c.N, maxn, c.Sl + (size_t)level_slot(r) * c.N,
+                           c.Sl + (size_t)level_slot(r + 1) * c.N,
                            c.Z[r & 1], c.Z[(r + 1) & 1], c.jc, c.blive, c.trace);
 }
 

@@ -271,6 +271,7 @@ for _f in ("psi_get_strategy", "psi_get_labels", "psi_get_solved", "psi_get_dept
 _sig("psi_get_digests", C.c_int, _vp, C.c_int, _u64p)
 _sig("psi_get_trace", C.c_int, _vp, C.c_int, _u64p, C.c_int, C.POINTER(C.c_int))
 _sig("psi_set_state", C.c_int, _vp, C.c_int, _i32p, _i32p, _i32p)
+_sig("psi_get_results", C.c_int, _vp, _i32p, _i32p, _i32p)
 
 
 class Solver:
@@ -364,6 +365,20 @@ class Solver:
         k = C.c_int()
         check(lib.psi_get_winning(self.h, game, player, out, C.byref(k)), "get_winning")
         return out[: k.value]
+
+    def results(self):
+        """Per game: ((iter, br, capped), strategy, labels), one copy each."""
+        N = sum(g.n for g in self.games)
+        st = np.empty(N, np.int32)
+        lb = np.empty(N, np.int32)
+        stats = np.empty(3 * len(self.games), np.int32)
+        check(lib.psi_get_results(self.h, st, lb, stats), "psi_get_results")
+        out, o = [], 0
+        for k, g in enumerate(self.games):
+            out.append(((int(stats[3 * k]), int(stats[3 * k + 1]), bool(stats[3 * k + 2])),
+                        st[o:o + g.n], lb[o:o + g.n]))
+            o += g.n
+        return out
 
     def trace_hashes(self, game=0):
         buf = np.zeros(4 * self.trace_cap, np.uint64)

@@ -160,6 +160,7 @@ def test_batch_equals_alone(gpu):
             games.append(gpu.gen_random_game(50 + i, n, max(1, n // 3), 1, 4))
     b = gpu.Solver(games, trace=True)
     b.solve()
+    bulk = b.results()
     for k, g in enumerate(games):
         r = oracle.solve(g, trace=True)
         it, br, capped = b.stats(k)
@@ -167,6 +168,8 @@ def test_batch_equals_alone(gpu):
         assert np.array_equal(b.strategy(k), r.strat), k
         assert np.array_equal(b.labels(k), r.inf), k
         assert np.array_equal(b.trace_hashes(k), r.hashes), k
+        assert bulk[k][0] == (it, br, capped)
+        assert np.array_equal(bulk[k][1], r.strat) and np.array_equal(bulk[k][2], r.inf)
 
 
 def test_round_cap(gpu):
@@ -187,3 +190,34 @@ def test_invalid_games_rejected(gpu):
     G = gpu.Game([0, 1], [-1, 2], [0, 1, 2], [1, 0])
     with pytest.raises(gpu.PsiError, match="priority"):
         gpu.Solver(G)
+
+
+def _ladder(n, seed, podd):
+    """Mostly-Even ladder v -> v+1, v+1+r with priorities rising along it and a
+    small odd priority at the tail: sink-tree depths reach ~n (beyond the
+    2^12 stored lifting levels, so deep compares take repeated top jumps)."""
+    rng = np.random.default_rng(seed)
+    import psi
+    owner = np.zeros(n, np.int32)
+    pri = (2 * (np.arange(n) // 7)).astype(np.int32)
+    pri[rng.random(n) < podd] += 1
+    pri[-1] = 1
+    off, tgt = [0], []
+    for v in range(n):
+        if v == n - 1:
+            tgt.append(v)
+        else:
+            tgt.append(v + 1)
+            j = v + 1 + int(rng.integers(1, 3))
+            if j < n:
+                tgt.append(j)
+        off.append(len(tgt))
+    return psi.Game(owner, pri, np.array(off, np.int32), np.array(tgt, np.int32))
+
+
+@pytest.mark.parametrize("podd", [0.0, 0.02, 0.1])
+def test_deep_sink_trees(gpu, podd):
+    g = _ladder(12000, 3, podd)
+    print(f"deep ladder podd={podd}", flush=True)
+    r, s = _solve_both(gpu, g)
+    _assert_same(r, s)
