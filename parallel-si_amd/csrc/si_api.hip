@@ -356,7 +356,7 @@ extern "C" int psi_get_results(psi_solver* s, int* strat_all, int* labels_all, i
         PSI_HIP(hipMemcpyAsync(gc.data(), c.gctl, c.G * sizeof(GameCtl), hipMemcpyDeviceToHost, c.s));
     }
     PSI_HIP(hipStreamSynchronize(c.s));
-    for (int g = 0; g < c.G; g++) {
+    parallel_games(c.G, [&](int g) {
         long long vb = c.h_vbase[g], n = c.h_n[g];
         if (strat_all)
             for (long long v = vb; v < vb + n; v++)
@@ -368,6 +368,6 @@ extern "C" int psi_get_results(psi_solver* s, int* strat_all, int* labels_all, i
             stats_all[3 * g + 1] = gc[g].br;
             stats_all[3 * g + 2] = gc[g].capped;
         }
-    }
+    });
     return PSI_OK;
 }

@@ -8,9 +8,30 @@
 
 #include "common.h"
 
+#include <algorithm>
+#include <atomic>
+#include <thread>
 #include <vector>
 
 namespace psi {
+
+// Runs f(g) for g in [0, G) on up to 16 host threads.
+template <class F> void parallel_games(int G, F f)
+{
+    int T = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    T = std::min(T, G);
+    if (T <= 1) {
+        for (int g = 0; g < G; g++) f(g);
+        return;
+    }
+    std::atomic<int> next(0);
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; t++)
+        th.emplace_back([&] {
+            for (int g; (g = next.fetch_add(1)) < G;) f(g);
+        });
+    for (auto& x : th) x.join();
+}
 
 // Per-game control block (device memory).
 struct GameCtl {

@@ -38,6 +38,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 
@@ -593,24 +595,6 @@ template <class T> static int dalloc(T** p, long long n)
     return PSI_OK;
 }
 
-// Runs f(g) for g in [0, G) on up to 16 host threads.
-template <class F> static void parallel_games(int G, F f)
-{
-    int T = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
-    T = std::min(T, G);
-    if (T <= 1) {
-        for (int g = 0; g < G; g++) f(g);
-        return;
-    }
-    std::atomic<int> next(0);
-    std::vector<std::thread> th;
-    for (int t = 0; t < T; t++)
-        th.emplace_back([&] {
-            for (int g; (g = next.fetch_add(1)) < G;) f(g);
-        });
-    for (auto& x : th) x.join();
-}
-
 static const char* validate_game(const GameView& gv, int* max_pri)
 {
     const int n = gv.n;
@@ -632,8 +616,112 @@ static const char* validate_game(const GameView& gv, int* max_pri)
     return nullptr;
 }
 
+// PSI_TIMING=1 in the environment prints the phases of solver creation.
+struct PhaseTimer {
+    bool on;
+    std::chrono::steady_clock::time_point t0;
+    PhaseTimer() : on(getenv("PSI_TIMING") != nullptr), t0(std::chrono::steady_clock::now()) {}
+    void mark(const char* what)
+    {
+        if (!on) return;
+        auto t1 = std::chrono::steady_clock::now();
+        fprintf(stderr, "[psi] %s %.3f s\n", what, std::chrono::duration<double>(t1 - t0).count());
+        t0 = t1;
+    }
+};
+
+// ---- device-side packing of a batch ----------------------------------------
+__device__ __forceinline__ int game_of(const long long* __restrict__ base, int G, long long x)
+{
+    int lo = 0, hi = G - 1;                       // last g with base[g] <= x
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// owner/priority were copied into own_tmp/pri_tmp; offsets (local) into off.
+__global__ void k_pack_vertices(long long N, int G, const long long* __restrict__ vbase,
+                                const long long* __restrict__ ebase, const int* __restrict__ own_tmp,
+                                const int* __restrict__ pri_tmp, int* __restrict__ vinfo,
+                                int* __restrict__ off, int* __restrict__ gid)
+{
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    int g = game_of(vbase, G, v);
+    gid[v] = g;
+    vinfo[v] = pri_tmp[v] | (own_tmp[v] << 30);
+    off[v] = (int)(off[v] + ebase[g]);
+}
+
+__global__ void k_globalize_targets(long long N, const int* __restrict__ off, const int* __restrict__ gid,
+                                    const GameCtl* __restrict__ gc, int* __restrict__ tgt)
+{
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    int vb = gc[gid[v]].vbase;
+    for (int e = off[v]; e < off[v + 1]; e++) tgt[e] += vb;
+}
+
+// Owner lists (vertex ids of each player, ascending) by a block-count /
+// scan / scatter compaction.
+__global__ void k_owner_count(long long N, const int* __restrict__ vinfo, int* __restrict__ cnt1)
+{
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int o = v < N ? owner_of(vinfo[v]) : 0;
+    int c = __syncthreads_count(o);
+    if (threadIdx.x == 0) cnt1[blockIdx.x] = c;
+}
+
+__global__ void __launch_bounds__(1024) k_owner_scan(int nb, int* __restrict__ cnt1)
+{
+    // exclusive scan of nb block counts in place, one workgroup
+    __shared__ int part[1024];
+    const int per = (nb + 1023) / 1024;
+    const int t = threadIdx.x, lo = t * per, hi = min(nb, lo + per);
+    int sum = 0;
+    for (int i = lo; i < hi; i++) sum += cnt1[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int x = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int i = lo; i < hi; i++) {
+        int c = cnt1[i];
+        cnt1[i] = run;
+        run += c;
+    }
+}
+
+__global__ void k_owner_scatter(long long N, const int* __restrict__ vinfo, const int* __restrict__ pre1,
+                                int* __restrict__ own0, int* __restrict__ own1)
+{
+    __shared__ int wsum[kBS / kWave];
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int o = v < N ? owner_of(vinfo[v]) : 0;
+    unsigned long long m = __ballot(o);
+    int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int before = 0;
+    for (int k = 0; k < w; k++) before += wsum[k];
+    int in_wave = __popcll(m & ((1ull << lane) - 1ull));
+    int r1 = pre1[blockIdx.x] + before + in_wave;            // ones before v
+    if (v < N) {
+        if (o) own1[r1] = (int)v;
+        else own0[v - r1] = (int)v;
+    }
+}
+
 int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap, hipStream_t s)
 {
+    PhaseTimer pt;
     if (G <= 0) return fail(PSI_ERR_INVALID, "no games");
     std::vector<long long> vbase(G + 1, 0), ebase(G + 1, 0);
     int maxn = 0;
@@ -650,6 +738,7 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
     parallel_games(G, [&](int g) { errs[g] = validate_game(games[g], &mp[g]); });
     for (int g = 0; g < G; g++)
         if (errs[g]) return fail(PSI_ERR_INVALID, errs[g]);
+    pt.mark("validate");
     c.G = G;
     c.N = N;
     c.M = M;
@@ -662,43 +751,15 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
     c.h_n.resize(G);
     c.h_m.resize(G);
     c.h_vbase.resize(G);
-    // Global CSR with global target ids; per-player vertex lists.
-    std::vector<int> vinfo(N), off(N + 1), tgt(M), gid(N);
     std::vector<GameCtl> gc(G);
-    std::vector<long long> own1_per(G + 1, 0);
-    parallel_games(G, [&](int g) {
-        const GameView& gv = games[g];
-        const long long vb = vbase[g], eb = ebase[g];
-        long long o1 = 0;
-        for (int v = 0; v < gv.n; v++) {
-            vinfo[vb + v] = gv.pri[v] | (gv.owner[v] << 30);
-            off[vb + v] = (int)(eb + gv.off[v]);
-            gid[vb + v] = g;
-            o1 += gv.owner[v];
-        }
-        for (long long e = 0; e < gv.m; e++) tgt[eb + e] = (int)(gv.tgt[e] + vb);
-        own1_per[g + 1] = o1;
-        std::memset(&gc[g], 0, sizeof(GameCtl));
-        gc[g].vbase = (int)vb;
-        gc[g].n = gv.n;
-    });
-    off[N] = (int)M;
     for (int g = 0; g < G; g++) {
-        own1_per[g + 1] += own1_per[g];
         c.h_n[g] = games[g].n;
         c.h_m[g] = games[g].m;
         c.h_vbase[g] = (int)vbase[g];
+        std::memset(&gc[g], 0, sizeof(GameCtl));
+        gc[g].vbase = (int)vbase[g];
+        gc[g].n = games[g].n;
     }
-    c.own_n[1] = own1_per[G];
-    c.own_n[0] = N - c.own_n[1];
-    std::vector<int> own0(c.own_n[0]), own1(c.own_n[1]);
-    parallel_games(G, [&](int g) {
-        long long k1 = own1_per[g], k0 = vbase[g] - own1_per[g];
-        for (long long v = vbase[g]; v < vbase[g + 1]; v++) {
-            if (vinfo[v] >> 30) own1[k1++] = (int)v;
-            else own0[k0++] = (int)v;
-        }
-    });
     if (int rc = dalloc(&c.vinfo, N)) return rc;
     if (int rc = dalloc(&c.off, N + 1)) return rc;
     if (int rc = dalloc(&c.tgt, M)) return rc;
@@ -711,8 +772,6 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
     if (int rc = dalloc(&c.lab, N)) return rc;
     if (int rc = dalloc(&c.blive, nblocks(N, kBS))) return rc;
     if (int rc = dalloc(&c.Sl, N * c.levels)) return rc;
-    if (int rc = dalloc(&c.own[0], c.own_n[0])) return rc;
-    if (int rc = dalloc(&c.own[1], c.own_n[1])) return rc;
     if (trace) {
         if (int rc = dalloc(&c.Z[0], N)) return rc;
         if (int rc = dalloc(&c.Z[1], N)) return rc;
@@ -722,14 +781,56 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
     if (int rc = dalloc(&c.gctl, G)) return rc;
     if (int rc = dalloc(&c.jc, 1)) return rc;
     if (int rc = dalloc(&c.flag, 4)) return rc;
-    PSI_HIP(hipMemcpyAsync(c.vinfo, vinfo.data(), N * 4, hipMemcpyHostToDevice, s));
-    PSI_HIP(hipMemcpyAsync(c.off, off.data(), (N + 1) * 4, hipMemcpyHostToDevice, s));
-    PSI_HIP(hipMemcpyAsync(c.tgt, tgt.data(), M * 4, hipMemcpyHostToDevice, s));
-    PSI_HIP(hipMemcpyAsync(c.gid, gid.data(), N * 4, hipMemcpyHostToDevice, s));
-    if (c.own_n[0]) PSI_HIP(hipMemcpyAsync(c.own[0], own0.data(), c.own_n[0] * 4, hipMemcpyHostToDevice, s));
-    if (c.own_n[1]) PSI_HIP(hipMemcpyAsync(c.own[1], own1.data(), c.own_n[1] * 4, hipMemcpyHostToDevice, s));
+    long long *d_vbase = nullptr, *d_ebase = nullptr;
+    int* d_cnt = nullptr;
+    const unsigned nb = nblocks(N, kBS);
+    if (int rc = dalloc(&d_vbase, G + 1)) return rc;
+    if (int rc = dalloc(&d_ebase, G + 1)) return rc;
+    if (int rc = dalloc(&d_cnt, nb)) return rc;
+    pt.mark("alloc");
+    // Each game's arrays go straight to their place; owner and priority are
+    // staged in par/dep, which the first valuation overwrites.
+    for (int g = 0; g < G; g++) {
+        const GameView& gv = games[g];
+        PSI_HIP(hipMemcpyAsync(c.par + vbase[g], gv.owner, (size_t)gv.n * 4, hipMemcpyHostToDevice, s));
+        PSI_HIP(hipMemcpyAsync(c.dep + vbase[g], gv.pri, (size_t)gv.n * 4, hipMemcpyHostToDevice, s));
+        PSI_HIP(hipMemcpyAsync(c.off + vbase[g], gv.off, (size_t)gv.n * 4, hipMemcpyHostToDevice, s));
+        PSI_HIP(hipMemcpyAsync(c.tgt + ebase[g], gv.tgt, (size_t)gv.m * 4, hipMemcpyHostToDevice, s));
+    }
+    int Mi = (int)M;
+    PSI_HIP(hipMemcpyAsync(c.off + N, &Mi, 4, hipMemcpyHostToDevice, s));
+    PSI_HIP(hipMemcpyAsync(d_vbase, vbase.data(), (G + 1) * 8, hipMemcpyHostToDevice, s));
+    PSI_HIP(hipMemcpyAsync(d_ebase, ebase.data(), (G + 1) * 8, hipMemcpyHostToDevice, s));
     PSI_HIP(hipMemcpyAsync(c.gctl, gc.data(), G * sizeof(GameCtl), hipMemcpyHostToDevice, s));
+    pt.mark("copy");
+    k_pack_vertices<<<nb, kBS, 0, s>>>(N, G, d_vbase, d_ebase, c.par, c.dep, c.vinfo, c.off, c.gid);
+    k_globalize_targets<<<nb, kBS, 0, s>>>(N, c.off, c.gid, c.gctl, c.tgt);
+    k_owner_count<<<nb, kBS, 0, s>>>(N, c.vinfo, d_cnt);
+    k_owner_scan<<<1, 1024, 0, s>>>((int)nb, d_cnt);
+    PSI_LAUNCH_CHECK();
+    // number of player-1 vertices = last prefix + last block's count
+    int last_pre = 0;
+    PSI_HIP(hipMemcpyAsync(&last_pre, d_cnt + nb - 1, 4, hipMemcpyDeviceToHost, s));
     PSI_HIP(hipStreamSynchronize(s));
+    int last_cnt = 0;
+    {
+        // count of the last block, recomputed on the host side from vinfo tail
+        long long lo = (long long)(nb - 1) * kBS;
+        std::vector<int> tail(N - lo);
+        PSI_HIP(hipMemcpy(tail.data(), c.vinfo + lo, (N - lo) * 4, hipMemcpyDeviceToHost));
+        for (int x : tail) last_cnt += (x >> 30) & 1;
+    }
+    c.own_n[1] = (long long)last_pre + last_cnt;
+    c.own_n[0] = N - c.own_n[1];
+    if (int rc = dalloc(&c.own[0], c.own_n[0])) return rc;
+    if (int rc = dalloc(&c.own[1], c.own_n[1])) return rc;
+    k_owner_scatter<<<nb, kBS, 0, s>>>(N, c.vinfo, d_cnt, c.own[0], c.own[1]);
+    PSI_LAUNCH_CHECK();
+    PSI_HIP(hipStreamSynchronize(s));
+    (void)hipFree(d_vbase);
+    (void)hipFree(d_ebase);
+    (void)hipFree(d_cnt);
+    pt.mark("pack on device");
     return PSI_OK;
 }
 
