@@ -136,6 +136,12 @@ def batch_bench(psi, n_games=1024, seed=3, lo=1000, hi=1000000, check_alone=True
     t3 = time.perf_counter()
     b.close()
     t_batch = time.perf_counter() - t
+    if os.environ.get("PSI_BATCH_PROFILE"):
+        b2 = psi.Solver(games)
+        prof = psi.solve_profiled(b2)
+        log("[bench] batch per-kernel ms/launches: " +
+            ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in prof.items()))
+        b2.close()
     log(f"[bench] batch: batched {t_batch:.2f}s (create+upload {t1 - t:.2f}, solve {t2 - t1:.2f}, "
         f"download {t3 - t2:.2f}, free {t_batch - (t3 - t):.2f})")
     # one at a time

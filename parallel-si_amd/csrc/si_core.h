@@ -82,11 +82,13 @@ struct Core {
     int8_t* solved = nullptr;
     int8_t* lab = nullptr; // infinite[]: -1, 0, +1
     uint8_t* blive = nullptr; // per 256-vertex block: holds a vertex of an unfinished game
+    uint8_t* bact = nullptr;  // 2 x blocks: block holds a vertex still jumping (by round parity)
+    uint8_t* blast = nullptr; // per block: last jump level written this valuation
     int4* Sl = nullptr;    // jump records: level j at Sl + level_slot(j)*N
     int levels = 0;        // buffers allocated
     int* own[2] = {nullptr, nullptr};   // vertices owned by each player
     long long own_n[2] = {0, 0};
-    unsigned long long* Z[2] = {nullptr, nullptr};
+    unsigned long long* Zl = nullptr;  // trace: digests per jump level (as Sl)
     unsigned long long* dig = nullptr;
     GameCtl* gctl = nullptr;
     JumpCnt* jc = nullptr;
@@ -96,6 +98,7 @@ struct Core {
     // host copies of per-game sizes
     std::vector<int> h_n, h_vbase;
     std::vector<long long> h_m;
+    unsigned long long* prof_ties = nullptr;  // profiled solves: compares taking the exact path
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
 };

@@ -165,7 +165,8 @@ __global__ void __launch_bounds__(kBS) k_val_init(
     const int* __restrict__ tgt, const int* __restrict__ gid, const GameCtl* __restrict__ gc,
     int* __restrict__ strat, const int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
     int* __restrict__ par, int4* __restrict__ S0, unsigned long long* __restrict__ Z0,
-    JumpCnt* __restrict__ jc, uint8_t* __restrict__ blive, int trace)
+    JumpCnt* __restrict__ jc, uint8_t* __restrict__ blive, uint8_t* __restrict__ bact0,
+    uint8_t* __restrict__ blast, int trace)
 {
     if (blockIdx.x == 0) {
         unsigned long long* sl = &jc->slot[0][0];
@@ -176,7 +177,7 @@ __global__ void __launch_bounds__(kBS) k_val_init(
         }
     }
     long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int live = 0;
+    int live = 0, jumping = 0;
     if (v < N) {
         int g = gid[v];
         live = !gc[g].done;
@@ -195,23 +196,35 @@ __global__ void __launch_bounds__(kBS) k_val_init(
                 par[v] = s;
                 S0[v] = make_int4(s, pri_of(vi), 1, 0);
                 if (trace) Z0[v] = splitmix64((unsigned long long)(uint32_t)pri_of(vi));
+                jumping = s >= 0;
             }
         }
     }
     // Blocks that hold no vertex of an unfinished game are skipped by the
-    // jump and label kernels of this valuation.
+    // jump and label kernels of this valuation; round 0 visits only blocks
+    // with a vertex that still has to jump.
     int any = __syncthreads_or(live);
-    if (threadIdx.x == 0) blive[blockIdx.x] = (uint8_t)any;
+    int anyj = __syncthreads_or(jumping);
+    if (threadIdx.x == 0) {
+        blive[blockIdx.x] = (uint8_t)any;
+        bact0[blockIdx.x] = (uint8_t)anyj;
+        blast[blockIdx.x] = 0;
+    }
 }
 
-// One pointer-jumping round; grid-stride so that launches after convergence
-// cost a few thousand one-word reads.
-__global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn,
-                                              const int4* __restrict__ Sin, int4* __restrict__ Sout,
-                                              const unsigned long long* __restrict__ Zin,
-                                              unsigned long long* __restrict__ Zout,
+// One pointer-jumping round over the blocks that still hold a jumping vertex
+// (grid-stride, so launches after convergence cost a few thousand one-word
+// reads).  A block that is not visited keeps its last written level; its
+// records there are final, which is where gathers and the label kernel read
+// them (level min(r, blast[block])).  Lifting compares only read a vertex's
+// levels below its resolution level, which its block always wrote.
+__global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int4* __restrict__ Sl,
+                                              unsigned long long* __restrict__ Zl,
                                               JumpCnt* __restrict__ jc,
-                                              const uint8_t* __restrict__ blive, int trace)
+                                              const uint8_t* __restrict__ blive,
+                                              const uint8_t* __restrict__ bact_in,
+                                              uint8_t* __restrict__ bact_out,
+                                              uint8_t* __restrict__ blast, int trace)
 {
     __shared__ int run_s;
     __shared__ unsigned int cnt_res, cnt_act;
@@ -229,30 +242,45 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn,
     }
     __syncthreads();
     if (!run_s) return;
+    const int4* Sin = Sl + (size_t)level_slot(r) * N;
+    int4* Sout = Sl + (size_t)level_slot(r + 1) * N;
+    const unsigned long long* Zin = Zl + (size_t)level_slot(r) * N;
+    unsigned long long* Zout = Zl + (size_t)level_slot(r + 1) * N;
     unsigned my_res = 0, my_act = 0;
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long base = (long long)blockIdx.x * blockDim.x; base < N; base += stride) {
-        if (!blive[base / kBS]) continue;         // whole chunk belongs to finished games
+        const long long b = base / kBS;
+        if (!blive[b] || !bact_in[b]) {           // uniform per block
+            if (threadIdx.x == 0) bact_out[b] = 0;
+            continue;
+        }
         long long v = base + threadIdx.x;
         bool res = false, act = false;
         if (v < N) {
             int4 s = Sin[v];
+            unsigned long long z = trace ? Zin[v] : 0;
             if (s.x >= 0) {
-                int4 t = Sin[s.x];
-                if (trace) Zout[v] = Zin[v] + Zin[s.x];
+                const int ly = min(r, (int)blast[s.x / kBS]);
+                const size_t at = (size_t)level_slot(ly) * N + s.x;
+                int4 t = Sl[at];
+                if (trace) z += Zl[at];
                 s.x = t.x;
                 s.y = max(s.y, t.y);
                 s.z += t.z;
                 res = s.x < 0;
                 act = !res;
-            } else if (trace) {
-                Zout[v] = Zin[v];
             }
             Sout[v] = s;
+            if (trace) Zout[v] = z;
         }
         unsigned long long mr = __ballot(res), ma = __ballot(act);
         my_res += (unsigned)__popcll(mr);
         my_act += (unsigned)__popcll(ma);
+        int any = __syncthreads_or(act);
+        if (threadIdx.x == 0) {
+            bact_out[b] = (uint8_t)any;
+            blast[b] = (uint8_t)(r + 1);
+        }
     }
     if ((threadIdx.x & (kWave - 1)) == 0 && (my_res | my_act)) {
         atomicAdd(&cnt_res, my_res);
@@ -269,7 +297,7 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn,
 // after every valuation.
 __global__ void __launch_bounds__(kBS) k_label(
     long long N, int maxn, int lmax, const int4* __restrict__ Sl,
-    const unsigned long long* __restrict__ Z0, const unsigned long long* __restrict__ Z1,
+    const unsigned long long* __restrict__ Zl, const uint8_t* __restrict__ blast,
     const int* __restrict__ gid, const GameCtl* __restrict__ gc, const int* __restrict__ strat,
     int8_t* __restrict__ solved, int8_t* __restrict__ lab, int* __restrict__ dep,
     int* __restrict__ key, unsigned long long* __restrict__ dig, unsigned long long* __restrict__ tr,
@@ -278,8 +306,10 @@ __global__ void __launch_bounds__(kBS) k_label(
 {
     const int last = jc->last;
     const int F = last + 1;
-    const int4* SF = Sl + (size_t)level_slot(F) * N;
-    const unsigned long long* ZF = (F & 1) ? Z1 : Z0;
+    // this block's final level (it may have stopped jumping earlier)
+    const int Fb = min(F, (int)blast[blockIdx.x]);
+    const int4* SF = Sl + (size_t)level_slot(Fb) * N;
+    const unsigned long long* ZF = Zl + (size_t)level_slot(Fb) * N;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         int stop, cyc, halt;
         if (decide_round(lmax, jc, maxn, &stop, &cyc, &halt)) jc->err = 1;
@@ -310,7 +340,10 @@ __global__ void __launch_bounds__(kBS) k_label(
             } else {
                 if (s.x == -2) l = 1;
                 else if (s.x == -3) l = -1;
-                else l = (SF[s.x].y & 1) ? -1 : 1;   // largest priority on the cycle
+                else {                             // largest priority on the cycle
+                    const int Fy = min(F, (int)blast[s.x / kBS]);
+                    l = (Sl[(size_t)level_slot(Fy) * N + s.x].y & 1) ? -1 : 1;
+                }
                 key[v] = (l + 1) << 30;
                 dep[v] = 0;
             }
@@ -419,7 +452,8 @@ __device__ __noinline__ int cmp_exact(int u, int w, const int* __restrict__ par,
 // keys; -1 is the sink (finite, empty path).
 __device__ __forceinline__ int cmp_keys(int u, int ku, int w, int kw, const int* __restrict__ par,
                                         const int* __restrict__ dep, const int* __restrict__ vinfo,
-                                        const int4* __restrict__ Sl, long long N, int F)
+                                        const int4* __restrict__ Sl, long long N, int F,
+                                        unsigned long long* ties)
 {
     if (u == w) return 0;
     int cu = (int)((unsigned)ku >> 30), cw = (int)((unsigned)kw >> 30);   // +inf keys are negative ints
@@ -431,6 +465,7 @@ __device__ __forceinline__ int cmp_keys(int u, int ku, int w, int kw, const int*
         if (pu > pw) return (pu & 1) ? -1 : 1;
         return (pw & 1) ? 1 : -1;
     }
+    if (ties) atomicAdd(ties, 1ull);
     return cmp_exact(u, w, par, dep, vinfo, Sl, N, F);
 }
 
@@ -450,7 +485,7 @@ __global__ void __launch_bounds__(kBS) k_switch(
     int* __restrict__ strat, int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
     const int* __restrict__ key, const int* __restrict__ dep, const int* __restrict__ par,
     const int4* __restrict__ Sl, const JumpCnt* __restrict__ jc, int loop_form,
-    int* __restrict__ flag)
+    int* __restrict__ flag, unsigned long long* __restrict__ ties)
 {
     const int F = jc->last + 1;                   // levels of the last valuation
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -477,7 +512,7 @@ __global__ void __launch_bounds__(kBS) k_switch(
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
                     if (k < cnt) {
-                        int c = cmp_keys(us[k], ks[k], cur, kcur, par, dep, vinfo, Sl, N, F);
+                        int c = cmp_keys(us[k], ks[k], cur, kcur, par, dep, vinfo, Sl, N, F, ties);
                         if (player == 0 ? c > 0 : c < 0) {
                             cur = us[k];
                             kcur = ks[k];
@@ -771,10 +806,11 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
     if (int rc = dalloc(&c.solved, N)) return rc;
     if (int rc = dalloc(&c.lab, N)) return rc;
     if (int rc = dalloc(&c.blive, nblocks(N, kBS))) return rc;
+    if (int rc = dalloc(&c.bact, 2LL * nblocks(N, kBS))) return rc;
+    if (int rc = dalloc(&c.blast, nblocks(N, kBS))) return rc;
     if (int rc = dalloc(&c.Sl, N * c.levels)) return rc;
     if (trace) {
-        if (int rc = dalloc(&c.Z[0], N)) return rc;
-        if (int rc = dalloc(&c.Z[1], N)) return rc;
+        if (int rc = dalloc(&c.Zl, N * c.levels)) return rc;
         if (int rc = dalloc(&c.dig, N)) return rc;
         if (int rc = dalloc(&c.tr, (long long)G * c.trace_cap * 4)) return rc;
     }
@@ -838,7 +874,7 @@ void core_destroy(Core& c)
 {
     if (c.graph) (void)hipGraphExecDestroy(c.graph);
     void* ptrs[] = {c.vinfo, c.off, c.tgt, c.gid, c.strat, c.par, c.dep, c.key, c.solved, c.lab, c.blive,
-                    c.Sl, c.own[0], c.own[1], c.Z[0], c.Z[1], c.dig, c.tr, c.gctl, c.jc, c.flag};
+                    c.bact, c.blast, c.Sl, c.own[0], c.own[1], c.Zl, c.dig, c.tr, c.gctl, c.jc, c.flag};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c = Core();
@@ -858,19 +894,19 @@ static int maxn_of(const Core& c)
 static void launch_val_init(Core& c)
 {
     k_val_init<<<GRID(c.N)>>>(c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
-                              c.par, c.Sl, c.Z[0], c.jc, c.blive, c.trace);
+                              c.par, c.Sl, c.Zl, c.jc, c.blive, c.bact, c.blast, c.trace);
 }
 
 static void launch_jump(Core& c, int r, int maxn)
 {
-    k_jump<<<JGRID(c.N)>>>(r, c.N, maxn, c.Sl + (size_t)level_slot(r) * c.N,
-                           c.Sl + (size_t)level_slot(r + 1) * c.N,
-                           c.Z[r & 1], c.Z[(r + 1) & 1], c.jc, c.blive, c.trace);
+    const long long nb = nblocks(c.N, kBS);
+    k_jump<<<JGRID(c.N)>>>(r, c.N, maxn, c.Sl, c.Zl, c.jc, c.blive, c.bact + (r & 1) * nb,
+                           c.bact + ((r + 1) & 1) * nb, c.blast, c.trace);
 }
 
 static void launch_label(Core& c, int maxn, bool loop_form)
 {
-    k_label<<<GRID(c.N)>>>(c.N, maxn, c.lmax, c.Sl, c.Z[0], c.Z[1], c.gid, c.gctl, c.strat,
+    k_label<<<GRID(c.N)>>>(c.N, maxn, c.lmax, c.Sl, c.Zl, c.blast, c.gid, c.gctl, c.strat,
                            c.solved, c.lab, c.dep, c.key, c.dig, c.tr, c.trace_cap, loop_form,
                            c.trace, c.jc, c.blive);
 }
@@ -881,7 +917,8 @@ static void launch_switch(Core& c, int player, bool loop_form)
     if (c.own_n[player] == 0) return;
     k_switch<<<GRID(c.own_n[player])>>>(player, c.N, c.own[player], c.own_n[player], c.vinfo,
                                         c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
-                                        c.key, c.dep, c.par, c.Sl, c.jc, loop_form, c.flag);
+                                        c.key, c.dep, c.par, c.Sl, c.jc, loop_form, c.flag,
+                                        c.prof_ties);
 }
 
 int core_set_params(Core& c, int reset, int max_rounds)
@@ -978,6 +1015,10 @@ int core_solve_profiled(Core& c, double* times, long long* counts)
         return (double)ms;
     };
     int rc = PSI_OK;
+    unsigned long long* ties = nullptr;
+    PSI_HIP(hipMalloc(&ties, 8));
+    PSI_HIP(hipMemsetAsync(ties, 0, 8, c.s));
+    c.prof_ties = ties;
     for (int step = 0; step < (1 << 20); step++) {
         int k = 0;
         PSI_HIP(hipEventRecord(ev[k++], c.s));
@@ -1031,6 +1072,11 @@ int core_solve_profiled(Core& c, double* times, long long* counts)
         if (done) break;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
+    unsigned long long h_ties = 0;
+    (void)hipMemcpy(&h_ties, ties, 8, hipMemcpyDeviceToHost);
+    counts[PSI_K_TIES] = (long long)h_ties;
+    c.prof_ties = nullptr;
+    (void)hipFree(ties);
     return rc;
 }
 
