@@ -189,7 +189,7 @@ def main():
     ap.add_argument("--no-listrank", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-games", type=int, default=1024)
-    ap.add_argument("--only", choices=["si", "listrank", "batch"], default=None,
+    ap.add_argument("--only", choices=["si", "listrank", "batch", "copy"], default=None,
                     help="run one workload only (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -201,6 +201,14 @@ def main():
     psi.init()
     if a.only == "listrank":
         print(json.dumps({"listrank": listrank_bench(psi)}), flush=True)
+        return
+    if a.only == "copy":
+        # calibration for hardware byte counters: one device-to-device copy of 1 GiB
+        nbytes = 1 << 30
+        src, dst = psi.DeviceArray(nbytes), psi.DeviceArray(nbytes)
+        psi.check(psi.lib.psi_memcpy_d2d(dst.ptr, src.ptr, nbytes), "d2d")
+        psi.synchronize()
+        print(json.dumps({"copy_bytes": nbytes}), flush=True)
         return
     if a.only == "batch":
         print(json.dumps({"batch": batch_bench(psi, a.batch_games)}), flush=True)
