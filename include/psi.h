@@ -91,6 +91,11 @@ int psi_game_create(int n, int m, const int* owner, const int* priority, const i
                     const int* targets, psi_game** out);
 int psi_game_destroy(psi_game* g);
 int psi_game_info(const psi_game* g, int* n, int* m, int* max_pri);
+int psi_game_csr(const psi_game* g, int* owner, int* priority, int* offsets, int* targets);
+/* Parses PGSolver text (Game::parse_pgsolver, game.cpp:51-107: header line
+ * skipped; "id priority owner s1,s2,...[ label];" per vertex, ids in order
+ * from 0).  Errors return PSI_ERR_INVALID with the reference's message. */
+int psi_game_from_gm(const char* text, size_t len, psi_game** out);
 
 /* ---- solvers --------------------------------------------------------------
  * Replaces GPUParity(Game&, GPUGame&) (gpu_parity.h:8-38) and GPUList

@@ -371,3 +371,13 @@ extern "C" int psi_get_results(psi_solver* s, int* strat_all, int* labels_all, i
     });
     return PSI_OK;
 }
+
+// CSR of a game (the arrays given to psi_game_create); sizes from psi_game_info.
+extern "C" int psi_game_csr(const psi_game* g, int* owner, int* priority, int* offsets, int* targets)
+{
+    std::copy(g->owner.begin(), g->owner.end(), owner);
+    std::copy(g->priority.begin(), g->priority.end(), priority);
+    std::copy(g->offsets.begin(), g->offsets.end(), offsets);
+    std::copy(g->targets.begin(), g->targets.end(), targets);
+    return PSI_OK;
+}

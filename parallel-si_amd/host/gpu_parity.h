@@ -1,0 +1,53 @@
+// MI355X solvers behind the reference's StrategyImprovement interface:
+//   GPUParity(Game&, GPUGame&)  - the reference's gpu mode  (gpu_parity.h:8-38)
+//   GPUList(Game&, GPUGame&)    - the reference's gpulist mode (gpu_list_ranking.h:11-41)
+// Both compute CPUParity's semantics (cpu_parity.cpp), the reference's
+// canonical CPU path, bit-exact: mark_solved freezes solved vertices (the
+// reference's GPUParity::mark_solved is a no-op, gpu_parity.cu:373-376) and
+// every vertex not reaching the sink is labelled by its cycle's largest
+// priority (the reference's GPUList labels them all +inf and can fail to
+// terminate, cpu_list_ranking.cpp:254-285).
+// switch_strategy returns 1 if any vertex switched (the reference's GPU
+// returns this flag too; its CPU returns a count; solve_si only tests 0).
+// solve() runs the whole solve_si loop on the device instead of from the host.
+// Failures throw std::runtime_error with the library's message.
+#ifndef PSI_GPU_PARITY_H
+#define PSI_GPU_PARITY_H
+
+#include "gpu_game.h"
+#include "si.h"
+
+class GPUParity : public StrategyImprovement
+{
+protected:
+    Game& g;
+    GPUGame& g_dev;
+    psi_solver* s = nullptr;
+
+public:
+    GPUParity(Game& g, GPUGame& g_dev);
+    ~GPUParity() override;
+    GPUParity(const GPUParity&) = delete;
+    GPUParity& operator=(const GPUParity&) = delete;
+
+    void init_strat() override;
+    void compute_valuation() override;
+    void mark_solved(int player) override;
+    int switch_strategy(int player) override;
+    void get_winning(std::vector<int>& out, int player) override;
+    void reset_opponent_strategy() override;
+
+    // Device-resident solve_si; returns (iter_count, br_count).
+    void solve(bool reset, int* iter_count, int* br_count);
+    std::vector<int> strategy();
+    std::vector<int> labels();
+    psi_solver* handle() { return s; }
+};
+
+class GPUList : public GPUParity
+{
+public:
+    GPUList(Game& g, GPUGame& g_dev) : GPUParity(g, g_dev) {}
+};
+
+#endif

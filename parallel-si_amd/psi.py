@@ -406,3 +406,23 @@ def jump_stats(solver, rounds=40):
     last, err = C.c_int(), C.c_int()
     check(lib.psi_get_jump_stats(solver.h, out, rounds, C.byref(last), C.byref(err)), "jump_stats")
     return out.reshape(-1, 2), last.value, err.value
+
+_sig("psi_game_from_gm", C.c_int, C.c_char_p, C.c_size_t, _pp)
+_sig("psi_game_csr", C.c_int, _vp, _i32p, _i32p, _i32p, _i32p)
+
+
+def game_from_gm(text):
+    """Parses PGSolver text with the library's loader (the reference grammar).
+    Raises PsiError with the reference's message on malformed input."""
+    b = text.encode() if isinstance(text, str) else text
+    h = _vp()
+    check(lib.psi_game_from_gm(b, len(b), C.byref(h)), "psi_game_from_gm")
+    n, m, mp = C.c_int(), C.c_int(), C.c_int()
+    lib.psi_game_info(h, C.byref(n), C.byref(m), C.byref(mp))
+    owner = np.empty(n.value, np.int32)
+    pri = np.empty(n.value, np.int32)
+    off = np.empty(n.value + 1, np.int32)
+    tgt = np.empty(m.value, np.int32)
+    lib.psi_game_csr(h, owner, pri, off, tgt)
+    lib.psi_game_destroy(h)
+    return Game(owner, pri, off, tgt)
