@@ -177,6 +177,46 @@ int psi_get_jump_stats(psi_solver* s, int* out, int rounds, int* last, int* err)
 int psi_set_state(psi_solver* s, int game, const int* strat, const int* solved,
                   const int* labels);
 
+/* ---- serving pool -------------------------------------------------------
+ * No class of the reference has this role: it serves the GPU solve
+ * (solve_si, gpumain.cpp:18-53, with GPUParity, gpu_parity.cu) to a stream of
+ * requests.  Continuous batching: device arrays of fixed capacity, sized from
+ * an HBM budget; a request is admitted when a control slot and vertex and
+ * edge ranges are free, and joins the SI steps of the games already running;
+ * finished games are copied out and their ranges reused.  Each result equals
+ * the game's solve alone (psi_solver_create + psi_solve).
+ *
+ * Capacity: vertices = budget / (psi_pool_bytes_per_vertex + 4 *
+ * edges_per_vertex), edges = edges_per_vertex * vertices; a game takes n + 1
+ * vertices and m edges.  max_n bounds the admissible game size. */
+typedef struct psi_pool psi_pool;
+int psi_pool_bytes_per_vertex(int max_n, int64_t* bytes);
+int psi_pool_create(int64_t hbm_budget, double edges_per_vertex, int max_games, int max_n,
+                    psi_pool** out);
+int psi_pool_destroy(psi_pool* p);
+/* Queues one game (host CSR arrays, as psi_game_create) and admits it at
+ * once if it fits.  The arrays are read when the request is admitted: they
+ * must stay valid until psi_pool_poll reports state >= 1.  PSI_ERR_INVALID
+ * with the reference's message for a malformed game; PSI_ERR_BUDGET if it can
+ * never fit (n > max_n or beyond the capacities).  reset/max_rounds as
+ * psi_solve.  *ticket names the request. */
+int psi_pool_submit(psi_pool* p, int n, int64_t m, const int* owner, const int* priority,
+                    const int* offsets, const int* targets, int reset, int max_rounds,
+                    int64_t* ticket);
+/* Waits for the step batch in flight, launches the next `steps` SI steps of
+ * every admitted unfinished game, and while they run copies out the games
+ * that finished, frees their ranges and admits queued requests.  The tickets
+ * finished in this call go to done_tickets[0..min(*n_done, cap)). */
+int psi_pool_step(psi_pool* p, int steps, int64_t* done_tickets, int cap, int* n_done);
+/* state: 0 queued, 1 running, 2 finished (result ready), -1 unknown. */
+int psi_pool_poll(psi_pool* p, int64_t ticket, int* state);
+/* Result of a finished request (strategy in the game's own ids, -1 sink;
+ * labels -1/0/+1); frees it.  Pointers may be null. */
+int psi_pool_result(psi_pool* p, int64_t ticket, int* iter, int* br, int* capped, int* strategy,
+                    int* labels);
+int psi_pool_info(psi_pool* p, int* running, int* queued, int64_t* vertex_cap, int64_t* edge_cap,
+                  int64_t* vertices_free, int64_t* largest_free);
+
 /* ---- seeded generators (host only, no device calls) --------------------- */
 /* Random game: owner Bernoulli(1/2), priority U[0,pri_bound), out-degree
  * U[deg_lo,deg_hi] distinct uniform successors; CSR with offsets[n+1]. */

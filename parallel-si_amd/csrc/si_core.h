@@ -93,7 +93,11 @@ struct Core {
     GameCtl* gctl = nullptr;
     JumpCnt* jc = nullptr;
     unsigned long long* tr = nullptr;  // G * trace_cap * 4
-    int* flag = nullptr;               // scratch flag for step-wise switch
+    int* flag = nullptr;               // [0] step-wise switch flag, [1] live count,
+                                       // [2] one past the last occupied vertex
+    bool pool = false;                 // serving pool (games come and go)
+    int maxn_cap = 0;                  // pool: longest admissible game
+    int* cnt_scratch = nullptr;        // pool: owner-list block counts of one admission
     hipStream_t s = nullptr;
     // host copies of per-game sizes
     std::vector<int> h_n, h_vbase;
@@ -110,6 +114,9 @@ struct GameView {
     const int *owner, *pri, *off, *tgt;
 };
 
+// nullptr, or the reason the game is refused (the reference's messages where
+// it has one); *max_pri = largest priority.
+const char* core_validate_game(const GameView& gv, int* max_pri);
 // Builds a core from G games (validated; packed on host threads).
 int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap, hipStream_t s);
 void core_destroy(Core& c);
@@ -125,7 +132,20 @@ int core_switch(Core& c, int player, int* switched_host);   // step-wise, syncs
 int core_enqueue_steps(Core& c, int steps);
 // Runs the loop until every game is done; graph-captured step batches.
 int core_solve(Core& c, int steps_per_check);
+// Launches one graph of `steps` SI steps (captured on first use) on c.s.
+int core_launch_steps(Core& c, int steps);
 int core_all_done(Core& c, int* done);
+// Serving pool: fixed capacity, games admitted and released between steps.
+// Slot G-1 is a permanently finished sentinel that free vertices belong to.
+int core_create_pool(Core& c, long long vcap, long long ecap, int slots, int maxn, hipStream_t s);
+long long core_pool_bytes_per_vertex(int maxn);
+// Places a validated game at vertices [vbase, vbase+n] (the last closes its
+// offsets) and edges [ebase, ebase+m) on stream s2; it joins the steps once
+// its control block is written on the step stream.
+int core_pool_admit(Core& c, int slot, long long vbase, long long ebase, const GameView& gv,
+                    hipStream_t s2);
+int core_pool_localize(Core& c, int slot, hipStream_t s2);
+int core_pool_release(Core& c, int slot, hipStream_t s2);
 int core_solve_profiled(Core& c, double* times, long long* counts);
 
 } // namespace psi

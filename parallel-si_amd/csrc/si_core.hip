@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(kBS) k_val_init(
     int* __restrict__ strat, const int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
     int* __restrict__ par, int4* __restrict__ S0, unsigned long long* __restrict__ Z0,
     JumpCnt* __restrict__ jc, uint8_t* __restrict__ blive, uint8_t* __restrict__ bact0,
-    uint8_t* __restrict__ blast, int trace)
+    uint8_t* __restrict__ blast, int trace, const int* __restrict__ hi)
 {
     if (blockIdx.x == 0) {
         unsigned long long* sl = &jc->slot[0][0];
@@ -176,7 +176,16 @@ __global__ void __launch_bounds__(kBS) k_val_init(
             jc->last = -1;                         // jc->err stays sticky
         }
     }
-    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long base = (long long)blockIdx.x * blockDim.x;
+    if (base >= *hi) {                            // above every occupied vertex (uniform)
+        if (threadIdx.x == 0) {
+            blive[blockIdx.x] = 0;
+            bact0[blockIdx.x] = 0;
+            blast[blockIdx.x] = 0;
+        }
+        return;
+    }
+    long long v = base + threadIdx.x;
     int live = 0, jumping = 0;
     if (v < N) {
         int g = gid[v];
@@ -205,6 +214,10 @@ __global__ void __launch_bounds__(kBS) k_val_init(
     // with a vertex that still has to jump.
     int any = __syncthreads_or(live);
     int anyj = __syncthreads_or(jumping);
+    // Other vertices of a visited block hold a terminal record, so that the
+    // jump rounds never follow what a finished game, or a free range of a
+    // serving pool, left in the level buffers.
+    if (any && !live && v < N) S0[v] = make_int4(-4, -1, 0, 0);
     if (threadIdx.x == 0) {
         blive[blockIdx.x] = (uint8_t)any;
         bact0[blockIdx.x] = (uint8_t)anyj;
@@ -485,14 +498,17 @@ __global__ void __launch_bounds__(kBS) k_switch(
     int* __restrict__ strat, int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
     const int* __restrict__ key, const int* __restrict__ dep, const int* __restrict__ par,
     const int4* __restrict__ Sl, const JumpCnt* __restrict__ jc, int loop_form,
-    int* __restrict__ flag, unsigned long long* __restrict__ ties)
+    int* __restrict__ flag, unsigned long long* __restrict__ ties, const int* __restrict__ hi)
 {
     const int F = jc->last + 1;                   // levels of the last valuation
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // A serving pool's owner lists are indexed like its vertices (holes are
+    // -1), so nothing at or above the occupied bound is a vertex.
+    if ((long long)blockIdx.x * blockDim.x >= *hi) return;
     bool sw = false;
     int g = -1;
-    if (i < own_n) {
-        int v = own[i];
+    int v = i < own_n ? own[i] : -1;
+    if (v >= 0) {
         g = gid[v];
         bool live = !gc[g].done && (!loop_form || player == 1 || gc[g].outer);
         if (live) {
@@ -550,10 +566,11 @@ __global__ void __launch_bounds__(kBS) k_switch(
 // mark_solved(0) of the games that take the outer step (fused form of
 // cpu_parity.cpp:306-315 as solve_si calls it, gpumain.cpp:44).
 __global__ void k_mark0_outer(long long N, const int* __restrict__ gid, const GameCtl* __restrict__ gc,
-                              const int8_t* __restrict__ lab, int8_t* __restrict__ solved)
+                              const int8_t* __restrict__ lab, int8_t* __restrict__ solved,
+                              const int* __restrict__ hi)
 {
     long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= N) return;
+    if (v >= N || v >= *hi) return;
     int g = gid[v];
     if (!gc[g].done && gc[g].outer && lab[v] == 1) solved[v] = 1;
 }
@@ -615,6 +632,21 @@ __global__ void k_count_live(int G, const GameCtl* gc, int* out)
     if (threadIdx.x == 0) *out = acc;
 }
 
+// Serving pool: one past the last vertex of an unfinished game, so that the
+// step's full-range kernels stop there (flag[2]).
+__global__ void k_pool_hi(int G, const GameCtl* __restrict__ gc, int* __restrict__ hi)
+{
+    __shared__ int acc;
+    if (threadIdx.x == 0) acc = 0;
+    __syncthreads();
+    int h = 0;
+    for (int g = threadIdx.x; g < G; g += blockDim.x)
+        if (!gc[g].done) h = max(h, gc[g].vbase + gc[g].n + 1);
+    atomicMax(&acc, h);
+    __syncthreads();
+    if (threadIdx.x == 0) *hi = acc;
+}
+
 template <class T> __global__ void k_fill(T* p, long long n, T v)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -630,7 +662,7 @@ template <class T> static int dalloc(T** p, long long n)
     return PSI_OK;
 }
 
-static const char* validate_game(const GameView& gv, int* max_pri)
+const char* core_validate_game(const GameView& gv, int* max_pri)
 {
     const int n = gv.n;
     const long long m = gv.m;
@@ -770,7 +802,7 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
         return fail(PSI_ERR_INVALID, "total vertices and edges must fit in int32");
     std::vector<const char*> errs(G, nullptr);
     std::vector<int> mp(G, 0);
-    parallel_games(G, [&](int g) { errs[g] = validate_game(games[g], &mp[g]); });
+    parallel_games(G, [&](int g) { errs[g] = core_validate_game(games[g], &mp[g]); });
     for (int g = 0; g < G; g++)
         if (errs[g]) return fail(PSI_ERR_INVALID, errs[g]);
     pt.mark("validate");
@@ -861,6 +893,7 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
     if (int rc = dalloc(&c.own[0], c.own_n[0])) return rc;
     if (int rc = dalloc(&c.own[1], c.own_n[1])) return rc;
     k_owner_scatter<<<nb, kBS, 0, s>>>(N, c.vinfo, d_cnt, c.own[0], c.own[1]);
+    k_fill<int><<<1, 64, 0, s>>>(c.flag + 2, 1, (int)N);   // every vertex is occupied
     PSI_LAUNCH_CHECK();
     PSI_HIP(hipStreamSynchronize(s));
     (void)hipFree(d_vbase);
@@ -870,11 +903,217 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
     return PSI_OK;
 }
 
+// ---- serving pool: games admitted into and released from fixed-capacity
+// device arrays while the SI steps of other games run.  Admission work runs
+// on a second stream; a region only joins the steps when its control block is
+// written on the step stream, between two step batches.  Until then (and
+// after it finishes) its vertices belong to a finished game, which every step
+// kernel skips.
+__global__ void k_pool_pack(int slot, long long vbase, int n, long long ebase, long long m,
+                            int sentinel, const int* __restrict__ own_tmp,
+                            const int* __restrict__ pri_tmp, int* __restrict__ vinfo,
+                            int* __restrict__ off, int* __restrict__ gid)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    long long v = vbase + i;
+    if (i == n) {                                 // the region's closing offset
+        off[v] = (int)(ebase + m);
+        gid[v] = sentinel;
+        return;
+    }
+    vinfo[v] = pri_tmp[v] | (own_tmp[v] << 30);
+    off[v] = (int)(off[v] + ebase);
+    gid[v] = slot;
+}
+
+__global__ void k_pool_targets(long long ebase, long long m, long long vbase, int* __restrict__ tgt)
+{
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < m) tgt[ebase + e] += (int)vbase;
+}
+
+// init_strat of one region (gpu_parity.cu:78-96).
+__global__ void k_pool_init(long long vbase, int n, const int* __restrict__ vinfo,
+                            const int* __restrict__ off, const int* __restrict__ tgt,
+                            int* __restrict__ strat, int8_t* __restrict__ solved,
+                            int8_t* __restrict__ lab)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long long v = vbase + i;
+    strat[v] = owner_of(vinfo[v]) == 0 ? -1 : tgt[off[v]];
+    solved[v] = 0;
+    lab[v] = 0;
+}
+
+// Owner lists of one region, written at the region's own positions:
+// own1[vbase + k] = k-th player-1 vertex, own0 likewise; the rest stay -1.
+__global__ void k_pool_owner_count(long long vbase, int n, const int* __restrict__ vinfo,
+                                   int* __restrict__ cnt1)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int o = i < n ? owner_of(vinfo[vbase + i]) : 0;
+    int c = __syncthreads_count(o);
+    if (threadIdx.x == 0) cnt1[blockIdx.x] = c;
+}
+
+__global__ void k_pool_owner_scatter(long long vbase, int n, const int* __restrict__ vinfo,
+                                     const int* __restrict__ pre1, int* __restrict__ own0,
+                                     int* __restrict__ own1)
+{
+    __shared__ int wsum[kBS / kWave];
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int o = i < n ? owner_of(vinfo[vbase + i]) : 0;
+    unsigned long long m = __ballot(o);
+    int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int before = 0;
+    for (int k = 0; k < w; k++) before += wsum[k];
+    int r1 = pre1[blockIdx.x] + before + __popcll(m & ((1ull << lane) - 1ull));
+    if (i < n) {
+        if (o) own1[vbase + r1] = (int)(vbase + i);
+        else own0[vbase + (i - r1)] = (int)(vbase + i);
+    }
+}
+
+// Strategy of a finished region in the game's own vertex ids.
+__global__ void k_pool_localize(long long vbase, int n, int* __restrict__ strat)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = strat[vbase + i];
+    if (s >= 0) strat[vbase + i] = (int)(s - vbase);
+}
+
+__global__ void k_pool_release(long long vbase, long long cnt, int sentinel, int* __restrict__ gid,
+                               int* __restrict__ own0, int* __restrict__ own1)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    gid[vbase + i] = sentinel;
+    own0[vbase + i] = -1;
+    own1[vbase + i] = -1;
+}
+
+long long core_pool_bytes_per_vertex(int maxn)
+{
+    const int lmax = ceil_log2(maxn) + 2;
+    const int levels = std::min(lmax + 1, kLiftCap + 3);
+    // vinfo off gid strat par dep key own0 own1 (ints), solved lab, records
+    return 9 * 4 + 2 + 16LL * levels + 1;          // + block flags, rounded up
+}
+
+int core_create_pool(Core& c, long long vcap, long long ecap, int slots, int maxn, hipStream_t s)
+{
+    if (vcap <= 1 || ecap <= 0 || slots <= 0 || maxn <= 0)
+        return fail(PSI_ERR_INVALID, "pool: capacities must be positive");
+    if (vcap >= (1LL << 31) - 1 || ecap >= (1LL << 31) - 1)
+        return fail(PSI_ERR_INVALID, "pool: capacities must fit in int32");
+    const long long N = vcap, M = ecap;
+    const int G = slots + 1;                      // the last slot is a finished sentinel
+    c.G = G;
+    c.N = N;
+    c.M = M;
+    c.s = s;
+    c.pool = true;
+    c.maxn_cap = maxn;
+    c.trace = false;
+    c.trace_cap = 0;
+    c.lmax = ceil_log2(maxn) + 2;
+    c.levels = std::min(c.lmax + 1, kLiftCap + 3);
+    c.h_n.assign(G, 0);
+    c.h_m.assign(G, 0);
+    c.h_vbase.assign(G, 0);
+    if (int rc = dalloc(&c.vinfo, N)) return rc;
+    if (int rc = dalloc(&c.off, N + 1)) return rc;
+    if (int rc = dalloc(&c.tgt, M)) return rc;
+    if (int rc = dalloc(&c.gid, N)) return rc;
+    if (int rc = dalloc(&c.strat, N)) return rc;
+    if (int rc = dalloc(&c.par, N)) return rc;
+    if (int rc = dalloc(&c.dep, N)) return rc;
+    if (int rc = dalloc(&c.key, N)) return rc;
+    if (int rc = dalloc(&c.solved, N)) return rc;
+    if (int rc = dalloc(&c.lab, N)) return rc;
+    if (int rc = dalloc(&c.blive, nblocks(N, kBS))) return rc;
+    if (int rc = dalloc(&c.bact, 2LL * nblocks(N, kBS))) return rc;
+    if (int rc = dalloc(&c.blast, nblocks(N, kBS))) return rc;
+    if (int rc = dalloc(&c.Sl, N * c.levels)) return rc;
+    if (int rc = dalloc(&c.own[0], N)) return rc;
+    if (int rc = dalloc(&c.own[1], N)) return rc;
+    if (int rc = dalloc(&c.gctl, G)) return rc;
+    if (int rc = dalloc(&c.jc, 1)) return rc;
+    if (int rc = dalloc(&c.flag, 4)) return rc;
+    if (int rc = dalloc(&c.cnt_scratch, nblocks((long long)maxn + 1, kBS))) return rc;
+    c.own_n[0] = c.own_n[1] = N;
+    std::vector<GameCtl> gc(G);
+    for (auto& x : gc) {
+        std::memset(&x, 0, sizeof x);
+        x.done = 1;
+    }
+    PSI_HIP(hipMemcpyAsync(c.gctl, gc.data(), G * sizeof(GameCtl), hipMemcpyHostToDevice, s));
+    k_fill<int><<<nblocks(N, kBS), kBS, 0, s>>>(c.gid, N, slots);
+    k_fill<int><<<nblocks(N, kBS), kBS, 0, s>>>(c.own[0], N, -1);
+    k_fill<int><<<nblocks(N, kBS), kBS, 0, s>>>(c.own[1], N, -1);
+    k_fill<int><<<1, 64, 0, s>>>(c.flag, 4, 0);
+    const long long jn = (long long)(sizeof(JumpCnt) / 4);
+    k_fill<int><<<nblocks(jn, kBS), kBS, 0, s>>>((int*)c.jc, jn, 0);
+    PSI_LAUNCH_CHECK();
+    PSI_HIP(hipStreamSynchronize(s));
+    return PSI_OK;
+}
+
+int core_pool_admit(Core& c, int slot, long long vbase, long long ebase, const GameView& gv,
+                    hipStream_t s2)
+{
+    const int n = gv.n;
+    const long long m = gv.m;
+    if (slot < 0 || slot >= c.G - 1 || vbase < 0 || vbase + n + 1 > c.N || ebase < 0 || ebase + m > c.M)
+        return fail(PSI_ERR_INVALID, "pool: region out of range");
+    PSI_HIP(hipMemcpyAsync(c.par + vbase, gv.owner, (size_t)n * 4, hipMemcpyHostToDevice, s2));
+    PSI_HIP(hipMemcpyAsync(c.dep + vbase, gv.pri, (size_t)n * 4, hipMemcpyHostToDevice, s2));
+    PSI_HIP(hipMemcpyAsync(c.off + vbase, gv.off, (size_t)n * 4, hipMemcpyHostToDevice, s2));
+    PSI_HIP(hipMemcpyAsync(c.tgt + ebase, gv.tgt, (size_t)m * 4, hipMemcpyHostToDevice, s2));
+    const unsigned nb = nblocks((long long)n, kBS);
+    k_pool_pack<<<nblocks((long long)n + 1, kBS), kBS, 0, s2>>>(slot, vbase, n, ebase, m, c.G - 1,
+                                                               c.par, c.dep, c.vinfo, c.off, c.gid);
+    k_pool_targets<<<nblocks(m, kBS), kBS, 0, s2>>>(ebase, m, vbase, c.tgt);
+    k_pool_init<<<nb, kBS, 0, s2>>>(vbase, n, c.vinfo, c.off, c.tgt, c.strat, c.solved, c.lab);
+    k_pool_owner_count<<<nb, kBS, 0, s2>>>(vbase, n, c.vinfo, c.cnt_scratch);
+    k_owner_scan<<<1, 1024, 0, s2>>>((int)nb, c.cnt_scratch);
+    k_pool_owner_scatter<<<nb, kBS, 0, s2>>>(vbase, n, c.vinfo, c.cnt_scratch, c.own[0], c.own[1]);
+    PSI_LAUNCH_CHECK();
+    c.h_n[slot] = n;
+    c.h_m[slot] = m;
+    c.h_vbase[slot] = (int)vbase;
+    return PSI_OK;
+}
+
+int core_pool_localize(Core& c, int slot, hipStream_t s2)
+{
+    k_pool_localize<<<nblocks((long long)c.h_n[slot], kBS), kBS, 0, s2>>>(c.h_vbase[slot], c.h_n[slot],
+                                                                        c.strat);
+    PSI_LAUNCH_CHECK();
+    return PSI_OK;
+}
+
+int core_pool_release(Core& c, int slot, hipStream_t s2)
+{
+    long long vb = c.h_vbase[slot], cnt = (long long)c.h_n[slot] + 1;
+    k_pool_release<<<nblocks(cnt, kBS), kBS, 0, s2>>>(vb, cnt, c.G - 1, c.gid, c.own[0], c.own[1]);
+    PSI_LAUNCH_CHECK();
+    c.h_n[slot] = 0;
+    c.h_m[slot] = 0;
+    return PSI_OK;
+}
+
 void core_destroy(Core& c)
 {
     if (c.graph) (void)hipGraphExecDestroy(c.graph);
     void* ptrs[] = {c.vinfo, c.off, c.tgt, c.gid, c.strat, c.par, c.dep, c.key, c.solved, c.lab, c.blive,
-                    c.bact, c.blast, c.Sl, c.own[0], c.own[1], c.Zl, c.dig, c.tr, c.gctl, c.jc, c.flag};
+                    c.bact, c.blast, c.Sl, c.own[0], c.own[1], c.Zl, c.dig, c.tr, c.gctl, c.jc, c.flag,
+                    c.cnt_scratch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c = Core();
@@ -886,6 +1125,7 @@ void core_destroy(Core& c)
 
 static int maxn_of(const Core& c)
 {
+    if (c.pool) return c.maxn_cap;               // any admitted game is at most this long
     int m = 0;
     for (int n : c.h_n) m = std::max(m, n);
     return m;
@@ -894,7 +1134,7 @@ static int maxn_of(const Core& c)
 static void launch_val_init(Core& c)
 {
     k_val_init<<<GRID(c.N)>>>(c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
-                              c.par, c.Sl, c.Zl, c.jc, c.blive, c.bact, c.blast, c.trace);
+                              c.par, c.Sl, c.Zl, c.jc, c.blive, c.bact, c.blast, c.trace, c.flag + 2);
 }
 
 static void launch_jump(Core& c, int r, int maxn)
@@ -913,12 +1153,13 @@ static void launch_label(Core& c, int maxn, bool loop_form)
 
 static void launch_switch(Core& c, int player, bool loop_form)
 {
-    if (loop_form && player == 0) k_mark0_outer<<<GRID(c.N)>>>(c.N, c.gid, c.gctl, c.lab, c.solved);
+    if (loop_form && player == 0)
+        k_mark0_outer<<<GRID(c.N)>>>(c.N, c.gid, c.gctl, c.lab, c.solved, c.flag + 2);
     if (c.own_n[player] == 0) return;
     k_switch<<<GRID(c.own_n[player])>>>(player, c.N, c.own[player], c.own_n[player], c.vinfo,
                                         c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
                                         c.key, c.dep, c.par, c.Sl, c.jc, loop_form, c.flag,
-                                        c.prof_ties);
+                                        c.prof_ties, c.flag + 2);
 }
 
 int core_set_params(Core& c, int reset, int max_rounds)
@@ -985,6 +1226,7 @@ int core_switch(Core& c, int player, int* switched_host)
 int core_enqueue_steps(Core& c, int steps)
 {
     for (int k = 0; k < steps; k++) {
+        if (c.pool) k_pool_hi<<<1, 256, 0, c.s>>>(c.G, c.gctl, c.flag + 2);
         if (int rc = core_valuation(c, true)) return rc;
         launch_switch(c, 1, true);
         k_ctl1<<<GRID(c.G)>>>(c.G, c.gctl);
@@ -1093,7 +1335,7 @@ int core_all_done(Core& c, int* done)
 
 // Captures `steps` SI steps once into a graph and replays it until every game
 // is done.  No host round trip inside a batch of steps.
-int core_solve(Core& c, int steps_per_check)
+int core_launch_steps(Core& c, int steps_per_check)
 {
     if (steps_per_check < 1) steps_per_check = 1;
     if (!c.graph || c.graph_steps != steps_per_check) {
@@ -1112,8 +1354,14 @@ int core_solve(Core& c, int steps_per_check)
         if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
         c.graph_steps = steps_per_check;
     }
+    PSI_HIP(hipGraphLaunch(c.graph, c.s));
+    return PSI_OK;
+}
+
+int core_solve(Core& c, int steps_per_check)
+{
     for (;;) {
-        PSI_HIP(hipGraphLaunch(c.graph, c.s));
+        if (int rc = core_launch_steps(c, steps_per_check)) return rc;
         int done = 0;
         if (int rc = core_all_done(c, &done)) return rc;
         if (done) break;

@@ -426,3 +426,103 @@ def game_from_gm(text):
     lib.psi_game_csr(h, owner, pri, off, tgt)
     lib.psi_game_destroy(h)
     return Game(owner, pri, off, tgt)
+
+
+# ---- serving pool ----------------------------------------------------------
+_sig("psi_pool_bytes_per_vertex", C.c_int, C.c_int, C.POINTER(C.c_int64))
+_sig("psi_pool_create", C.c_int, C.c_int64, C.c_double, C.c_int, C.c_int, _pp)
+_sig("psi_pool_destroy", C.c_int, _vp)
+_sig("psi_pool_submit", C.c_int, _vp, C.c_int, C.c_int64, _i32p, _i32p, _i32p, _i32p, C.c_int,
+     C.c_int, C.POINTER(C.c_int64))
+_sig("psi_pool_step", C.c_int, _vp, C.c_int, _i64p, C.c_int, C.POINTER(C.c_int))
+_sig("psi_pool_poll", C.c_int, _vp, C.c_int64, C.POINTER(C.c_int))
+_sig("psi_pool_result", C.c_int, _vp, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int),
+     C.POINTER(C.c_int), _vp, _vp)
+_sig("psi_pool_info", C.c_int, _vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64),
+     C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64))
+
+QUEUED, RUNNING, FINISHED = 0, 1, 2
+
+
+def pool_bytes_per_vertex(max_n):
+    b = C.c_int64()
+    check(lib.psi_pool_bytes_per_vertex(max_n, C.byref(b)), "psi_pool_bytes_per_vertex")
+    return b.value
+
+
+class Pool:
+    """Serving pool (psi_pool_*): submit games, step, collect results.  The
+    pool keeps each submitted Game referenced until it is admitted (the C
+    side reads its arrays then)."""
+
+    def __init__(self, hbm_budget, edges_per_vertex=5.0, max_games=256, max_n=1 << 22):
+        self.h = _vp()
+        self.max_games = max_games
+        check(lib.psi_pool_create(int(hbm_budget), float(edges_per_vertex), max_games, max_n,
+                                  C.byref(self.h)), "psi_pool_create")
+        self._held = {}
+        self._n = {}
+        self._done = np.zeros(max_games, np.int64)
+
+    def close(self):
+        if self.h:
+            lib.psi_pool_destroy(self.h)
+            self.h = _vp()
+            self._held.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, g, reset=False, max_rounds=0):
+        t = C.c_int64()
+        check(lib.psi_pool_submit(self.h, g.n, g.m, g.owner, g.priority, g.offsets, g.targets,
+                                  int(reset), max_rounds, C.byref(t)), "psi_pool_submit")
+        self._held[t.value] = g
+        self._n[t.value] = g.n
+        return t.value
+
+    def step(self, steps=4):
+        """Returns the tickets that finished in this call."""
+        k = C.c_int()
+        check(lib.psi_pool_step(self.h, steps, self._done, self.max_games, C.byref(k)),
+              "psi_pool_step")
+        done = [int(x) for x in self._done[: k.value]]
+        for t in list(self._held):
+            if self.poll(t) >= RUNNING:
+                del self._held[t]
+        return done
+
+    def poll(self, ticket):
+        s = C.c_int()
+        check(lib.psi_pool_poll(self.h, ticket, C.byref(s)), "psi_pool_poll")
+        return s.value
+
+    def result(self, ticket):
+        """((iter, br, capped), strategy, labels); frees the stored result."""
+        n = self._n.pop(ticket)
+        st = np.empty(n, np.int32)
+        lb = np.empty(n, np.int32)
+        it, br, cap = C.c_int(), C.c_int(), C.c_int()
+        check(lib.psi_pool_result(self.h, ticket, C.byref(it), C.byref(br), C.byref(cap),
+                                  st.ctypes.data, lb.ctypes.data), "psi_pool_result")
+        return (it.value, br.value, bool(cap.value)), st, lb
+
+    def info(self):
+        r, q = C.c_int(), C.c_int()
+        vc, ec, vf, lf = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib.psi_pool_info(self.h, C.byref(r), C.byref(q), C.byref(vc), C.byref(ec),
+                                C.byref(vf), C.byref(lf)), "psi_pool_info")
+        return {"running": r.value, "queued": q.value, "vertex_cap": vc.value,
+                "edge_cap": ec.value, "vertices_free": vf.value, "largest_free": lf.value}
+
+    def drain(self, steps=4):
+        """Steps until nothing is queued or running; returns finished tickets."""
+        out = []
+        while True:
+            out += self.step(steps)
+            i = self.info()
+            if i["running"] == 0 and i["queued"] == 0:
+                return out
