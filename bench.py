@@ -12,7 +12,10 @@ One JSON line on stdout.  Besides the contract keys it carries
                  its algorithmic bytes per launch over its HIP-event time;
   cpu_baseline - the oracle's sparse CPUParity restatement (1 core) on the
                  first rounds of the same game (a bounded sample);
-  listrank     - 16,777,216-node list and 1,048,576-list forest rankings.
+  listrank     - 16,777,216-node list and 1,048,576-list forest rankings;
+  batch        - 1,024 mixed-size games in one handle vs one at a time;
+  serve        - Poisson arrivals into the serving pool: p50/p99 latency and
+                 throughput.
 Progress goes to stderr.
 """
 import argparse
@@ -167,6 +170,119 @@ def batch_bench(psi, n_games=1024, seed=3, lo=1000, hi=1000000, check_alone=True
             "batched_equals_alone": bool(same)}
 
 
+def serve_bench(psi, requests=96, rate=2.0, seed=4, budget_gib=64.0, steps=4, lookahead=64):
+    """Serving: `requests` games arrive as a seeded Poisson stream of `rate`
+    per second.  Request k: with probability 1/5 a chain game, else a random
+    game of the 4M-game generator (priority U[0, 2^20), out-degree U[2, 8]);
+    n log-uniform in [10^4, 4,194,304]; game seed = seed * 100000 + k.  Games
+    are generated ahead on host threads (outside the timing; a request whose
+    game is not ready at its arrival time waits, and the wait counts in its
+    latency).  Latency = harvest time - arrival time.  A sample of requests is
+    solved again alone afterwards and compared bit for bit."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    sizes = psi.gen_sizes(seed, requests, 1e4, 4194304)
+    rng = np.random.default_rng(seed)
+    chain = rng.random(requests) < 0.2
+    arrival = np.cumsum(rng.exponential(1.0 / rate, requests))
+    arrival -= arrival[0]
+
+    def make(k):
+        n = int(sizes[k])
+        if chain[k]:
+            return psi.gen_chain_game(seed * 100000 + k, n)
+        return psi.gen_random_game(seed * 100000 + k, n, 1 << 20, 2, 8)
+
+    ex = ThreadPoolExecutor(16)
+    futs = [None] * requests
+    made = 0
+
+    def top_up(upto):
+        nonlocal made
+        while made < min(requests, upto):
+            futs[made] = ex.submit(make, made)
+            made += 1
+
+    top_up(lookahead)
+    pool = psi.Pool(int(budget_gib * (1 << 30)), edges_per_vertex=5.0, max_games=256,
+                    max_n=4194304)
+    # warm-up: graph capture and first launches on a tiny game
+    pool.submit(psi.gen_random_game(1, 1000, 100, 1, 4))
+    pool.drain(steps)
+    for f in futs[:min(lookahead, requests)]:
+        f.result()
+    order = np.argsort(sizes)
+    sample = set([int(k) for k in order if not chain[k]][:6] +
+                 [int(k) for k in order if chain[k]][:3] +
+                 [int(k) for k in order[::-1] if not chain[k]][:1])
+    kept, n_of, br_of, finish, ticket_of = {}, {}, {}, {}, {}
+    sub, late, max_running, max_queued, n_steps = 0, 0, 0, 0, 0
+    t0 = time.perf_counter()
+    t_log = 0.0
+    while len(finish) < requests:
+        now = time.perf_counter() - t0
+        while sub < requests and arrival[sub] <= now:
+            if not futs[sub].done():
+                late += 1
+                futs[sub].result()
+            g = futs[sub].result()
+            futs[sub] = None
+            ticket_of[pool.submit(g)] = sub
+            n_of[sub] = g.n
+            if sub in sample:
+                kept[sub] = g
+            sub += 1
+            top_up(sub + lookahead)
+        done = pool.step(steps)
+        n_steps += 1
+        tn = time.perf_counter() - t0
+        for t in done:
+            k = ticket_of.pop(t)
+            finish[k] = tn
+            stats, st, lb = pool.result(t)
+            br_of[k] = stats[1]
+            if k in kept:
+                kept[k] = (kept[k], stats, st, lb)
+        info = pool.info()
+        if tn - t_log > 10.0:
+            t_log = tn
+            log(f"[bench] serve: t={tn:.0f}s submitted {sub} finished {len(finish)} "
+                f"running {info['running']} queued {info['queued']}")
+        max_running = max(max_running, info["running"])
+        max_queued = max(max_queued, info["queued"])
+        if not done and info["running"] == 0 and info["queued"] == 0 and sub < requests:
+            time.sleep(max(0.0, arrival[sub] - (time.perf_counter() - t0)))
+    makespan = max(finish.values())
+    pool.close()
+    ex.shutdown()
+    lat = np.array([finish[k] - arrival[k] for k in range(requests)])
+    same = True
+    for k, (g, stats, st, lb) in kept.items():
+        s = psi.Solver(g)
+        s.solve()
+        same &= stats == s.stats() and np.array_equal(st, s.strategy()) and \
+            np.array_equal(lb, s.labels())
+        s.close()
+    vit = sum(n_of[k] * br_of[k] for k in range(requests))
+    lat_c = lat[chain]
+    lat_r = lat[~chain]
+    return {"requests": requests, "rate_per_s": rate, "seed": seed, "chain_requests": int(chain.sum()),
+            "sum_n": int(sizes.sum()), "budget_gib": budget_gib, "steps_per_sync": steps,
+            "makespan_s": round(makespan, 3),
+            "throughput_req_per_s": round(requests / makespan, 3),
+            "vertex_iterations_per_s": round(vit / makespan, 1),
+            "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+            "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
+            "latency_mean_s": round(float(lat.mean()), 4),
+            "latency_p50_random_s": round(float(np.percentile(lat_r, 50)), 4) if len(lat_r) else None,
+            "latency_p99_random_s": round(float(np.percentile(lat_r, 99)), 4) if len(lat_r) else None,
+            "latency_p50_chain_s": round(float(np.percentile(lat_c, 50)), 4) if len(lat_c) else None,
+            "latency_p99_chain_s": round(float(np.percentile(lat_c, 99)), 4) if len(lat_c) else None,
+            "br_max": int(max(br_of.values())), "max_running": max_running,
+            "max_queued": max_queued, "late_generation": late, "pool_steps": n_steps,
+            "served_equals_alone": bool(same), "checked_alone": len(kept)}
+
+
 def cpu_baseline(g, max_rounds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -189,7 +305,10 @@ def main():
     ap.add_argument("--no-listrank", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-games", type=int, default=1024)
-    ap.add_argument("--only", choices=["si", "listrank", "batch", "copy"], default=None,
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--requests", type=int, default=96, help="serving: requests")
+    ap.add_argument("--rate", type=float, default=2.0, help="serving: arrivals per second")
+    ap.add_argument("--only", choices=["si", "listrank", "batch", "serve", "copy"], default=None,
                     help="run one workload only (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -213,8 +332,11 @@ def main():
     if a.only == "batch":
         print(json.dumps({"batch": batch_bench(psi, a.batch_games)}), flush=True)
         return
+    if a.only == "serve":
+        print(json.dumps({"serve": serve_bench(psi, a.requests, a.rate)}), flush=True)
+        return
     if a.only == "si":
-        a.no_listrank = a.no_cpu = a.no_batch = True
+        a.no_listrank = a.no_cpu = a.no_batch = a.no_serve = True
     g, s, it, br, wall, prof = si_bench(psi, a.n, a.steps, a.warmup, a.seed)
     value = g.n * br * a.steps / wall
     # Roofline of the dominant kernel: a pointer-jumping round reads and
@@ -254,6 +376,9 @@ def main():
         line["listrank"] = listrank_bench(psi)
     if not a.no_batch:
         line["batch"] = batch_bench(psi, a.batch_games)
+    if not a.no_serve:
+        log("[bench] serving ...")
+        line["serve"] = serve_bench(psi, a.requests, a.rate)
     if not a.no_cpu:
         log("[bench] cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(g, a.cpu_rounds)

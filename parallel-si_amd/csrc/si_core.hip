@@ -176,52 +176,50 @@ __global__ void __launch_bounds__(kBS) k_val_init(
             jc->last = -1;                         // jc->err stays sticky
         }
     }
-    const long long base = (long long)blockIdx.x * blockDim.x;
-    if (base >= *hi) {                            // above every occupied vertex (uniform)
+    // Vertices at or above *hi belong to no unfinished game; no kernel of the
+    // step visits their blocks.
+    const long long lim = min(N, (long long)*hi);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < lim; base += stride) {
+        const long long b = base / kBS;
+        const long long v = base + threadIdx.x;
+        int live = 0, jumping = 0;
+        if (v < N) {
+            int g = gid[v];
+            live = !gc[g].done;
+            if (live) {
+                int vi = vinfo[v];
+                int s = strat[v];
+                if (gc[g].reset_pending && owner_of(vi) == 1) {   // main.cpp:25-26
+                    s = tgt[off[v]];
+                    strat[v] = s;
+                }
+                if (solved[v]) {
+                    S0[v] = make_int4(lab[v] == 1 ? -2 : -3, -1, 0, 0);
+                    par[v] = -1;
+                    if (trace) Z0[v] = 0;
+                } else {
+                    par[v] = s;
+                    S0[v] = make_int4(s, pri_of(vi), 1, 0);
+                    if (trace) Z0[v] = splitmix64((unsigned long long)(uint32_t)pri_of(vi));
+                    jumping = s >= 0;
+                }
+            }
+        }
+        // Blocks that hold no vertex of an unfinished game are skipped by the
+        // jump and label kernels of this valuation; round 0 visits only
+        // blocks with a vertex that still has to jump.
+        int any = __syncthreads_or(live);
+        int anyj = __syncthreads_or(jumping);
+        // Other vertices of a visited block hold a terminal record, so that
+        // the jump rounds never follow what a finished game, or a free range
+        // of a serving pool, left in the level buffers.
+        if (any && !live && v < N) S0[v] = make_int4(-4, -1, 0, 0);
         if (threadIdx.x == 0) {
-            blive[blockIdx.x] = 0;
-            bact0[blockIdx.x] = 0;
-            blast[blockIdx.x] = 0;
+            blive[b] = (uint8_t)any;
+            bact0[b] = (uint8_t)anyj;
+            blast[b] = 0;
         }
-        return;
-    }
-    long long v = base + threadIdx.x;
-    int live = 0, jumping = 0;
-    if (v < N) {
-        int g = gid[v];
-        live = !gc[g].done;
-        if (live) {
-            int vi = vinfo[v];
-            int s = strat[v];
-            if (gc[g].reset_pending && owner_of(vi) == 1) {   // main.cpp:25-26
-                s = tgt[off[v]];
-                strat[v] = s;
-            }
-            if (solved[v]) {
-                S0[v] = make_int4(lab[v] == 1 ? -2 : -3, -1, 0, 0);
-                par[v] = -1;
-                if (trace) Z0[v] = 0;
-            } else {
-                par[v] = s;
-                S0[v] = make_int4(s, pri_of(vi), 1, 0);
-                if (trace) Z0[v] = splitmix64((unsigned long long)(uint32_t)pri_of(vi));
-                jumping = s >= 0;
-            }
-        }
-    }
-    // Blocks that hold no vertex of an unfinished game are skipped by the
-    // jump and label kernels of this valuation; round 0 visits only blocks
-    // with a vertex that still has to jump.
-    int any = __syncthreads_or(live);
-    int anyj = __syncthreads_or(jumping);
-    // Other vertices of a visited block hold a terminal record, so that the
-    // jump rounds never follow what a finished game, or a free range of a
-    // serving pool, left in the level buffers.
-    if (any && !live && v < N) S0[v] = make_int4(-4, -1, 0, 0);
-    if (threadIdx.x == 0) {
-        blive[blockIdx.x] = (uint8_t)any;
-        bact0[blockIdx.x] = (uint8_t)anyj;
-        blast[blockIdx.x] = 0;
     }
 }
 
@@ -237,7 +235,8 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int4
                                               const uint8_t* __restrict__ blive,
                                               const uint8_t* __restrict__ bact_in,
                                               uint8_t* __restrict__ bact_out,
-                                              uint8_t* __restrict__ blast, int trace)
+                                              uint8_t* __restrict__ blast, int trace,
+                                              const int* __restrict__ hi)
 {
     __shared__ int run_s;
     __shared__ unsigned int cnt_res, cnt_act;
@@ -261,7 +260,8 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int4
     unsigned long long* Zout = Zl + (size_t)level_slot(r + 1) * N;
     unsigned my_res = 0, my_act = 0;
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long base = (long long)blockIdx.x * blockDim.x; base < N; base += stride) {
+    const long long lim = min(N, (long long)*hi);
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < lim; base += stride) {
         const long long b = base / kBS;
         if (!blive[b] || !bact_in[b]) {           // uniform per block
             if (threadIdx.x == 0) bact_out[b] = 0;
@@ -315,66 +315,71 @@ __global__ void __launch_bounds__(kBS) k_label(
     int8_t* __restrict__ solved, int8_t* __restrict__ lab, int* __restrict__ dep,
     int* __restrict__ key, unsigned long long* __restrict__ dig, unsigned long long* __restrict__ tr,
     int trace_cap, int loop_form, int trace, JumpCnt* __restrict__ jc,
-    const uint8_t* __restrict__ blive)
+    const uint8_t* __restrict__ blive, const int* __restrict__ hi)
 {
     const int last = jc->last;
     const int F = last + 1;
-    // this block's final level (it may have stopped jumping earlier)
-    const int Fb = min(F, (int)blast[blockIdx.x]);
-    const int4* SF = Sl + (size_t)level_slot(Fb) * N;
-    const unsigned long long* ZF = Zl + (size_t)level_slot(Fb) * N;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         int stop, cyc, halt;
         if (decide_round(lmax, jc, maxn, &stop, &cyc, &halt)) jc->err = 1;
     }
-    if (!blive[blockIdx.x]) return;               // uniform: the whole block is finished
-    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int g = -1, l = 0, sol = 0, loc = 0;
-    unsigned long long d = 0;
-    bool live = false;
-    if (v < N) {
-        g = gid[v];
-        live = !gc[g].done;
-    }
-    if (live) {
-        sol = solved[v];
-        loc = (int)(v - gc[g].vbase);
-        if (sol) {
-            l = lab[v];
-            key[v] = (l + 1) << 30;
-            dep[v] = 0;
-        } else {
-            int4 s = SF[v];
-            if (s.x == -1) {
-                l = 0;
-                dep[v] = s.z;
-                key[v] = (1 << 30) | s.y;
-                if (trace) d = ZF[v];
-            } else {
-                if (s.x == -2) l = 1;
-                else if (s.x == -3) l = -1;
-                else {                             // largest priority on the cycle
-                    const int Fy = min(F, (int)blast[s.x / kBS]);
-                    l = (Sl[(size_t)level_slot(Fy) * N + s.x].y & 1) ? -1 : 1;
-                }
+    const long long lim = min(N, (long long)*hi);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long vb = (long long)blockIdx.x * blockDim.x; vb < lim; vb += stride) {
+        const long long b = vb / kBS;
+        if (!blive[b]) continue;                  // uniform: the whole block is finished
+        // this block's final level (it may have stopped jumping earlier)
+        const int Fb = min(F, (int)blast[b]);
+        const int4* SF = Sl + (size_t)level_slot(Fb) * N;
+        const unsigned long long* ZF = Zl + (size_t)level_slot(Fb) * N;
+        const long long v = vb + threadIdx.x;
+        int g = -1, l = 0, sol = 0, loc = 0;
+        unsigned long long d = 0;
+        bool live = false;
+        if (v < N) {
+            g = gid[v];
+            live = !gc[g].done;
+        }
+        if (live) {
+            sol = solved[v];
+            loc = (int)(v - gc[g].vbase);
+            if (sol) {
+                l = lab[v];
                 key[v] = (l + 1) << 30;
                 dep[v] = 0;
+            } else {
+                int4 s = SF[v];
+                if (s.x == -1) {
+                    l = 0;
+                    dep[v] = s.z;
+                    key[v] = (1 << 30) | s.y;
+                    if (trace) d = ZF[v];
+                } else {
+                    if (s.x == -2) l = 1;
+                    else if (s.x == -3) l = -1;
+                    else {                         // largest priority on the cycle
+                        const int Fy = min(F, (int)blast[s.x / kBS]);
+                        l = (Sl[(size_t)level_slot(Fy) * N + s.x].y & 1) ? -1 : 1;
+                    }
+                    key[v] = (l + 1) << 30;
+                    dep[v] = 0;
+                }
+                lab[v] = (int8_t)l;
+                if (loop_form && l == -1) solved[v] = 1;
             }
-            lab[v] = (int8_t)l;
-            if (loop_form && l == -1) solved[v] = 1;
+            if (trace) dig[v] = d;
         }
-        if (trace) dig[v] = d;
-    }
-    if (trace) {
-        int br = live ? gc[g].br : 0;
-        bool rec = live && br < trace_cap;
-        unsigned long long* base = rec ? tr + ((size_t)g * trace_cap + br) * 4 : nullptr;
-        int sv = rec ? strat[v] : -1;
-        if (sv >= 0) sv -= gc[g].vbase;                  // local ids, as a game alone
-        wave_add_u64(rec ? base + 0 : nullptr, rec ? hmix_i32(loc, sv) : 0);
-        wave_add_u64(rec ? base + 1 : nullptr, rec ? hmix_i32(loc, sol) : 0);
-        wave_add_u64(rec ? base + 2 : nullptr, rec ? hmix_i32(loc, l) : 0);
-        wave_add_u64(rec ? base + 3 : nullptr, rec ? hmix_u64(loc, d) : 0);
+        if (trace) {
+            int br = live ? gc[g].br : 0;
+            bool rec = live && br < trace_cap;
+            unsigned long long* base = rec ? tr + ((size_t)g * trace_cap + br) * 4 : nullptr;
+            int sv = rec ? strat[v] : -1;
+            if (sv >= 0) sv -= gc[g].vbase;              // local ids, as a game alone
+            wave_add_u64(rec ? base + 0 : nullptr, rec ? hmix_i32(loc, sv) : 0);
+            wave_add_u64(rec ? base + 1 : nullptr, rec ? hmix_i32(loc, sol) : 0);
+            wave_add_u64(rec ? base + 2 : nullptr, rec ? hmix_i32(loc, l) : 0);
+            wave_add_u64(rec ? base + 3 : nullptr, rec ? hmix_u64(loc, d) : 0);
+        }
     }
 }
 
@@ -501,63 +506,67 @@ __global__ void __launch_bounds__(kBS) k_switch(
     int* __restrict__ flag, unsigned long long* __restrict__ ties, const int* __restrict__ hi)
 {
     const int F = jc->last + 1;                   // levels of the last valuation
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     // A serving pool's owner lists are indexed like its vertices (holes are
     // -1), so nothing at or above the occupied bound is a vertex.
-    if ((long long)blockIdx.x * blockDim.x >= *hi) return;
-    bool sw = false;
-    int g = -1;
-    int v = i < own_n ? own[i] : -1;
-    if (v >= 0) {
-        g = gid[v];
-        bool live = !gc[g].done && (!loop_form || player == 1 || gc[g].outer);
-        if (live) {
-            int cur = strat[v];
-            int kcur = key_of(key, cur);
-            const int e1 = off[v + 1];
-            // Successors in batches of 8: the target and key loads of a batch are
-            // independent, so they are in flight together; the scan itself is
-            // sequential in CSR order (the reference's tie-break).
-            for (int e0 = off[v]; e0 < e1; e0 += 8) {
-                const int cnt = min(8, e1 - e0);
-                int us[8], ks[8];
+    const long long lim = min(own_n, (long long)*hi);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long ib = (long long)blockIdx.x * blockDim.x; ib < lim; ib += stride) {
+        const long long i = ib + threadIdx.x;
+        bool sw = false;
+        int g = -1;
+        const int v = i < lim ? own[i] : -1;
+        if (v >= 0) {
+            g = gid[v];
+            bool live = !gc[g].done && (!loop_form || player == 1 || gc[g].outer);
+            if (live) {
+                int cur = strat[v];
+                int kcur = key_of(key, cur);
+                const int e1 = off[v + 1];
+                // Successors in batches of 8: the target and key loads of a
+                // batch are independent, so they are in flight together; the
+                // scan itself is sequential in CSR order (the reference's
+                // tie-break).
+                for (int e0 = off[v]; e0 < e1; e0 += 8) {
+                    const int cnt = min(8, e1 - e0);
+                    int us[8], ks[8];
 #pragma unroll
-                for (int k = 0; k < 8; k++) us[k] = k < cnt ? tgt[e0 + k] : -1;
+                    for (int k = 0; k < 8; k++) us[k] = k < cnt ? tgt[e0 + k] : -1;
 #pragma unroll
-                for (int k = 0; k < 8; k++) ks[k] = k < cnt ? key[us[k]] : 0;
+                    for (int k = 0; k < 8; k++) ks[k] = k < cnt ? key[us[k]] : 0;
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    if (k < cnt) {
-                        int c = cmp_keys(us[k], ks[k], cur, kcur, par, dep, vinfo, Sl, N, F, ties);
-                        if (player == 0 ? c > 0 : c < 0) {
-                            cur = us[k];
-                            kcur = ks[k];
-                            sw = true;
+                    for (int k = 0; k < 8; k++) {
+                        if (k < cnt) {
+                            int c = cmp_keys(us[k], ks[k], cur, kcur, par, dep, vinfo, Sl, N, F, ties);
+                            if (player == 0 ? c > 0 : c < 0) {
+                                cur = us[k];
+                                kcur = ks[k];
+                                sw = true;
+                            }
                         }
                     }
                 }
+                if (sw) strat[v] = cur;
             }
-            if (sw) strat[v] = cur;
         }
-    }
-    // one store per wave and game: the flag only has to become nonzero
-    unsigned long long m = __ballot(sw);
-    if (m) {
-        int g0 = __shfl(g, __ffsll((long long)m) - 1);
-        bool uni = __all(!sw || g == g0);
-        if (uni) {
-            if ((threadIdx.x & (kWave - 1)) == (unsigned)(__ffsll((long long)m) - 1)) {
+        // one store per wave and game: the flag only has to become nonzero
+        unsigned long long m = __ballot(sw);
+        if (m) {
+            int g0 = __shfl(g, __ffsll((long long)m) - 1);
+            bool uni = __all(!sw || g == g0);
+            if (uni) {
+                if ((threadIdx.x & (kWave - 1)) == (unsigned)(__ffsll((long long)m) - 1)) {
+                    if (loop_form) {
+                        if (player == 1) gc[g0].sw1 = 1; else gc[g0].sw0 = 1;
+                    } else {
+                        *flag = 1;
+                    }
+                }
+            } else if (sw) {
                 if (loop_form) {
-                    if (player == 1) gc[g0].sw1 = 1; else gc[g0].sw0 = 1;
+                    if (player == 1) gc[g].sw1 = 1; else gc[g].sw0 = 1;
                 } else {
                     *flag = 1;
                 }
-            }
-        } else if (sw) {
-            if (loop_form) {
-                if (player == 1) gc[g].sw1 = 1; else gc[g].sw0 = 1;
-            } else {
-                *flag = 1;
             }
         }
     }
@@ -569,10 +578,12 @@ __global__ void k_mark0_outer(long long N, const int* __restrict__ gid, const Ga
                               const int8_t* __restrict__ lab, int8_t* __restrict__ solved,
                               const int* __restrict__ hi)
 {
-    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= N || v >= *hi) return;
-    int g = gid[v];
-    if (!gc[g].done && gc[g].outer && lab[v] == 1) solved[v] = 1;
+    const long long lim = min(N, (long long)*hi);
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < lim; v += stride) {
+        int g = gid[v];
+        if (!gc[g].done && gc[g].outer && lab[v] == 1) solved[v] = 1;
+    }
 }
 
 // mark_solved(player) as a separate step (cpu_parity.cpp:306-315).
@@ -1122,6 +1133,9 @@ void core_destroy(Core& c)
 #define GRID(n) nblocks((n), kBS), kBS, 0, c.s
 // Jump rounds: capped grid (grid-stride inside) so idle launches stay cheap.
 #define JGRID(n) std::min(nblocks((n), kBS), 4096u), kBS, 0, c.s
+// Step kernels: capped grid-stride launches, so that a serving pool's large
+// capacity costs nothing above its occupied bound.
+#define SGRID(n) std::min(nblocks((n), kBS), c.pool ? 4096u : 16384u), kBS, 0, c.s
 
 static int maxn_of(const Core& c)
 {
@@ -1133,7 +1147,7 @@ static int maxn_of(const Core& c)
 
 static void launch_val_init(Core& c)
 {
-    k_val_init<<<GRID(c.N)>>>(c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
+    k_val_init<<<SGRID(c.N)>>>(c.N, c.vinfo, c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
                               c.par, c.Sl, c.Zl, c.jc, c.blive, c.bact, c.blast, c.trace, c.flag + 2);
 }
 
@@ -1141,22 +1155,22 @@ static void launch_jump(Core& c, int r, int maxn)
 {
     const long long nb = nblocks(c.N, kBS);
     k_jump<<<JGRID(c.N)>>>(r, c.N, maxn, c.Sl, c.Zl, c.jc, c.blive, c.bact + (r & 1) * nb,
-                           c.bact + ((r + 1) & 1) * nb, c.blast, c.trace);
+                           c.bact + ((r + 1) & 1) * nb, c.blast, c.trace, c.flag + 2);
 }
 
 static void launch_label(Core& c, int maxn, bool loop_form)
 {
-    k_label<<<GRID(c.N)>>>(c.N, maxn, c.lmax, c.Sl, c.Zl, c.blast, c.gid, c.gctl, c.strat,
+    k_label<<<SGRID(c.N)>>>(c.N, maxn, c.lmax, c.Sl, c.Zl, c.blast, c.gid, c.gctl, c.strat,
                            c.solved, c.lab, c.dep, c.key, c.dig, c.tr, c.trace_cap, loop_form,
-                           c.trace, c.jc, c.blive);
+                           c.trace, c.jc, c.blive, c.flag + 2);
 }
 
 static void launch_switch(Core& c, int player, bool loop_form)
 {
     if (loop_form && player == 0)
-        k_mark0_outer<<<GRID(c.N)>>>(c.N, c.gid, c.gctl, c.lab, c.solved, c.flag + 2);
+        k_mark0_outer<<<SGRID(c.N)>>>(c.N, c.gid, c.gctl, c.lab, c.solved, c.flag + 2);
     if (c.own_n[player] == 0) return;
-    k_switch<<<GRID(c.own_n[player])>>>(player, c.N, c.own[player], c.own_n[player], c.vinfo,
+    k_switch<<<SGRID(c.own_n[player])>>>(player, c.N, c.own[player], c.own_n[player], c.vinfo,
                                         c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
                                         c.key, c.dep, c.par, c.Sl, c.jc, loop_form, c.flag,
                                         c.prof_ties, c.flag + 2);
