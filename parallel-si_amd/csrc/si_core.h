@@ -76,7 +76,7 @@ struct Core {
     int* gid = nullptr;    // N game of each vertex
     // state (device)
     int* strat = nullptr;
-    int* par = nullptr;    // strategy snapshot of the last valuation (-1 sink)
+    int* par = nullptr;    // link at the last valuation: successor, -1 sink, -2/-3 solved +inf/-inf
     int* dep = nullptr;    // path length to the sink of finite vertices
     int* key = nullptr;    // (label+1) << 30 | max priority on the path
     int8_t* solved = nullptr;
@@ -85,6 +85,7 @@ struct Core {
     uint8_t* bact = nullptr;  // 2 x blocks: block holds a vertex still jumping (by round parity)
     uint8_t* blast = nullptr; // per block: last jump level written this valuation
     int4* Sl = nullptr;    // jump records: level j at Sl + level_slot(j)*N
+    unsigned long long* chg = nullptr;  // bit per vertex: valuation changed at the last valuation
     int levels = 0;        // buffers allocated
     int* own[2] = {nullptr, nullptr};   // vertices owned by each player
     long long own_n[2] = {0, 0};

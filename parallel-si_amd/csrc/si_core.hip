@@ -194,13 +194,18 @@ __global__ void __launch_bounds__(kBS) k_val_init(
                     s = tgt[off[v]];
                     strat[v] = s;
                 }
+                // w = 1 if this vertex's link differs from the previous
+                // valuation's (the OR along a path says the path changed).
+                const int old = par[v];
+                const int first = gc[g].br == 0;
                 if (solved[v]) {
-                    S0[v] = make_int4(lab[v] == 1 ? -2 : -3, -1, 0, 0);
-                    par[v] = -1;
+                    const int code = lab[v] == 1 ? -2 : -3;
+                    S0[v] = make_int4(code, -1, 0, first | (old != code));
+                    par[v] = code;
                     if (trace) Z0[v] = 0;
                 } else {
                     par[v] = s;
-                    S0[v] = make_int4(s, pri_of(vi), 1, 0);
+                    S0[v] = make_int4(s, pri_of(vi), 1, first | (old != s));
                     if (trace) Z0[v] = splitmix64((unsigned long long)(uint32_t)pri_of(vi));
                     jumping = s >= 0;
                 }
@@ -280,6 +285,7 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int4
                 s.x = t.x;
                 s.y = max(s.y, t.y);
                 s.z += t.z;
+                s.w |= t.w;
                 res = s.x < 0;
                 act = !res;
             }
@@ -315,7 +321,8 @@ __global__ void __launch_bounds__(kBS) k_label(
     int8_t* __restrict__ solved, int8_t* __restrict__ lab, int* __restrict__ dep,
     int* __restrict__ key, unsigned long long* __restrict__ dig, unsigned long long* __restrict__ tr,
     int trace_cap, int loop_form, int trace, JumpCnt* __restrict__ jc,
-    const uint8_t* __restrict__ blive, const int* __restrict__ hi)
+    const uint8_t* __restrict__ blive, const int* __restrict__ hi,
+    unsigned long long* __restrict__ chg)
 {
     const int last = jc->last;
     const int F = last + 1;
@@ -335,7 +342,7 @@ __global__ void __launch_bounds__(kBS) k_label(
         const long long v = vb + threadIdx.x;
         int g = -1, l = 0, sol = 0, loc = 0;
         unsigned long long d = 0;
-        bool live = false;
+        bool live = false, ch = false;
         if (v < N) {
             g = gid[v];
             live = !gc[g].done;
@@ -343,8 +350,9 @@ __global__ void __launch_bounds__(kBS) k_label(
         if (live) {
             sol = solved[v];
             loc = (int)(v - gc[g].vbase);
+            const int lold = lab[v];
             if (sol) {
-                l = lab[v];
+                l = lold;
                 key[v] = (l + 1) << 30;
                 dep[v] = 0;
             } else {
@@ -353,6 +361,7 @@ __global__ void __launch_bounds__(kBS) k_label(
                     l = 0;
                     dep[v] = s.z;
                     key[v] = (1 << 30) | s.y;
+                    ch = s.w != 0;                 // a link on the path changed
                     if (trace) d = ZF[v];
                 } else {
                     if (s.x == -2) l = 1;
@@ -364,11 +373,15 @@ __global__ void __launch_bounds__(kBS) k_label(
                     key[v] = (l + 1) << 30;
                     dep[v] = 0;
                 }
+                ch = ch || l != lold;
                 lab[v] = (int8_t)l;
                 if (loop_form && l == -1) solved[v] = 1;
             }
             if (trace) dig[v] = d;
         }
+        // valuation changed since the previous one (one word per wave)
+        const unsigned long long cm = __ballot(ch);
+        if ((threadIdx.x & (kWave - 1)) == 0) chg[v >> 6] = cm;
         if (trace) {
             int br = live ? gc[g].br : 0;
             bool rec = live && br < trace_cap;
@@ -503,8 +516,15 @@ __global__ void __launch_bounds__(kBS) k_switch(
     int* __restrict__ strat, int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
     const int* __restrict__ key, const int* __restrict__ dep, const int* __restrict__ par,
     const int4* __restrict__ Sl, const JumpCnt* __restrict__ jc, int loop_form,
-    int* __restrict__ flag, unsigned long long* __restrict__ ties, const int* __restrict__ hi)
+    int* __restrict__ flag, unsigned long long* __restrict__ ties, const int* __restrict__ hi,
+    const unsigned long long* __restrict__ chg)
 {
+    // Loop form, player 1: switch(1) runs after every valuation, so a vertex
+    // left at the scan's fixed point stays there unless the valuation of its
+    // current successor or of a candidate changed; only such vertices gather
+    // keys.  (The result of a scan is at least as good as every successor,
+    // so scanning it again switches nothing.)
+    const bool incremental = loop_form && player == 1;
     const int F = jc->last + 1;                   // levels of the last valuation
     // A serving pool's owner lists are indexed like its vertices (holes are
     // -1), so nothing at or above the occupied bound is a vertex.
@@ -518,6 +538,21 @@ __global__ void __launch_bounds__(kBS) k_switch(
         if (v >= 0) {
             g = gid[v];
             bool live = !gc[g].done && (!loop_form || player == 1 || gc[g].outer);
+            if (live && incremental && !gc[g].reset_pending) {
+                const int c0 = strat[v];
+                bool dirty = c0 >= 0 && ((chg[c0 >> 6] >> (c0 & 63)) & 1);
+                const int e1 = off[v + 1];
+                for (int e0 = off[v]; e0 < e1 && !dirty; e0 += 8) {
+                    const int cnt = min(8, e1 - e0);
+                    int us[8];
+#pragma unroll
+                    for (int k = 0; k < 8; k++) us[k] = k < cnt ? tgt[e0 + k] : -1;
+#pragma unroll
+                    for (int k = 0; k < 8; k++)
+                        if (us[k] >= 0) dirty |= ((chg[us[k] >> 6] >> (us[k] & 63)) & 1) != 0;
+                }
+                live = dirty;
+            }
             if (live) {
                 int cur = strat[v];
                 int kcur = key_of(key, cur);
@@ -852,6 +887,7 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
     if (int rc = dalloc(&c.bact, 2LL * nblocks(N, kBS))) return rc;
     if (int rc = dalloc(&c.blast, nblocks(N, kBS))) return rc;
     if (int rc = dalloc(&c.Sl, N * c.levels)) return rc;
+    if (int rc = dalloc(&c.chg, 4LL * nblocks(N, kBS))) return rc;
     if (trace) {
         if (int rc = dalloc(&c.Zl, N * c.levels)) return rc;
         if (int rc = dalloc(&c.dig, N)) return rc;
@@ -1051,6 +1087,7 @@ int core_create_pool(Core& c, long long vcap, long long ecap, int slots, int max
     if (int rc = dalloc(&c.bact, 2LL * nblocks(N, kBS))) return rc;
     if (int rc = dalloc(&c.blast, nblocks(N, kBS))) return rc;
     if (int rc = dalloc(&c.Sl, N * c.levels)) return rc;
+    if (int rc = dalloc(&c.chg, 4LL * nblocks(N, kBS))) return rc;
     if (int rc = dalloc(&c.own[0], N)) return rc;
     if (int rc = dalloc(&c.own[1], N)) return rc;
     if (int rc = dalloc(&c.gctl, G)) return rc;
@@ -1124,7 +1161,7 @@ void core_destroy(Core& c)
     if (c.graph) (void)hipGraphExecDestroy(c.graph);
     void* ptrs[] = {c.vinfo, c.off, c.tgt, c.gid, c.strat, c.par, c.dep, c.key, c.solved, c.lab, c.blive,
                     c.bact, c.blast, c.Sl, c.own[0], c.own[1], c.Zl, c.dig, c.tr, c.gctl, c.jc, c.flag,
-                    c.cnt_scratch};
+                    c.cnt_scratch, c.chg};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c = Core();
@@ -1162,7 +1199,7 @@ static void launch_label(Core& c, int maxn, bool loop_form)
 {
     k_label<<<SGRID(c.N)>>>(c.N, maxn, c.lmax, c.Sl, c.Zl, c.blast, c.gid, c.gctl, c.strat,
                            c.solved, c.lab, c.dep, c.key, c.dig, c.tr, c.trace_cap, loop_form,
-                           c.trace, c.jc, c.blive, c.flag + 2);
+                           c.trace, c.jc, c.blive, c.flag + 2, c.chg);
 }
 
 static void launch_switch(Core& c, int player, bool loop_form)
@@ -1173,7 +1210,7 @@ static void launch_switch(Core& c, int player, bool loop_form)
     k_switch<<<SGRID(c.own_n[player])>>>(player, c.N, c.own[player], c.own_n[player], c.vinfo,
                                         c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
                                         c.key, c.dep, c.par, c.Sl, c.jc, loop_form, c.flag,
-                                        c.prof_ties, c.flag + 2);
+                                        c.prof_ties, c.flag + 2, c.chg);
 }
 
 int core_set_params(Core& c, int reset, int max_rounds)
