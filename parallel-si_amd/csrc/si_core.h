@@ -47,6 +47,7 @@ struct GameCtl {
     int reset;      // cpu-reset mode (main.cpp:25-26)
     int vbase;      // first global vertex id of the game
     int n;          // vertices of the game
+    int clear0;     // the last step was an outer step: restart the switch(0) change record
 };
 
 // Jump-round bookkeeping (device memory), rewritten by every valuation.
@@ -85,7 +86,9 @@ struct Core {
     uint8_t* bact = nullptr;  // 2 x blocks: block holds a vertex still jumping (by round parity)
     uint8_t* blast = nullptr; // per block: last jump level written this valuation
     int4* Sl = nullptr;    // jump records: level j at Sl + level_slot(j)*N
-    unsigned long long* chg = nullptr;  // bit per vertex: valuation changed at the last valuation
+    unsigned long long* chg = nullptr;  // bit per vertex: valuation changed at the last valuation,
+                                        // then (at chg + chg_words) since the last switch(0) pass
+    long long chg_words = 0;
     int levels = 0;        // buffers allocated
     int* own[2] = {nullptr, nullptr};   // vertices owned by each player
     long long own_n[2] = {0, 0};

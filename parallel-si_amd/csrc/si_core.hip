@@ -146,6 +146,7 @@ __global__ void k_reset_ctl(int G, GameCtl* gc)
     c.outer = 0;
     c.reset_pending = 0;
     c.capped = 0;
+    c.clear0 = 0;
     gc[g] = c;
 }
 
@@ -278,9 +279,17 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int4
             int4 s = Sin[v];
             unsigned long long z = trace ? Zin[v] : 0;
             if (s.x >= 0) {
-                const int ly = min(r, (int)blast[s.x / kBS]);
-                const size_t at = (size_t)level_slot(ly) * N + s.x;
-                int4 t = Sl[at];
+                // The successor's record at level r, loaded together with its
+                // block's last level; a block that stopped earlier (its
+                // records final) is read again at that level.
+                const int bl = blast[s.x / kBS];
+                const size_t at_r = (size_t)level_slot(r) * N + s.x;
+                int4 t = Sl[at_r];
+                size_t at = at_r;
+                if (bl < r) {
+                    at = (size_t)level_slot(bl) * N + s.x;
+                    t = Sl[at];
+                }
                 if (trace) z += Zl[at];
                 s.x = t.x;
                 s.y = max(s.y, t.y);
@@ -322,7 +331,7 @@ __global__ void __launch_bounds__(kBS) k_label(
     int* __restrict__ key, unsigned long long* __restrict__ dig, unsigned long long* __restrict__ tr,
     int trace_cap, int loop_form, int trace, JumpCnt* __restrict__ jc,
     const uint8_t* __restrict__ blive, const int* __restrict__ hi,
-    unsigned long long* __restrict__ chg)
+    unsigned long long* __restrict__ chg, unsigned long long* __restrict__ chg0)
 {
     const int last = jc->last;
     const int F = last + 1;
@@ -379,9 +388,16 @@ __global__ void __launch_bounds__(kBS) k_label(
             }
             if (trace) dig[v] = d;
         }
-        // valuation changed since the previous one (one word per wave)
+        // Valuation changed since the previous one (one word per wave), and
+        // since the last switch(0) pass of the vertex's game: that word is
+        // cleared for games that took the outer step in the previous step.
         const unsigned long long cm = __ballot(ch);
-        if ((threadIdx.x & (kWave - 1)) == 0) chg[v >> 6] = cm;
+        const int restart = live ? gc[g].clear0 : 0;
+        const unsigned long long clr = __ballot(restart != 0);
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            chg[v >> 6] = cm;
+            chg0[v >> 6] = (chg0[v >> 6] & ~clr) | cm;
+        }
         if (trace) {
             int br = live ? gc[g].br : 0;
             bool rec = live && br < trace_cap;
@@ -517,14 +533,16 @@ __global__ void __launch_bounds__(kBS) k_switch(
     const int* __restrict__ key, const int* __restrict__ dep, const int* __restrict__ par,
     const int4* __restrict__ Sl, const JumpCnt* __restrict__ jc, int loop_form,
     int* __restrict__ flag, unsigned long long* __restrict__ ties, const int* __restrict__ hi,
-    const unsigned long long* __restrict__ chg)
+    const unsigned long long* __restrict__ chg, const unsigned long long* __restrict__ chg0)
 {
-    // Loop form, player 1: switch(1) runs after every valuation, so a vertex
-    // left at the scan's fixed point stays there unless the valuation of its
-    // current successor or of a candidate changed; only such vertices gather
-    // keys.  (The result of a scan is at least as good as every successor,
-    // so scanning it again switches nothing.)
-    const bool incremental = loop_form && player == 1;
+    // Loop form: a vertex left at the scan's fixed point stays there unless
+    // the valuation of its current successor or of a candidate changed since
+    // its last scan; only such vertices gather keys.  (The result of a scan
+    // is at least as good as every successor, so scanning it again switches
+    // nothing.)  Player 1 scans after every valuation (chg: changed at the
+    // last one); player 0 at outer steps (chg0: changed since then).
+    const bool incremental = loop_form != 0;
+    const unsigned long long* __restrict__ cb = player == 1 ? chg : chg0;
     const int F = jc->last + 1;                   // levels of the last valuation
     // A serving pool's owner lists are indexed like its vertices (holes are
     // -1), so nothing at or above the occupied bound is a vertex.
@@ -538,9 +556,9 @@ __global__ void __launch_bounds__(kBS) k_switch(
         if (v >= 0) {
             g = gid[v];
             bool live = !gc[g].done && (!loop_form || player == 1 || gc[g].outer);
-            if (live && incremental && !gc[g].reset_pending) {
+            if (live && incremental && !(player == 1 && gc[g].reset_pending)) {
                 const int c0 = strat[v];
-                bool dirty = c0 >= 0 && ((chg[c0 >> 6] >> (c0 & 63)) & 1);
+                bool dirty = c0 >= 0 && ((cb[c0 >> 6] >> (c0 & 63)) & 1);
                 const int e1 = off[v + 1];
                 for (int e0 = off[v]; e0 < e1 && !dirty; e0 += 8) {
                     const int cnt = min(8, e1 - e0);
@@ -549,7 +567,7 @@ __global__ void __launch_bounds__(kBS) k_switch(
                     for (int k = 0; k < 8; k++) us[k] = k < cnt ? tgt[e0 + k] : -1;
 #pragma unroll
                     for (int k = 0; k < 8; k++)
-                        if (us[k] >= 0) dirty |= ((chg[us[k] >> 6] >> (us[k] & 63)) & 1) != 0;
+                        if (us[k] >= 0) dirty |= ((cb[us[k] >> 6] >> (us[k] & 63)) & 1) != 0;
                 }
                 live = dirty;
             }
@@ -642,6 +660,7 @@ __global__ void k_ctl1(int G, GameCtl* gc)
     c.outer = c.sw1 == 0;
     c.sw1 = 0;
     c.reset_pending = 0;
+    c.clear0 = 0;
     gc[g] = c;
 }
 
@@ -654,6 +673,7 @@ __global__ void k_ctl0(int G, GameCtl* gc)
     if (c.done) return;
     if (c.outer) {
         c.iter++;
+        c.clear0 = 1;              // the next valuation restarts the switch(0) record
         if (c.sw0 == 0) c.done = 1;
         else if (c.reset) c.reset_pending = 1;
     }
@@ -887,7 +907,8 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
     if (int rc = dalloc(&c.bact, 2LL * nblocks(N, kBS))) return rc;
     if (int rc = dalloc(&c.blast, nblocks(N, kBS))) return rc;
     if (int rc = dalloc(&c.Sl, N * c.levels)) return rc;
-    if (int rc = dalloc(&c.chg, 4LL * nblocks(N, kBS))) return rc;
+    c.chg_words = 4LL * nblocks(N, kBS);
+    if (int rc = dalloc(&c.chg, 2 * c.chg_words)) return rc;
     if (trace) {
         if (int rc = dalloc(&c.Zl, N * c.levels)) return rc;
         if (int rc = dalloc(&c.dig, N)) return rc;
@@ -1087,7 +1108,8 @@ int core_create_pool(Core& c, long long vcap, long long ecap, int slots, int max
     if (int rc = dalloc(&c.bact, 2LL * nblocks(N, kBS))) return rc;
     if (int rc = dalloc(&c.blast, nblocks(N, kBS))) return rc;
     if (int rc = dalloc(&c.Sl, N * c.levels)) return rc;
-    if (int rc = dalloc(&c.chg, 4LL * nblocks(N, kBS))) return rc;
+    c.chg_words = 4LL * nblocks(N, kBS);
+    if (int rc = dalloc(&c.chg, 2 * c.chg_words)) return rc;
     if (int rc = dalloc(&c.own[0], N)) return rc;
     if (int rc = dalloc(&c.own[1], N)) return rc;
     if (int rc = dalloc(&c.gctl, G)) return rc;
@@ -1169,10 +1191,21 @@ void core_destroy(Core& c)
 
 #define GRID(n) nblocks((n), kBS), kBS, 0, c.s
 // Jump rounds: capped grid (grid-stride inside) so idle launches stay cheap.
-#define JGRID(n) std::min(nblocks((n), kBS), 4096u), kBS, 0, c.s
+#define JGRID(n) std::min(nblocks((n), kBS), jump_grid_cap()), kBS, 0, c.s
 // Step kernels: capped grid-stride launches, so that a serving pool's large
 // capacity costs nothing above its occupied bound.
 #define SGRID(n) std::min(nblocks((n), kBS), c.pool ? 4096u : 16384u), kBS, 0, c.s
+
+// Largest jump-round grid (PSI_JUMP_GRID overrides 2048 for tuning).
+static unsigned jump_grid_cap()
+{
+    static unsigned cap = [] {
+        const char* e = getenv("PSI_JUMP_GRID");
+        long v = e ? strtol(e, nullptr, 10) : 0;
+        return v >= 64 && v <= (1 << 20) ? (unsigned)v : 2048u;
+    }();
+    return cap;
+}
 
 static int maxn_of(const Core& c)
 {
@@ -1199,7 +1232,7 @@ static void launch_label(Core& c, int maxn, bool loop_form)
 {
     k_label<<<SGRID(c.N)>>>(c.N, maxn, c.lmax, c.Sl, c.Zl, c.blast, c.gid, c.gctl, c.strat,
                            c.solved, c.lab, c.dep, c.key, c.dig, c.tr, c.trace_cap, loop_form,
-                           c.trace, c.jc, c.blive, c.flag + 2, c.chg);
+                           c.trace, c.jc, c.blive, c.flag + 2, c.chg, c.chg + c.chg_words);
 }
 
 static void launch_switch(Core& c, int player, bool loop_form)
@@ -1210,7 +1243,7 @@ static void launch_switch(Core& c, int player, bool loop_form)
     k_switch<<<SGRID(c.own_n[player])>>>(player, c.N, c.own[player], c.own_n[player], c.vinfo,
                                         c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
                                         c.key, c.dep, c.par, c.Sl, c.jc, loop_form, c.flag,
-                                        c.prof_ties, c.flag + 2, c.chg);
+                                        c.prof_ties, c.flag + 2, c.chg, c.chg + c.chg_words);
 }
 
 int core_set_params(Core& c, int reset, int max_rounds)
