@@ -340,11 +340,12 @@ def main():
     g, s, it, br, wall, prof = si_bench(psi, a.n, a.steps, a.warmup, a.seed)
     value = g.n * br * a.steps / wall
     # Roofline of the dominant kernel: a pointer-jumping round reads and
-    # writes every vertex's 16-byte record (next, max, count) and gathers the
-    # 16-byte record of each still-jumping vertex's successor.
+    # writes every vertex's 8-byte record (successor or terminal code and
+    # count, largest priority) and gathers the 8-byte record of each
+    # still-jumping vertex's successor.
     jump_ms, jump_launches = prof["jump"]
     gathers = prof["gathers"][1]
-    jump_bytes = (32.0 * g.n * jump_launches + 16.0 * gathers) / max(jump_launches, 1)
+    jump_bytes = (16.0 * g.n * jump_launches + 8.0 * gathers) / max(jump_launches, 1)
     achieved = jump_bytes / (jump_ms / jump_launches * 1e-3) / 1e9 if jump_launches else 0.0
     roof = {"bound": "hbm", "kernel": "k_jump", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -85,7 +85,7 @@ struct Core {
     uint8_t* blive = nullptr; // per 256-vertex block: holds a vertex of an unfinished game
     uint8_t* bact = nullptr;  // 2 x blocks: block holds a vertex still jumping (by round parity)
     uint8_t* blast = nullptr; // per block: last jump level written this valuation
-    int4* Sl = nullptr;    // jump records: level j at Sl + level_slot(j)*N
+    int2* Sl = nullptr;    // jump records: level j at Sl + level_slot(j)*N
     unsigned long long* chg = nullptr;  // bit per vertex: valuation changed at the last valuation,
                                         // then (at chg + chg_words) since the last switch(0) pass
     long long chg_words = 0;

@@ -56,6 +56,17 @@ __host__ __device__ __forceinline__ int level_slot(int j)
 }
 
 __device__ __forceinline__ int pri_of(int vi) { return vi & kPriMask; }
+
+// Jump record (8 bytes): x = the 2^j-th successor, or, once the path ends
+// within 2^j vertices, a terminal code -(1 + kind + 8 * count) with kind 0
+// sink, 1 solved +inf, 2 solved -inf, 3 finished game, and count = vertices on
+// the path before the terminal (a record that has not ended spans exactly 2^j
+// vertices, so it needs no count); y = largest priority on the span (0 when
+// empty) | bit 30: some link on the span changed since the last valuation.
+constexpr int kChangedBit = 1 << 30;
+__host__ __device__ __forceinline__ int term_code(int kind, int count) { return -(1 + kind + 8 * count); }
+__device__ __forceinline__ int term_kind(int x) { return (-x - 1) & 7; }
+__device__ __forceinline__ int term_count(int x) { return (-x - 1) >> 3; }
 __device__ __forceinline__ int owner_of(int vi) { return (vi >> 30) & 1; }
 
 __device__ __forceinline__ unsigned long long hmix_i32(int i, int v)
@@ -165,7 +176,7 @@ __global__ void __launch_bounds__(kBS) k_val_init(
     long long N, const int* __restrict__ vinfo, const int* __restrict__ off,
     const int* __restrict__ tgt, const int* __restrict__ gid, const GameCtl* __restrict__ gc,
     int* __restrict__ strat, const int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
-    int* __restrict__ par, int4* __restrict__ S0, unsigned long long* __restrict__ Z0,
+    int* __restrict__ par, int2* __restrict__ S0, unsigned long long* __restrict__ Z0,
     JumpCnt* __restrict__ jc, uint8_t* __restrict__ blive, uint8_t* __restrict__ bact0,
     uint8_t* __restrict__ blast, int trace, const int* __restrict__ hi)
 {
@@ -200,13 +211,14 @@ __global__ void __launch_bounds__(kBS) k_val_init(
                 const int old = par[v];
                 const int first = gc[g].br == 0;
                 if (solved[v]) {
-                    const int code = lab[v] == 1 ? -2 : -3;
-                    S0[v] = make_int4(code, -1, 0, first | (old != code));
+                    const int code = term_code(lab[v] == 1 ? 1 : 2, 0);
+                    S0[v] = make_int2(code, (first | (old != code)) ? kChangedBit : 0);
                     par[v] = code;
                     if (trace) Z0[v] = 0;
                 } else {
                     par[v] = s;
-                    S0[v] = make_int4(s, pri_of(vi), 1, first | (old != s));
+                    S0[v] = make_int2(s >= 0 ? s : term_code(0, 1),
+                                      pri_of(vi) | ((first | (old != s)) ? kChangedBit : 0));
                     if (trace) Z0[v] = splitmix64((unsigned long long)(uint32_t)pri_of(vi));
                     jumping = s >= 0;
                 }
@@ -220,7 +232,7 @@ __global__ void __launch_bounds__(kBS) k_val_init(
         // Other vertices of a visited block hold a terminal record, so that
         // the jump rounds never follow what a finished game, or a free range
         // of a serving pool, left in the level buffers.
-        if (any && !live && v < N) S0[v] = make_int4(-4, -1, 0, 0);
+        if (any && !live && v < N) S0[v] = make_int2(term_code(3, 0), 0);
         if (threadIdx.x == 0) {
             blive[b] = (uint8_t)any;
             bact0[b] = (uint8_t)anyj;
@@ -235,7 +247,7 @@ __global__ void __launch_bounds__(kBS) k_val_init(
 // records there are final, which is where gathers and the label kernel read
 // them (level min(r, blast[block])).  Lifting compares only read a vertex's
 // levels below its resolution level, which its block always wrote.
-__global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int4* __restrict__ Sl,
+__global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int2* __restrict__ Sl,
                                               unsigned long long* __restrict__ Zl,
                                               JumpCnt* __restrict__ jc,
                                               const uint8_t* __restrict__ blive,
@@ -260,8 +272,9 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int4
     }
     __syncthreads();
     if (!run_s) return;
-    const int4* Sin = Sl + (size_t)level_slot(r) * N;
-    int4* Sout = Sl + (size_t)level_slot(r + 1) * N;
+    const int2* Sin = Sl + (size_t)level_slot(r) * N;
+    int2* Sout = Sl + (size_t)level_slot(r + 1) * N;
+    const int span = r < 28 ? 1 << r : 0;         // vertices of an unfinished level-r record
     const unsigned long long* Zin = Zl + (size_t)level_slot(r) * N;
     unsigned long long* Zout = Zl + (size_t)level_slot(r + 1) * N;
     unsigned my_res = 0, my_act = 0;
@@ -276,7 +289,7 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int4
         long long v = base + threadIdx.x;
         bool res = false, act = false;
         if (v < N) {
-            int4 s = Sin[v];
+            int2 s = Sin[v];
             unsigned long long z = trace ? Zin[v] : 0;
             if (s.x >= 0) {
                 // The successor's record at level r, loaded together with its
@@ -284,17 +297,15 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int4
                 // records final) is read again at that level.
                 const int bl = blast[s.x / kBS];
                 const size_t at_r = (size_t)level_slot(r) * N + s.x;
-                int4 t = Sl[at_r];
+                int2 t = Sl[at_r];
                 size_t at = at_r;
                 if (bl < r) {
                     at = (size_t)level_slot(bl) * N + s.x;
                     t = Sl[at];
                 }
                 if (trace) z += Zl[at];
-                s.x = t.x;
-                s.y = max(s.y, t.y);
-                s.z += t.z;
-                s.w |= t.w;
+                s.x = t.x >= 0 ? t.x : term_code(term_kind(t.x), span + term_count(t.x));
+                s.y = max(s.y & kPriMask, t.y & kPriMask) | ((s.y | t.y) & kChangedBit);
                 res = s.x < 0;
                 act = !res;
             }
@@ -324,7 +335,7 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int4
 // also mark_solved(1) (cpu_parity.cpp:306-315), which solve_si calls right
 // after every valuation.
 __global__ void __launch_bounds__(kBS) k_label(
-    long long N, int maxn, int lmax, const int4* __restrict__ Sl,
+    long long N, int maxn, int lmax, const int2* __restrict__ Sl,
     const unsigned long long* __restrict__ Zl, const uint8_t* __restrict__ blast,
     const int* __restrict__ gid, const GameCtl* __restrict__ gc, const int* __restrict__ strat,
     int8_t* __restrict__ solved, int8_t* __restrict__ lab, int* __restrict__ dep,
@@ -346,7 +357,7 @@ __global__ void __launch_bounds__(kBS) k_label(
         if (!blive[b]) continue;                  // uniform: the whole block is finished
         // this block's final level (it may have stopped jumping earlier)
         const int Fb = min(F, (int)blast[b]);
-        const int4* SF = Sl + (size_t)level_slot(Fb) * N;
+        const int2* SF = Sl + (size_t)level_slot(Fb) * N;
         const unsigned long long* ZF = Zl + (size_t)level_slot(Fb) * N;
         const long long v = vb + threadIdx.x;
         int g = -1, l = 0, sol = 0, loc = 0;
@@ -365,19 +376,22 @@ __global__ void __launch_bounds__(kBS) k_label(
                 key[v] = (l + 1) << 30;
                 dep[v] = 0;
             } else {
-                int4 s = SF[v];
-                if (s.x == -1) {
+                int2 s = SF[v];
+                const int kind = s.x < 0 ? term_kind(s.x) : -1;
+                if (kind == 0) {                   // reaches the sink: finite
                     l = 0;
-                    dep[v] = s.z;
-                    key[v] = (1 << 30) | s.y;
-                    ch = s.w != 0;                 // a link on the path changed
+                    dep[v] = term_count(s.x);
+                    key[v] = (1 << 30) | (s.y & kPriMask);
+                    ch = (s.y & kChangedBit) != 0; // a link on the path changed
                     if (trace) d = ZF[v];
                 } else {
-                    if (s.x == -2) l = 1;
-                    else if (s.x == -3) l = -1;
-                    else {                         // largest priority on the cycle
+                    if (kind == 1) l = 1;
+                    else if (kind == 2) l = -1;
+                    else if (kind < 0) {           // largest priority on the cycle
                         const int Fy = min(F, (int)blast[s.x / kBS]);
                         l = (Sl[(size_t)level_slot(Fy) * N + s.x].y & 1) ? -1 : 1;
+                    } else {
+                        l = 1;                     // kind 3 (finished game): never on a live path
                     }
                     key[v] = (l + 1) << 30;
                     dep[v] = 0;
@@ -419,7 +433,7 @@ __global__ void __launch_bounds__(kBS) k_label(
 // O(log depth) gathers.  Only when the two segment maxima are equal too does
 // the exact scan walk the segments level by level.
 __device__ __noinline__ int cmp_exact(int u, int w, const int* __restrict__ par, const int* __restrict__ dep,
-                         const int* __restrict__ vinfo, const int4* __restrict__ Sl, long long N,
+                         const int* __restrict__ vinfo, const int2* __restrict__ Sl, long long N,
                          int F)
 {
     const int T = min(F, kLiftCap);                   // highest stored table level
@@ -427,11 +441,11 @@ __device__ __noinline__ int cmp_exact(int u, int w, const int* __restrict__ par,
     int a = u, b = w, da = dep[u], db = dep[w];
     int mA = -1, mB = -1;
     // Bring the deeper one up to the other's depth.
-    while (da - db >= (1 << T)) { int4 t = rec(T, a); mA = max(mA, t.y); a = t.x; da -= 1 << T; }
-    while (db - da >= (1 << T)) { int4 t = rec(T, b); mB = max(mB, t.y); b = t.x; db -= 1 << T; }
+    while (da - db >= (1 << T)) { int2 t = rec(T, a); mA = max(mA, t.y & kPriMask); a = t.x; da -= 1 << T; }
+    while (db - da >= (1 << T)) { int2 t = rec(T, b); mB = max(mB, t.y & kPriMask); b = t.x; db -= 1 << T; }
     for (int j = T - 1; j >= 0; j--) {
-        if (da - db >= (1 << j)) { int4 t = rec(j, a); mA = max(mA, t.y); a = t.x; da -= 1 << j; }
-        if (db - da >= (1 << j)) { int4 t = rec(j, b); mB = max(mB, t.y); b = t.x; db -= 1 << j; }
+        if (da - db >= (1 << j)) { int2 t = rec(j, a); mA = max(mA, t.y & kPriMask); a = t.x; da -= 1 << j; }
+        if (db - da >= (1 << j)) { int2 t = rec(j, b); mB = max(mB, t.y & kPriMask); b = t.x; db -= 1 << j; }
     }
     int dl;                                           // depth of the common ancestor
     if (a == b) {
@@ -440,15 +454,15 @@ __device__ __noinline__ int cmp_exact(int u, int w, const int* __restrict__ par,
         // Jump both while their ancestors differ (only levels shorter than the
         // remaining path are read: longer windows end at the sink for both).
         while ((1 << T) < da) {
-            int4 ta = rec(T, a), tb = rec(T, b);
+            int2 ta = rec(T, a), tb = rec(T, b);
             if (ta.x == tb.x) break;
-            mA = max(mA, ta.y); mB = max(mB, tb.y); a = ta.x; b = tb.x; da -= 1 << T;
+            mA = max(mA, ta.y & kPriMask); mB = max(mB, tb.y & kPriMask); a = ta.x; b = tb.x; da -= 1 << T;
         }
         for (int j = T - 1; j >= 0; j--) {
             if ((1 << j) >= da) continue;
-            int4 ta = rec(j, a), tb = rec(j, b);
+            int2 ta = rec(j, a), tb = rec(j, b);
             if (ta.x != tb.x) {
-                mA = max(mA, ta.y); mB = max(mB, tb.y); a = ta.x; b = tb.x; da -= 1 << j;
+                mA = max(mA, ta.y & kPriMask); mB = max(mB, tb.y & kPriMask); a = ta.x; b = tb.x; da -= 1 << j;
             }
         }
         mA = max(mA, pri_of(vinfo[a]));
@@ -499,7 +513,7 @@ __device__ __noinline__ int cmp_exact(int u, int w, const int* __restrict__ par,
 // keys; -1 is the sink (finite, empty path).
 __device__ __forceinline__ int cmp_keys(int u, int ku, int w, int kw, const int* __restrict__ par,
                                         const int* __restrict__ dep, const int* __restrict__ vinfo,
-                                        const int4* __restrict__ Sl, long long N, int F,
+                                        const int2* __restrict__ Sl, long long N, int F,
                                         unsigned long long* ties)
 {
     if (u == w) return 0;
@@ -531,7 +545,7 @@ __global__ void __launch_bounds__(kBS) k_switch(
     const int* __restrict__ tgt, const int* __restrict__ gid, GameCtl* __restrict__ gc,
     int* __restrict__ strat, int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
     const int* __restrict__ key, const int* __restrict__ dep, const int* __restrict__ par,
-    const int4* __restrict__ Sl, const JumpCnt* __restrict__ jc, int loop_form,
+    const int2* __restrict__ Sl, const JumpCnt* __restrict__ jc, int loop_form,
     int* __restrict__ flag, unsigned long long* __restrict__ ties, const int* __restrict__ hi,
     const unsigned long long* __restrict__ chg, const unsigned long long* __restrict__ chg0)
 {
@@ -733,6 +747,7 @@ const char* core_validate_game(const GameView& gv, int* max_pri)
     const int n = gv.n;
     const long long m = gv.m;
     if (n <= 0) return "game with no vertices";
+    if (n >= (1 << 28)) return "game too large (at most 2^28 - 1 vertices)";
     if (gv.off[0] != 0 || (long long)gv.off[n] != m)
         return "offsets must start at 0 and end at the edge count";
     int mp = 0;
@@ -1070,7 +1085,7 @@ long long core_pool_bytes_per_vertex(int maxn)
     const int lmax = ceil_log2(maxn) + 2;
     const int levels = std::min(lmax + 1, kLiftCap + 3);
     // vinfo off gid strat par dep key own0 own1 (ints), solved lab, records
-    return 9 * 4 + 2 + 16LL * levels + 1;          // + block flags, rounded up
+    return 9 * 4 + 2 + 8LL * levels + 1;           // + block flags, change bits, rounded up
 }
 
 int core_create_pool(Core& c, long long vcap, long long ecap, int slots, int maxn, hipStream_t s)
