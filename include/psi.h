@@ -151,7 +151,9 @@ int psi_get_stats(psi_solver* s, int game, int* iter, int* br, int* capped);
 #define PSI_K_TOTAL 7
 #define PSI_K_GATHERS 8   /* counts only: successor records gathered by jump rounds */
 #define PSI_K_TIES 9      /* counts only: compares that took the exact (equal path max) path */
-#define PSI_K_COUNT 10
+#define PSI_K_SCANS 10    /* counts only: exact compares that walked the private segments */
+#define PSI_K_SCAN_STEPS 11 /* counts only: vertices visited by those walks */
+#define PSI_K_COUNT 12
 int psi_solve_profiled(psi_solver* s, int reset, int max_rounds, double* times_ms,
                        int64_t* counts);  /* both arrays have PSI_K_COUNT entries */
 

@@ -430,11 +430,13 @@ __global__ void __launch_bounds__(kBS) k_label(
 // records of every level are binary-lifting tables (level j: 2^j-th
 // successor and the largest priority over those 2^j vertices), so the lowest
 // common ancestor and the largest priority of each private segment take
-// O(log depth) gathers.  Only when the two segment maxima are equal too does
-// the exact scan walk the segments level by level.
-__device__ __noinline__ int cmp_exact(int u, int w, const int* __restrict__ par, const int* __restrict__ dep,
-                         const int* __restrict__ vinfo, const int2* __restrict__ Sl, long long N,
-                         int F)
+// O(log depth) gathers.  Returns the sign when the two segment maxima differ;
+// otherwise 2, with the lengths of the private segments (u's first la
+// vertices and w's first lb vertices, before the common part), whose
+// multisets coop_scan compares.
+__device__ __noinline__ int cmp_lca(int u, int w, const int* __restrict__ dep,
+                                    const int2* __restrict__ Sl, const int* __restrict__ vinfo,
+                                    long long N, int F, int* la_out, int* lb_out)
 {
     const int T = min(F, kLiftCap);                   // highest stored table level
     auto rec = [&](int j, int x) { return Sl[(size_t)j * N + x]; };
@@ -473,49 +475,93 @@ __device__ __noinline__ int cmp_exact(int u, int w, const int* __restrict__ par,
         if (mA > mB) return (mA & 1) ? -1 : 1;
         return (mB & 1) ? 1 : -1;
     }
-    // Equal segment maxima: compare the two multisets from the top down.
-    const int la = dep[u] - dl, lb = dep[w] - dl;
+    *la_out = dep[u] - dl;
+    *lb_out = dep[w] - dl;
+    return 2;
+}
+
+// The vertex d steps along x's path (d below the path's length).
+__device__ __forceinline__ int path_step(int x, int d, const int2* __restrict__ Sl, long long N, int T)
+{
+    while (d >= (1 << T)) { x = Sl[(size_t)T * N + x].x; d -= 1 << T; }
+    for (int j = T - 1; j >= 0; j--)
+        if (d >= (1 << j)) { x = Sl[(size_t)j * N + x].x; d -= 1 << j; }
+    return x;
+}
+
+__device__ __forceinline__ int wave_max(int x)
+{
+    for (int o = kWave / 2; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o));
+    return x;
+}
+
+__device__ __forceinline__ int wave_sum(int x)
+{
+    for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+// Compares the priority multisets of u's first la and w's first lb path
+// vertices from the top down (compare_valuation's finite case,
+// cpu_parity.cpp:235-277): the largest priority whose counts differ decides,
+// even priorities counting for player 0.  Run by the whole wave: each lane
+// walks 1/64 of each segment (its start found through the lifting tables), and
+// each step of the top-down compare is a wave reduction.  Every lane returns
+// the result.
+__device__ __noinline__ int coop_scan(int u, int la, int w, int lb, const int* __restrict__ par,
+                                      const int* __restrict__ vinfo, const int2* __restrict__ Sl,
+                                      long long N, int F, unsigned long long* __restrict__ prof)
+{
+    const int T = min(F, kLiftCap);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int pa = (int)(((long long)lane * la) >> 6), na = (int)(((long long)(lane + 1) * la) >> 6) - pa;
+    const int pb = (int)(((long long)lane * lb) >> 6), nb = (int)(((long long)(lane + 1) * lb) >> 6) - pb;
+    const int xa = na > 0 ? path_step(u, pa, Sl, N, T) : -1;
+    const int xb = nb > 0 ? path_step(w, pb, Sl, N, T) : -1;
+    if (prof && lane == 0) atomicAdd(prof + 1, 1ull);
     int thr = 0x7FFFFFFF;
     for (;;) {
+        if (prof && lane == 0) atomicAdd(prof + 2, (unsigned long long)(la + lb));
         int ma = -1, ca = 0, mb = -1, cb = 0;
-        int x = u;
-        for (int i = 0; i < la; i++) {
-            int p = pri_of(vinfo[x]);
+        int x = xa;
+        for (int i = 0; i < na; i++) {
+            const int p = pri_of(vinfo[x]);
             if (p < thr) {
                 if (p > ma) { ma = p; ca = 1; }
                 else if (p == ma) ca++;
             }
             x = par[x];
         }
-        x = w;
-        for (int i = 0; i < lb; i++) {
-            int p = pri_of(vinfo[x]);
+        x = xb;
+        for (int i = 0; i < nb; i++) {
+            const int p = pri_of(vinfo[x]);
             if (p < thr) {
                 if (p > mb) { mb = p; cb = 1; }
                 else if (p == mb) cb++;
             }
             x = par[x];
         }
-        if (ma != mb) {
-            if (ma > mb) return (ma & 1) ? -1 : 1;
-            return (mb & 1) ? 1 : -1;
+        const int MA = wave_max(ma), MB = wave_max(mb);
+        if (MA != MB) {
+            if (MA > MB) return (MA & 1) ? -1 : 1;
+            return (MB & 1) ? 1 : -1;
         }
-        if (ma < 0) return 0;
-        if (ca != cb) {
-            if ((ma & 1) == 0) return ca > cb ? 1 : -1;
-            return ca < cb ? 1 : -1;
+        if (MA < 0) return 0;
+        const int CA = wave_sum(ma == MA ? ca : 0), CB = wave_sum(mb == MB ? cb : 0);
+        if (CA != CB) {
+            if ((MA & 1) == 0) return CA > CB ? 1 : -1;
+            return CA < CB ? 1 : -1;
         }
-        thr = ma;
+        thr = MA;
     }
 }
 
-// Sign of compare_valuation(u, w) (cpu_parity.cpp:235-277) from the two
-// keys; -1 is the sink (finite, empty path).
-__device__ __forceinline__ int cmp_keys(int u, int ku, int w, int kw, const int* __restrict__ par,
-                                        const int* __restrict__ dep, const int* __restrict__ vinfo,
-                                        const int2* __restrict__ Sl, long long N, int F,
-                                        unsigned long long* ties)
+// Sign of compare_valuation(u, w) (cpu_parity.cpp:235-277) from the two keys;
+// -1 is the sink (finite, empty path).  *tie: both finite with equal path
+// maxima (the exact compare decides).
+__device__ __forceinline__ int cmp_fast(int u, int ku, int w, int kw, bool* tie)
 {
+    *tie = false;
     if (u == w) return 0;
     int cu = (int)((unsigned)ku >> 30), cw = (int)((unsigned)kw >> 30);   // +inf keys are negative ints
     if (cu != cw) return cu > cw ? 1 : -1;
@@ -526,8 +572,8 @@ __device__ __forceinline__ int cmp_keys(int u, int ku, int w, int kw, const int*
         if (pu > pw) return (pu & 1) ? -1 : 1;
         return (pw & 1) ? 1 : -1;
     }
-    if (ties) atomicAdd(ties, 1ull);
-    return cmp_exact(u, w, par, dep, vinfo, Sl, N, F);
+    *tie = true;
+    return 0;
 }
 
 __device__ __forceinline__ int key_of(const int* __restrict__ key, int u)
@@ -564,12 +610,12 @@ __global__ void __launch_bounds__(kBS) k_switch(
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long ib = (long long)blockIdx.x * blockDim.x; ib < lim; ib += stride) {
         const long long i = ib + threadIdx.x;
-        bool sw = false;
+        bool sw = false, live = false;
         int g = -1;
         const int v = i < lim ? own[i] : -1;
         if (v >= 0) {
             g = gid[v];
-            bool live = !gc[g].done && (!loop_form || player == 1 || gc[g].outer);
+            live = !gc[g].done && (!loop_form || player == 1 || gc[g].outer);
             if (live && incremental && !(player == 1 && gc[g].reset_pending)) {
                 const int c0 = strat[v];
                 bool dirty = c0 >= 0 && ((cb[c0 >> 6] >> (c0 & 63)) & 1);
@@ -585,36 +631,51 @@ __global__ void __launch_bounds__(kBS) k_switch(
                 }
                 live = dirty;
             }
-            if (live) {
-                int cur = strat[v];
-                int kcur = key_of(key, cur);
-                const int e1 = off[v + 1];
-                // Successors in batches of 8: the target and key loads of a
-                // batch are independent, so they are in flight together; the
-                // scan itself is sequential in CSR order (the reference's
-                // tie-break).
-                for (int e0 = off[v]; e0 < e1; e0 += 8) {
-                    const int cnt = min(8, e1 - e0);
-                    int us[8], ks[8];
+        }
+        // Successors in batches of 8: the target and key loads of a batch are
+        // independent, so they are in flight together; the scan itself is
+        // sequential in CSR order (the reference's tie-break).  The loop runs
+        // for the whole wave, so that exact compares whose segment maxima tie
+        // are done by all lanes together (coop_scan), one at a time.
+        int cur = live ? strat[v] : -1;
+        int kcur = live ? key_of(key, cur) : 0;
+        int e = live ? off[v] : 0;
+        const int e1 = live ? off[v + 1] : 0;
+        while (__any(e < e1)) {
+            const int cnt = e < e1 ? min(8, e1 - e) : 0;
+            int us[8], ks[8];
 #pragma unroll
-                    for (int k = 0; k < 8; k++) us[k] = k < cnt ? tgt[e0 + k] : -1;
+            for (int k = 0; k < 8; k++) us[k] = k < cnt ? tgt[e + k] : -1;
 #pragma unroll
-                    for (int k = 0; k < 8; k++) ks[k] = k < cnt ? key[us[k]] : 0;
+            for (int k = 0; k < 8; k++) ks[k] = k < cnt ? key[us[k]] : 0;
 #pragma unroll
-                    for (int k = 0; k < 8; k++) {
-                        if (k < cnt) {
-                            int c = cmp_keys(us[k], ks[k], cur, kcur, par, dep, vinfo, Sl, N, F, ties);
-                            if (player == 0 ? c > 0 : c < 0) {
-                                cur = us[k];
-                                kcur = ks[k];
-                                sw = true;
-                            }
-                        }
+            for (int k = 0; k < 8; k++) {
+                int c = 0, la = 0, lb = 0;
+                bool tie = false;
+                if (k < cnt) {
+                    c = cmp_fast(us[k], ks[k], cur, kcur, &tie);
+                    if (tie) {
+                        if (ties) atomicAdd(ties, 1ull);
+                        c = cmp_lca(us[k], cur, dep, Sl, vinfo, N, F, &la, &lb);
                     }
                 }
-                if (sw) strat[v] = cur;
+                unsigned long long need = __ballot(c == 2);
+                while (need) {
+                    const int L = __ffsll((long long)need) - 1;
+                    const int r = coop_scan(__shfl(us[k], L), __shfl(la, L), __shfl(cur, L),
+                                            __shfl(lb, L), par, vinfo, Sl, N, F, ties);
+                    if ((int)(threadIdx.x & (kWave - 1)) == L) c = r;
+                    need &= need - 1;
+                }
+                if (k < cnt && (player == 0 ? c > 0 : c < 0)) {
+                    cur = us[k];
+                    kcur = ks[k];
+                    sw = true;
+                }
             }
+            e += 8;
         }
+        if (sw) strat[v] = cur;
         // one store per wave and game: the flag only has to become nonzero
         unsigned long long m = __ballot(sw);
         if (m) {
@@ -1357,8 +1418,8 @@ int core_solve_profiled(Core& c, double* times, long long* counts)
     };
     int rc = PSI_OK;
     unsigned long long* ties = nullptr;
-    PSI_HIP(hipMalloc(&ties, 8));
-    PSI_HIP(hipMemsetAsync(ties, 0, 8, c.s));
+    PSI_HIP(hipMalloc(&ties, 24));
+    PSI_HIP(hipMemsetAsync(ties, 0, 24, c.s));
     c.prof_ties = ties;
     for (int step = 0; step < (1 << 20); step++) {
         int k = 0;
@@ -1413,9 +1474,11 @@ int core_solve_profiled(Core& c, double* times, long long* counts)
         if (done) break;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
-    unsigned long long h_ties = 0;
-    (void)hipMemcpy(&h_ties, ties, 8, hipMemcpyDeviceToHost);
-    counts[PSI_K_TIES] = (long long)h_ties;
+    unsigned long long h_ties[3] = {0, 0, 0};
+    (void)hipMemcpy(h_ties, ties, 24, hipMemcpyDeviceToHost);
+    counts[PSI_K_TIES] = (long long)h_ties[0];
+    counts[PSI_K_SCANS] = (long long)h_ties[1];
+    counts[PSI_K_SCAN_STEPS] = (long long)h_ties[2];
     c.prof_ties = nullptr;
     (void)hipFree(ties);
     return rc;

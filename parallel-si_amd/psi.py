@@ -387,14 +387,14 @@ class Solver:
         return buf[: 4 * k.value].reshape(-1, 4)
 
 K_NAMES = ["val_init", "jump", "jump_idle", "label", "switch1", "switch0", "ctl", "total",
-           "gathers", "ties"]
+           "gathers", "ties", "scans", "scan_steps"]
 _sig("psi_solve_profiled", C.c_int, _vp, C.c_int, C.c_int,
      np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS"), _i64p)
 
 
 def solve_profiled(solver, reset=False, max_rounds=0):
-    t = np.zeros(10, np.float64)
-    c = np.zeros(10, np.int64)
+    t = np.zeros(len(K_NAMES), np.float64)
+    c = np.zeros(len(K_NAMES), np.int64)
     check(lib.psi_solve_profiled(solver.h, int(reset), max_rounds, t, c), "psi_solve_profiled")
     return {k: (float(t[i]), int(c[i])) for i, k in enumerate(K_NAMES)}
 
