@@ -283,6 +283,18 @@ def serve_bench(psi, requests=96, rate=2.0, seed=4, budget_gib=64.0, steps=4, lo
             "served_equals_alone": bool(same), "checked_alone": len(kept)}
 
 
+def measured_traffic(n, seed):
+    """HBM bytes per active jump round of this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None for
+    other workloads or if the summary is absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", "si_solve_4M_hbm_bytes.md")
+    if n != 1 << 22 or seed != 2 or not os.path.exists(path):
+        return None
+    m = re.search(r"k_jump active launch HBM traffic: (\d+) bytes", open(path).read())
+    return int(m.group(1)) if m else None
+
+
 def cpu_baseline(g, max_rounds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -349,7 +361,7 @@ def main():
     achieved = jump_bytes / (jump_ms / jump_launches * 1e-3) / 1e9 if jump_launches else 0.0
     roof = {"bound": "hbm", "kernel": "k_jump", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None, "alg_bytes_per_launch": round(jump_bytes),
+            "traffic": measured_traffic(g.n, a.seed), "alg_bytes_per_launch": round(jump_bytes),
             "avg_launch_us": round(jump_ms / max(jump_launches, 1) * 1e3, 2),
             "launches": jump_launches}
     line = {
