@@ -83,7 +83,7 @@ int psi_list_rank_tune(int param, int value);
 /* Replaces Game (game.h:9-34) as input and GPUGame (gpu_game.h:6-21,
  * gpu_game.cpp:11-41) as the int32 CSR layout: offsets[n+1] (offsets[n] = m),
  * targets[m] (file order = the switch tie-break order), owner[n] in {0,1},
- * priority[n] in [0, 2^30).  Unlike the reference, successor ids are range
+ * priority[n] in [0, 2^30), n < 2^28.  Unlike the reference, successor ids are range
  * checked and negative priorities are rejected (both are undefined behaviour
  * there).  The arrays are copied. */
 typedef struct psi_game psi_game;
@@ -138,6 +138,21 @@ int psi_reset_opponent_strategy(psi_solver* s);
  * max_rounds > 0 caps valuation rounds per game (then psi_get_stats reports
  * capped); steps_per_sync SI steps run per host check (a captured hipGraph). */
 int psi_solve(psi_solver* s, int reset, int max_rounds, int steps_per_sync);
+/* solve_bf (main.cpp:57-77) with the CPU_BellmanFord best response
+ * (cpu_bf.cpp:25-166), every game of the handle on the device: the first best
+ * response is an SI inner loop (CPUBV semantics: an odd strategy cycle stops
+ * the game, as cpu_bv.cpp:73-84 exits; PSI_ERR_INVALID with that message and
+ * psi_get_bf_status = 1); later ones are relaxations to Odd's best response,
+ * in which a vertex that cannot reach the sink stays +inf.  Odd's final
+ * strategy is succ (the last finite choice of each Odd vertex, 0 before any),
+ * as the reference leaves it.  The relaxation runs in parallel rounds
+ * (Jacobi, on exact path valuations) where the reference updates in place
+ * (Gauss-Seidel): values, labels, solved sets, Even's strategies, iteration
+ * counts and winning regions equal the reference's on every game whose
+ * relaxations converge; Odd's succ equals it where the minimum successor is
+ * unique; br counts device valuation rounds, not the reference's sweeps. */
+int psi_solve_bf(psi_solver* s, int max_rounds, int steps_per_sync);
+int psi_get_bf_status(psi_solver* s, int game, int* status);   /* 1 = odd cycle */
 int psi_get_stats(psi_solver* s, int game, int* iter, int* br, int* capped);
 /* The same solve launched kernel by kernel with HIP events between launches
  * (no graph): device milliseconds and launch counts per kernel class. */

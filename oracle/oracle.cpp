@@ -17,6 +17,8 @@
 //       - get_winning ............ cpu_parity.cpp:329-338
 //       - reset_opponent_strategy  cpu_parity.cpp:341-348
 //   * solve_si driver loop ....... main.cpp:20-55 (== gpumain.cpp:18-53)
+//   * CPU_BellmanFord + solve_bf . cpu_bf.cpp:8-166, main.cpp:57-77 (dense rows,
+//                                  in-place relaxation in vertex order)
 //   * list ranking ............... the walk of cpu_list_ranking.cpp:197-250 with
 //                                  unit weights is the sequential distance from
 //                                  the head; restated as a brute-force walk.
@@ -42,6 +44,8 @@
 //                   inverse permutation) and hand-made forests
 //   zielonka      - textbook algorithm; cross-checked against the winning
 //                   regions of the reference fixtures
+//   solve_bf      - the compiled reference's cpubf run (final strategy,
+//                   labels, iter and br counts) on the committed fixtures
 
 #include <algorithm>
 #include <chrono>
@@ -421,6 +425,103 @@ void record(Parity& s, Trace* t, std::vector<uint64_t>& dig)
 }
 
 // ---------------------------------------------------------------------------
+// CPU_BellmanFord<ArrayVal> (cpu_bf.cpp:8-166) driven by solve_bf (main.cpp:57-77),
+// on top of CPUBV<ArrayVal> (cpu_bv.cpp), whose init_strat / compute_valuation /
+// compare / switch / mark_solved are CPUParity's (the Parity struct, dense
+// rows) except that compute_valuation aborts on an odd strategy cycle
+// (cpu_bv.cpp:73-84).
+struct BellmanFord {
+    Parity s;
+    std::vector<int> bf_inf, succ;
+    bool init = false;
+    int status = 0;                  // 1: odd cycle (the reference exits), 3: a relaxation hit n rounds
+    std::vector<int> tmp;
+
+    explicit BellmanFord(const Game& g) : s(g, true)
+    {
+        bf_inf.assign(s.n, 1);       // cpu_bf.cpp:13-17 (p1_vertices is never read)
+        succ.assign(s.n, 0);
+        tmp.assign(s.P, 0);
+    }
+
+    // ArrayVal: a row plus the priority of v (add_vertex, valuation.cpp)
+    void row_plus(const int* from, int v, int* out) const
+    {
+        if (from) std::copy(from, from + s.P, out);
+        else std::fill(out, out + s.P, 0);
+        out[s.g.pri[v]] += 1;
+    }
+
+    // cpu_bf.cpp:25-49
+    void best_response(int* br)
+    {
+        if (!init) {
+            for (;;) {
+                s.compute_valuation();
+                for (int v = 0; v < s.n; v++)
+                    if (!s.solved[v] && s.inf[v] == -1) { status = 1; return; }   // cpu_bv.cpp:79-83
+                s.mark_solved(1);
+                if (s.switch_strategy(1) == 0) break;
+            }
+            init = true;
+            return;
+        }
+        std::fill(bf_inf.begin(), bf_inf.end(), 1);
+        best_response_alg(br);
+    }
+
+    // cpu_bf.cpp:51-166: in-place (Gauss-Seidel) relaxation in vertex order
+    void best_response_alg(int* br)
+    {
+        const int n = s.n, P = s.P;
+        std::fill(s.vals.begin(), s.vals.end(), 0);
+        bool done = false;
+        for (int i = 0; i < n; i++) {
+            (*br)++;
+            done = true;
+            for (int v = 0; v < n; v++) {
+                if (s.solved[v]) continue;
+                int* vrow = &s.vals[(size_t)v * P];
+                if (s.g.owner[v] == 0) {
+                    const int u = s.strat[v];
+                    if (u != -1 && bf_inf[u]) continue;
+                    if (bf_inf[v]) { bf_inf[v] = 0; done = false; }
+                    row_plus(u == -1 ? nullptr : &s.vals[(size_t)u * P], v, tmp.data());
+                    if (!std::equal(tmp.begin(), tmp.end(), vrow)) {
+                        std::copy(tmp.begin(), tmp.end(), vrow);
+                        done = false;
+                    }
+                    continue;
+                }
+                for (int e = s.g.off[v]; e < s.g.off[v + 1]; e++) {
+                    const int u = s.g.tgt[e];
+                    if (bf_inf[u]) continue;
+                    if (bf_inf[v]) {
+                        row_plus(&s.vals[(size_t)u * P], v, vrow);
+                        bf_inf[v] = 0;
+                        succ[v] = u;
+                        done = false;
+                        continue;
+                    }
+                    row_plus(&s.vals[(size_t)u * P], v, tmp.data());
+                    if (s.compare_dense(vrow, tmp.data()) == 1) {
+                        std::copy(tmp.begin(), tmp.end(), vrow);
+                        succ[v] = u;
+                        done = false;
+                    }
+                }
+            }
+            if (done) break;
+        }
+        if (!done) status = 3;
+        for (int v = 0; v < n; v++) {
+            if (bf_inf[v] && !s.solved[v]) s.inf[v] = 1;
+            if (s.g.owner[v] == 1) s.strat[v] = succ[v];
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
 // Zielonka's recursive algorithm (max-parity: the largest priority seen
 // infinitely often decides; even -> player 0).  Independent check on winners.
 struct Zielonka {
@@ -569,6 +670,47 @@ int orc_solve(int n, const int* off, const int* tgt, const int* owner, const int
     if (br_out) *br_out = br;
     if (n_rounds) *n_rounds = tp ? tr.count : br;
     return capped;
+}
+
+// solve_bf (main.cpp:57-77) with CPU_BellmanFord<ArrayVal>.  status: 0 ok,
+// 1 odd strategy cycle in the first best response (the reference prints
+// "Game contains an odd cycle that uses only odd nodes" and exits), 2 stopped
+// at max_iters, 3 some relaxation ran n rounds without converging.
+// dig_out: digest of each final valuation (sum count[p] * pri_weight(p), 0 if
+// infinite).
+int orc_solve_bf(int n, const int* off, const int* tgt, const int* owner, const int* pri,
+                 int max_iters, int* strat_out, int* inf_out, int* solved_out, uint64_t* dig_out,
+                 int* iter_out, int* br_out, int* status_out)
+{
+    Game g;
+    if (load_csr(n, off, tgt, owner, pri, g)) return -1;
+    if ((double)n * (g.max_pri + 1) > 4.0e9) { g_err = "dense valuation too large"; return -1; }
+    BellmanFord bf(g);
+    Parity& s = bf.s;
+    int br = 0, iter = 0;
+    s.init_strat();
+    for (;;) {
+        if (max_iters > 0 && iter >= max_iters) { bf.status = bf.status ? bf.status : 2; break; }
+        bf.best_response(&br);
+        if (bf.status == 1) break;
+        s.mark_solved(0);
+        iter++;
+        if (s.switch_strategy(0) == 0) break;
+    }
+    if (strat_out) std::copy(s.strat.begin(), s.strat.end(), strat_out);
+    if (inf_out) std::copy(s.inf.begin(), s.inf.end(), inf_out);
+    if (solved_out) std::copy(s.solved.begin(), s.solved.end(), solved_out);
+    if (dig_out)
+        for (int v = 0; v < n; v++) {
+            uint64_t d = 0;
+            if (s.inf[v] == 0)
+                for (int p = 0; p < s.P; p++) d += (uint64_t)s.vals[(size_t)v * s.P + p] * pri_weight(p);
+            dig_out[v] = d;
+        }
+    if (iter_out) *iter_out = iter;
+    if (br_out) *br_out = br;
+    if (status_out) *status_out = bf.status;
+    return 0;
 }
 
 // One valuation of a given strategy with a given solved set (labels and

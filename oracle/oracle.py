@@ -198,3 +198,32 @@ def zielonka(game):
     if lib.orc_zielonka(g.n, g.offsets, g.targets, g.owner, g.priority, win) != 0:
         raise _err("zielonka")
     return win
+
+
+lib.orc_solve_bf.restype = C.c_int
+lib.orc_solve_bf.argtypes = [C.c_int, _i32p, _i32p, _i32p, _i32p, C.c_int, _i32p, _i32p, _i32p,
+                             _u64p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+BF_OK, BF_ODD_CYCLE, BF_CAPPED, BF_NO_CONVERGENCE = 0, 1, 2, 3
+
+
+def solve_bf(game, max_iters=0):
+    """solve_bf with CPU_BellmanFord<ArrayVal> (dense rows).  Returns an object
+    with strat, inf, solved, dig (final valuation digests), iter, br, status."""
+    g = game
+    n = g.n
+    strat = np.empty(n, np.int32)
+    inf = np.empty(n, np.int32)
+    solved = np.empty(n, np.int32)
+    dig = np.empty(n, np.uint64)
+    it, br, st = C.c_int(), C.c_int(), C.c_int()
+    rc = lib.orc_solve_bf(n, g.offsets, g.targets, g.owner, g.priority, max_iters, strat, inf, solved,
+                          dig, C.byref(it), C.byref(br), C.byref(st))
+    if rc < 0:
+        raise RuntimeError(lib.orc_last_error().decode())
+
+    class R:
+        pass
+    r = R()
+    r.strat, r.inf, r.solved, r.dig = strat, inf, solved, dig
+    r.iter, r.br, r.status = it.value, br.value, st.value
+    return r

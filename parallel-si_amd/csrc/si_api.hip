@@ -150,6 +150,35 @@ extern "C" int psi_switch_strategy(psi_solver* s, int player, int* switched)
 
 extern "C" int psi_reset_opponent_strategy(psi_solver* s) { return core_reset_opp(s->core); }
 
+static int check_game(psi_solver* s, int game);
+
+extern "C" int psi_solve_bf(psi_solver* s, int max_rounds, int steps_per_sync)
+{
+    Core& c = s->core;
+    if (int rc = core_init_strat(c)) return rc;
+    if (int rc = core_set_params(c, 0, max_rounds, 1)) return rc;
+    if (steps_per_sync <= 0) steps_per_sync = 4;
+    if (int rc = core_solve_bf(c, steps_per_sync)) return rc;
+    std::vector<GameCtl> gc(c.G);
+    PSI_HIP(hipMemcpy(gc.data(), c.gctl, c.G * sizeof(GameCtl), hipMemcpyDeviceToHost));
+    for (int g = 0; g < c.G; g++)
+        if (gc[g].err)
+            return fail(PSI_ERR_INVALID, "game " + std::to_string(g) +
+                                             ": Game contains an odd cycle that uses only odd nodes");
+    return PSI_OK;
+}
+
+extern "C" int psi_get_bf_status(psi_solver* s, int game, int* status)
+{
+    if (int rc = check_game(s, game)) return rc;
+    Core& c = s->core;
+    GameCtl gc;
+    PSI_HIP(hipMemcpyAsync(&gc, c.gctl + game, sizeof gc, hipMemcpyDeviceToHost, c.s));
+    PSI_HIP(hipStreamSynchronize(c.s));
+    *status = gc.err ? 1 : 0;
+    return PSI_OK;
+}
+
 extern "C" int psi_solve(psi_solver* s, int reset, int max_rounds, int steps_per_sync)
 {
     Core& c = s->core;

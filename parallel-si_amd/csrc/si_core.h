@@ -48,6 +48,9 @@ struct GameCtl {
     int vbase;      // first global vertex id of the game
     int n;          // vertices of the game
     int clear0;     // the last step was an outer step: restart the switch(0) change record
+    int bf;         // solve_bf (main.cpp:57-77): later best responses by relaxation
+    int bf_init;    // solve_bf: the first best response (an SI inner loop) is done
+    int err;        // solve_bf: odd strategy cycle in the first best response (cpu_bv.cpp:73-84)
 };
 
 // Jump-round bookkeeping (device memory), rewritten by every valuation.
@@ -102,6 +105,7 @@ struct Core {
     bool pool = false;                 // serving pool (games come and go)
     int maxn_cap = 0;                  // pool: longest admissible game
     int* cnt_scratch = nullptr;        // pool: owner-list block counts of one admission
+    int* succ = nullptr;               // solve_bf: CPU_BellmanFord's succ (allocated on first use)
     hipStream_t s = nullptr;
     // host copies of per-game sizes
     std::vector<int> h_n, h_vbase;
@@ -125,7 +129,7 @@ const char* core_validate_game(const GameView& gv, int* max_pri);
 int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap, hipStream_t s);
 void core_destroy(Core& c);
 int core_init_strat(Core& c);
-int core_set_params(Core& c, int reset, int max_rounds);
+int core_set_params(Core& c, int reset, int max_rounds, int bf = 0);
 // One valuation (val_init, jump rounds, labels); fuse_mark1 also applies
 // mark_solved(1) and counts the round (the device-resident loop's form).
 int core_valuation(Core& c, bool loop_form);
@@ -151,5 +155,9 @@ int core_pool_admit(Core& c, int slot, long long vbase, long long ebase, const G
 int core_pool_localize(Core& c, int slot, hipStream_t s2);
 int core_pool_release(Core& c, int slot, hipStream_t s2);
 int core_solve_profiled(Core& c, double* times, long long* counts);
+// solve_bf for every game of the core (after core_init_strat and
+// core_set_params(..., bf = 1)): the device loop, then the Odd strategy of
+// games with at least one relaxation replaced by succ (cpu_bf.cpp:156-165).
+int core_solve_bf(Core& c, int steps_per_check);
 
 } // namespace psi

@@ -158,6 +158,8 @@ __global__ void k_reset_ctl(int G, GameCtl* gc)
     c.reset_pending = 0;
     c.capped = 0;
     c.clear0 = 0;
+    c.bf_init = 0;
+    c.err = 0;
     gc[g] = c;
 }
 
@@ -337,7 +339,7 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int2
 __global__ void __launch_bounds__(kBS) k_label(
     long long N, int maxn, int lmax, const int2* __restrict__ Sl,
     const unsigned long long* __restrict__ Zl, const uint8_t* __restrict__ blast,
-    const int* __restrict__ gid, const GameCtl* __restrict__ gc, const int* __restrict__ strat,
+    const int* __restrict__ gid, GameCtl* __restrict__ gc, const int* __restrict__ strat,
     int8_t* __restrict__ solved, int8_t* __restrict__ lab, int* __restrict__ dep,
     int* __restrict__ key, unsigned long long* __restrict__ dig, unsigned long long* __restrict__ tr,
     int trace_cap, int loop_form, int trace, JumpCnt* __restrict__ jc,
@@ -390,6 +392,13 @@ __global__ void __launch_bounds__(kBS) k_label(
                     else if (kind < 0) {           // largest priority on the cycle
                         const int Fy = min(F, (int)blast[s.x / kBS]);
                         l = (Sl[(size_t)level_slot(Fy) * N + s.x].y & 1) ? -1 : 1;
+                        // solve_bf: the relaxation never closes a cycle (such
+                        // vertices stay infinite, cpu_bf.cpp:156-160); the
+                        // first best response stops on an odd one.
+                        if (gc[g].bf) {
+                            if (gc[g].bf_init) l = 1;
+                            else if (l == -1) gc[g].err = 1;
+                        }
                     } else {
                         l = 1;                     // kind 3 (finished game): never on a live path
                     }
@@ -702,16 +711,43 @@ __global__ void __launch_bounds__(kBS) k_switch(
 
 // mark_solved(0) of the games that take the outer step (fused form of
 // cpu_parity.cpp:306-315 as solve_si calls it, gpumain.cpp:44).
+// solve_bf games after a relaxation also record succ: the chosen successor of
+// every unsolved Odd vertex with a finite valuation (cpu_bf.cpp:120-147).
 __global__ void k_mark0_outer(long long N, const int* __restrict__ gid, const GameCtl* __restrict__ gc,
                               const int8_t* __restrict__ lab, int8_t* __restrict__ solved,
-                              const int* __restrict__ hi)
+                              const int* __restrict__ hi, const int* __restrict__ vinfo,
+                              const int* __restrict__ strat, int* __restrict__ succ)
 {
     const long long lim = min(N, (long long)*hi);
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < lim; v += stride) {
         int g = gid[v];
-        if (!gc[g].done && gc[g].outer && lab[v] == 1) solved[v] = 1;
+        if (gc[g].done || !gc[g].outer) continue;
+        const int l = lab[v];
+        if (succ && gc[g].bf && gc[g].bf_init && l == 0 && !solved[v] && owner_of(vinfo[v]) == 1)
+            succ[v] = strat[v];
+        if (l == 1) solved[v] = 1;
     }
+}
+
+// succ starts at the game's vertex 0 (cpu_bf.cpp:17).
+__global__ void k_succ_init(long long N, const int* __restrict__ gid, const GameCtl* __restrict__ gc,
+                            int* __restrict__ succ)
+{
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < N) succ[v] = gc[gid[v]].vbase;
+}
+
+// End of solve_bf: Odd's strategy is succ once a relaxation has run
+// (cpu_bf.cpp:161-164; solve_bf's first best response is an SI inner loop).
+__global__ void k_bf_final(long long N, const int* __restrict__ gid, const GameCtl* __restrict__ gc,
+                           const int* __restrict__ vinfo, const int* __restrict__ succ,
+                           int* __restrict__ strat)
+{
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const int g = gid[v];
+    if (gc[g].bf && gc[g].iter >= 2 && !gc[g].err && owner_of(vinfo[v]) == 1) strat[v] = succ[v];
 }
 
 // mark_solved(player) as a separate step (cpu_parity.cpp:306-315).
@@ -731,6 +767,11 @@ __global__ void k_ctl1(int G, GameCtl* gc)
     if (g >= G) return;
     GameCtl c = gc[g];
     if (c.done) return;
+    if (c.err) {                   // solve_bf: odd strategy cycle (the reference exits)
+        c.done = 1;
+        gc[g] = c;
+        return;
+    }
     c.br++;
     c.outer = c.sw1 == 0;
     c.sw1 = 0;
@@ -749,6 +790,7 @@ __global__ void k_ctl0(int G, GameCtl* gc)
     if (c.outer) {
         c.iter++;
         c.clear0 = 1;              // the next valuation restarts the switch(0) record
+        c.bf_init = 1;             // solve_bf: later best responses relax
         if (c.sw0 == 0) c.done = 1;
         else if (c.reset) c.reset_pending = 1;
     }
@@ -1259,7 +1301,7 @@ void core_destroy(Core& c)
     if (c.graph) (void)hipGraphExecDestroy(c.graph);
     void* ptrs[] = {c.vinfo, c.off, c.tgt, c.gid, c.strat, c.par, c.dep, c.key, c.solved, c.lab, c.blive,
                     c.bact, c.blast, c.Sl, c.own[0], c.own[1], c.Zl, c.dig, c.tr, c.gctl, c.jc, c.flag,
-                    c.cnt_scratch, c.chg};
+                    c.cnt_scratch, c.chg, c.succ};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c = Core();
@@ -1314,7 +1356,8 @@ static void launch_label(Core& c, int maxn, bool loop_form)
 static void launch_switch(Core& c, int player, bool loop_form)
 {
     if (loop_form && player == 0)
-        k_mark0_outer<<<SGRID(c.N)>>>(c.N, c.gid, c.gctl, c.lab, c.solved, c.flag + 2);
+        k_mark0_outer<<<SGRID(c.N)>>>(c.N, c.gid, c.gctl, c.lab, c.solved, c.flag + 2, c.vinfo,
+                                      c.strat, c.succ);
     if (c.own_n[player] == 0) return;
     k_switch<<<SGRID(c.own_n[player])>>>(player, c.N, c.own[player], c.own_n[player], c.vinfo,
                                         c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
@@ -1322,7 +1365,7 @@ static void launch_switch(Core& c, int player, bool loop_form)
                                         c.prof_ties, c.flag + 2, c.chg, c.chg + c.chg_words);
 }
 
-int core_set_params(Core& c, int reset, int max_rounds)
+int core_set_params(Core& c, int reset, int max_rounds, int bf)
 {
     std::vector<GameCtl> gc(c.G);
     PSI_HIP(hipMemcpyAsync(gc.data(), c.gctl, c.G * sizeof(GameCtl), hipMemcpyDeviceToHost, c.s));
@@ -1330,7 +1373,12 @@ int core_set_params(Core& c, int reset, int max_rounds)
     for (auto& x : gc) {
         x.reset = reset;
         x.max_rounds = max_rounds;
+        x.bf = bf;
     }
+    if (bf && !c.succ) {
+        if (int rc = dalloc(&c.succ, c.N)) return rc;
+    }
+    if (bf) k_succ_init<<<GRID(c.N)>>>(c.N, c.gid, c.gctl, c.succ);
     PSI_HIP(hipMemcpyAsync(c.gctl, gc.data(), c.G * sizeof(GameCtl), hipMemcpyHostToDevice, c.s));
     PSI_HIP(hipStreamSynchronize(c.s));
     return PSI_OK;
@@ -1532,6 +1580,15 @@ int core_solve(Core& c, int steps_per_check)
     PSI_HIP(hipMemcpyAsync(&jc, c.jc, sizeof jc, hipMemcpyDeviceToHost, c.s));
     PSI_HIP(hipStreamSynchronize(c.s));
     if (jc.err) return fail(PSI_ERR_INVALID, "internal: jump rounds exhausted");
+    return PSI_OK;
+}
+
+int core_solve_bf(Core& c, int steps_per_check)
+{
+    if (int rc = core_solve(c, steps_per_check)) return rc;
+    k_bf_final<<<GRID(c.N)>>>(c.N, c.gid, c.gctl, c.vinfo, c.succ, c.strat);
+    PSI_LAUNCH_CHECK();
+    PSI_HIP(hipStreamSynchronize(c.s));
     return PSI_OK;
 }
 

@@ -263,6 +263,8 @@ _sig("psi_mark_solved", C.c_int, _vp, C.c_int)
 _sig("psi_switch_strategy", C.c_int, _vp, C.c_int, C.POINTER(C.c_int))
 _sig("psi_reset_opponent_strategy", C.c_int, _vp)
 _sig("psi_solve", C.c_int, _vp, C.c_int, C.c_int, C.c_int)
+_sig("psi_solve_bf", C.c_int, _vp, C.c_int, C.c_int)
+_sig("psi_get_bf_status", C.c_int, _vp, C.c_int, C.POINTER(C.c_int))
 _sig("psi_get_stats", C.c_int, _vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
      C.POINTER(C.c_int))
 _sig("psi_get_winning", C.c_int, _vp, C.c_int, C.c_int, _i32p, C.POINTER(C.c_int))
@@ -306,6 +308,15 @@ class Solver:
 
     def solve(self, reset=False, max_rounds=0, steps_per_sync=4):
         check(lib.psi_solve(self.h, int(reset), max_rounds, steps_per_sync), "psi_solve")
+
+    def solve_bf(self, max_rounds=0, steps_per_sync=4):
+        """solve_bf with the Bellman-Ford best response (cpu_bf role)."""
+        check(lib.psi_solve_bf(self.h, max_rounds, steps_per_sync), "psi_solve_bf")
+
+    def bf_status(self, game=0):
+        s = C.c_int()
+        check(lib.psi_get_bf_status(self.h, game, C.byref(s)), "psi_get_bf_status")
+        return s.value
 
     def sync(self):
         check(lib.psi_solver_synchronize(self.h), "sync")

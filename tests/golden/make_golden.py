@@ -92,9 +92,44 @@ def make(name, g, modes, full, max_rounds=2000, gm_text=None):
     print(name, g.n, st)
 
 
+def make_bf():
+    """cpubf fixtures: games the reference's solve_bf completes, and games where
+    its first best response meets an odd strategy cycle and exits."""
+    rng = np.random.default_rng(77)
+    done_ok, done_abort, k = 0, 0, 0
+    while done_ok < 8 or done_abort < 2:
+        k += 1
+        n = int(rng.integers(4, 40))
+        if k % 2 == 0:
+            g = psi.gen_random_game(7000 + k, n, int(rng.integers(2, 12)), 1, 3)
+            g.priority[g.owner == 1] &= ~1      # Odd vertices even: no Odd-only odd cycles
+        else:
+            g = psi.gen_random_game(7000 + k, n, 4 * n, 1, 4)
+            g.priority[:] = rng.permutation(n).astype(np.int32)   # distinct priorities
+        with tempfile.NamedTemporaryFile("w", suffix=".gm", delete=False) as fh:
+            fh.write(g.to_gm())
+            path = fh.name
+        res = run_ref("cpubf", path, 100000)
+        os.unlink(path)
+        ok = res["final"] is not None and res["final"]["status"] == "ok"
+        if ok and done_ok < 8:
+            name, done_ok = f"bf_{done_ok:02d}", done_ok + 1
+        elif not ok and res["returncode"] != 0 and done_abort < 2:
+            name, done_abort = f"bf_abort_{done_abort:02d}", done_abort + 1
+        else:
+            continue
+        fx = {"name": name, "game": game_dict(g), "modes": {"cpubf": summarize(res, False)}}
+        with open(os.path.join(HERE, name + ".json"), "w") as fh:
+            json.dump(fx, fh, separators=(",", ":"))
+        print(name, g.n, res["final"]["status"] if res["final"] else res["message"])
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
+    if "--bf-only" in sys.argv:
+        make_bf()
+        return
     all_modes = ["cpu", "cpu-reset", "cpubv", "cpulist", "cpubf"]
     # 1. the reference's own input fixture
     text = open(TEST2).read()
@@ -118,6 +153,8 @@ def main():
     for seed, n in enumerate([12, 40, 150]):
         g = psi.gen_chain_game(3000 + seed, n)
         make(f"chain_{seed:02d}", g, ["cpu", "cpulist"], full=n <= 40, max_rounds=2000)
+    # 6. solve_bf (cpubf) fixtures
+    make_bf()
 
 
 if __name__ == "__main__":

@@ -42,7 +42,8 @@ def test_solve_matches_reference(name, mode, rep):
     assert got == exp["round_hashes"]
 
 
-@pytest.mark.parametrize("name", [fx["name"] for fx in FIXTURES if "rounds" in fx["modes"]["cpu"]])
+@pytest.mark.parametrize("name", [fx["name"] for fx in FIXTURES
+                                  if "cpu" in fx["modes"] and "rounds" in fx["modes"]["cpu"]])
 def test_full_round_arrays(name):
     fx = _fx(name)
     g = fixture_game(fx)
@@ -55,7 +56,7 @@ def test_full_round_arrays(name):
         assert [str(int(x)) for x in r.dig_rounds[k]] == e["dig"]
 
 
-@pytest.mark.parametrize("name", [fx["name"] for fx in FIXTURES])
+@pytest.mark.parametrize("name", [fx["name"] for fx in FIXTURES if "cpu" in fx["modes"]])
 def test_zielonka_agrees_with_reference_winners(name):
     fx = _fx(name)
     fin = fx["modes"]["cpu"]["final"]
@@ -185,3 +186,58 @@ def test_planted_mistakes_are_caught():
     fin = np.array([(u < 0 or inf[u] == 0) and (w < 0 or inf[w] == 0) for u, w in zip(us, ws)])
     planted = np.where(fin, -good, good)
     assert not np.array_equal(planted, good)
+
+
+BF_FIXTURES = [fx["name"] for fx in FIXTURES if "cpubf" in fx["modes"]]
+
+
+@pytest.mark.parametrize("name", BF_FIXTURES)
+def test_solve_bf_matches_reference(name):
+    """CPU_BellmanFord restatement vs the compiled reference's cpubf run:
+    final strategy (with succ for Odd), labels, iteration and relaxation-round
+    counts; games whose first best response meets an odd cycle abort."""
+    fx = _fx(name)
+    exp = fx["modes"]["cpubf"]
+    r = oracle.solve_bf(fixture_game(fx))
+    if exp["returncode"] != 0:
+        assert "odd cycle" in exp["message"]
+        assert r.status == oracle.BF_ODD_CYCLE
+        return
+    fin = exp["final"]
+    assert r.status == oracle.BF_OK
+    assert (r.iter, r.br) == (fin["iter"], fin["br"])
+    assert r.strat.tolist() == fin["strat"]
+    assert r.inf.tolist() == fin["inf"]
+
+
+@pytest.mark.parametrize("name", [n for n in BF_FIXTURES if _fx(n)["modes"]["cpubf"]["returncode"] == 0])
+def test_solve_bf_winners_are_the_games(name):
+    """On games solve_bf completes, its winning regions are the game's
+    (Zielonka) and CPUParity's."""
+    fx = _fx(name)
+    g = fixture_game(fx)
+    fin = fx["modes"]["cpubf"]["final"]
+    w = oracle.zielonka(g)
+    assert [v for v in range(g.n) if w[v] == 0] == fin["w0"]
+    r = oracle.solve(g)
+    assert [v for v in range(g.n) if r.inf[v] == 1] == fin["w0"]
+
+
+def test_solve_bf_relaxation_is_a_best_response():
+    """Property of the fixed point the relaxation reaches (independent of
+    the update order): every unsolved finite Odd vertex's valuation is the
+    minimum over its successors, so SI switch(1) on the final state switches
+    nothing once Odd follows succ (checked with the dense CPUParity compare)."""
+    rng = np.random.default_rng(3)
+    checked = 0
+    for k in range(60):
+        n = int(rng.integers(4, 30))
+        g = psi.gen_random_game(8100 + k, n, int(rng.integers(2, 9)), 1, 3)
+        g.priority[g.owner == 1] &= ~1
+        r = oracle.solve_bf(g)
+        if r.status != oracle.BF_OK or r.iter < 2:
+            continue
+        cnt, after = oracle.switch(g, r.strat, r.solved, r.inf, 1)
+        assert cnt == 0, k
+        checked += 1
+    assert checked >= 10
