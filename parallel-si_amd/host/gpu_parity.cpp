@@ -46,6 +46,13 @@ void GPUParity::solve(bool reset, int* iter_count, int* br_count)
     check(psi_get_stats(s, 0, iter_count, br_count, &capped), "solve");
 }
 
+void GPUParity::solve_bf(int* iter_count, int* br_count)
+{
+    check(psi_solve_bf(s, 0, 4), "solve_bf");
+    int capped = 0;
+    check(psi_get_stats(s, 0, iter_count, br_count, &capped), "solve_bf");
+}
+
 std::vector<int> GPUParity::strategy()
 {
     std::vector<int> out(g_dev.num_vert);

@@ -39,6 +39,10 @@ public:
 
     // Device-resident solve_si; returns (iter_count, br_count).
     void solve(bool reset, int* iter_count, int* br_count);
+    // Device-resident solve_bf (main.cpp:57-77): the cpu_bf role, Bellman-Ford
+    // best responses for Odd.  Throws on an odd strategy cycle in the first
+    // best response (the reference prints the message and exits).
+    void solve_bf(int* iter_count, int* br_count);
     std::vector<int> strategy();
     std::vector<int> labels();
     psi_solver* handle() { return s; }

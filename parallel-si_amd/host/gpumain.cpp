@@ -1,11 +1,14 @@
 // gpuparsi: the reference's GPU command line (gpumain.cpp:102-146) on MI355X.
 //
-//   gpuparsi [options] <gpu|gpulist> <game.gm | game.gm.gz>
+//   gpuparsi [options] <gpu|gpulist|gpubf> <game.gm | game.gm.gz>
 //
 // Prints "<seconds> <iter_count> <br_count>" for the solve_si region, as the
 // reference's test() does (gpumain.cpp:56-71).  By default the whole solve_si
 // loop runs on the device; --host-loop drives it from the host through the
-// StrategyImprovement calls exactly as gpumain.cpp:18-53 does.
+// StrategyImprovement calls exactly as gpumain.cpp:18-53 does.  gpubf runs
+// solve_bf (main.cpp:57-77, the CPU command line's cpubf) on the device; an
+// odd strategy cycle in its first best response prints the reference's
+// message and exits with status 1, as cpu_bv.cpp:73-84 does.
 // Options: --host-loop, --reset (the cpu-reset schedule), --winning (print W0
 // and W1), --strategy (print the final strategy), --json (one JSON line).
 // Exit status: 0 ok, 1 usage, 255 parse error ("parsing error: ..." on stderr,
@@ -92,13 +95,13 @@ int main(int argc, char** argv)
         std::cout << "usage: " << argv[0] << " [--host-loop] [--reset] [--winning] [--strategy] [--json]"
                   << " [algorithm] [game]" << std::endl
                   << std::endl
-                  << "    algorithms: gpu, gpulist" << std::endl
+                  << "    algorithms: gpu, gpulist, gpubf" << std::endl
                   << std::endl;
         return 1;
     }
     const std::string& algorithm = pos[0];
     const std::string& filename = pos[1];
-    if (algorithm != "gpu" && algorithm != "gpulist") {
+    if (algorithm != "gpu" && algorithm != "gpulist" && algorithm != "gpubf") {
         std::cout << "unknown algorithm: " << algorithm << std::endl;
         return 1;
     }
@@ -118,11 +121,25 @@ int main(int argc, char** argv)
     }
     try {
         GPUGame gpu_g(g);
-        GPUParity* si = algorithm == "gpu" ? new GPUParity(g, gpu_g) : new GPUList(g, gpu_g);
+        GPUParity* si = algorithm == "gpulist" ? new GPUList(g, gpu_g) : new GPUParity(g, gpu_g);
         int iter = 0, br = 0;
         auto t0 = std::chrono::steady_clock::now();
-        if (host_loop) solve_si(*si, &br, &iter, reset);
-        else si->solve(reset, &iter, &br);
+        if (algorithm == "gpubf") {
+            try {
+                si->solve_bf(&iter, &br);
+            } catch (const std::exception& e) {
+                if (std::strstr(e.what(), "odd cycle")) {
+                    std::cout << "Game contains an odd cycle that uses only odd nodes" << std::endl;
+                    delete si;
+                    return 1;
+                }
+                throw;
+            }
+        } else if (host_loop) {
+            solve_si(*si, &br, &iter, reset);
+        } else {
+            si->solve(reset, &iter, &br);
+        }
         auto t1 = std::chrono::steady_clock::now();
         double sec = std::chrono::duration<double>(t1 - t0).count();
         std::vector<int> w0, w1;

@@ -62,3 +62,25 @@ def test_cli_errors(tmp_path):
     assert rc == 255 and "parsing error: invalid player" in err
     rc, out, err = run(["gpu"])
     assert rc == 1 and "usage" in out
+
+
+@pytest.mark.parametrize("case", ["bf_00", "bf_03", "bf_06", "bf_abort_00"])
+def test_cli_gpubf_matches_reference(tmp_path, case):
+    """gpubf: solve_bf on the device against the reference's cpubf run."""
+    fx = next(f for f in load_fixtures() if f["name"] == case)
+    g = fixture_game(fx)
+    path = tmp_path / "g.gm"
+    path.write_text(g.to_gm())
+    print(f"cli gpubf {case}", flush=True)
+    rc, out, err = run(["--json", "--winning", "--strategy", "gpubf", str(path)])
+    exp = fx["modes"]["cpubf"]
+    if exp["returncode"] != 0:
+        assert rc == 1 and "odd cycle that uses only odd nodes" in out
+        return
+    assert rc == 0, err
+    res = json.loads(out)
+    fin = exp["final"]
+    assert res["iter"] == fin["iter"]
+    assert res["w0"] == fin["w0"] and res["w1"] == fin["w1"]
+    even = g.owner == 0
+    assert np.array_equal(np.array(res["strategy"])[even], np.array(fin["strat"])[even])
