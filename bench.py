@@ -15,7 +15,9 @@ One JSON line on stdout.  Besides the contract keys it carries
   listrank     - 16,777,216-node list and 1,048,576-list forest rankings;
   batch        - 1,024 mixed-size games in one handle vs one at a time;
   serve        - Poisson arrivals into the serving pool: p50/p99 latency and
-                 throughput.
+                 throughput;
+  bf           - solve_bf (the cpu_bf role) at 1M vertices, and a 16K-vertex
+                 sample timed on the GPU and with the oracle on one core.
 Progress goes to stderr.
 """
 import argparse
@@ -283,6 +285,56 @@ def serve_bench(psi, requests=96, rate=2.0, seed=4, budget_gib=64.0, steps=4, lo
             "served_equals_alone": bool(same), "checked_alone": len(kept)}
 
 
+def bf_game(psi, n, seed):
+    """Random game of the 4M generator shape (out-degree U[2, 8]) with 64
+    priorities, Odd vertices' priorities made even: no strategy cycle of Odd
+    vertices alone is odd, so solve_bf never stops on one (the reference
+    exits there)."""
+    g = psi.gen_random_game(seed, n, 64, 2, 8)
+    g.priority[g.owner == 1] &= ~1
+    return g
+
+
+def bf_bench(psi, n=1 << 20, seed=6, reps=3, cpu_n=1 << 14, cpu=True):
+    """solve_bf (the cpu_bf role) on the device; the same solve of a smaller
+    game of the same generator timed on the GPU and with the oracle's
+    CPU_BellmanFord restatement (one core) for a like-for-like ratio."""
+    g = bf_game(psi, n, seed)
+    s = psi.Solver(g)
+    s.solve_bf()
+    s.sync()
+    t = time.perf_counter()
+    for _ in range(reps):
+        s.solve_bf()
+    s.sync()
+    dt = (time.perf_counter() - t) / reps
+    it, br, _ = s.stats()
+    s.close()
+    out = {"workload": f"solve_bf, n={n}, priority U[0,64) with Odd's made even, out-degree U[2,8], "
+                       f"seed {seed}", "n": n, "m": g.m, "iter": it, "br_device_rounds": br,
+           "ms": round(dt * 1e3, 3), "vertex_iterations_per_s": round(n * br / dt, 1)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        gs = bf_game(psi, cpu_n, seed)
+        s = psi.Solver(gs)
+        s.solve_bf()
+        s.sync()
+        t = time.perf_counter()
+        s.solve_bf()
+        s.sync()
+        gpu_s = time.perf_counter() - t
+        t = time.perf_counter()
+        r = oracle.solve_bf(gs)
+        cpu_s = time.perf_counter() - t
+        same = r.iter == s.stats()[0] and bool(np.array_equal(r.inf, s.labels()))
+        s.close()
+        out["sample"] = {"n": cpu_n, "gpu_ms": round(gpu_s * 1e3, 3), "cpu_ms": round(cpu_s * 1e3, 1),
+                         "cpu_kind": "port (oracle CPU_BellmanFord, dense rows, one core)",
+                         "iter": r.iter, "cpu_relaxation_sweeps": r.br, "gpu_equals_cpu": same}
+    return out
+
+
 def measured_traffic(n, seed):
     """HBM bytes per active jump round of this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None for
@@ -318,9 +370,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-games", type=int, default=1024)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-bf", action="store_true")
     ap.add_argument("--requests", type=int, default=96, help="serving: requests")
     ap.add_argument("--rate", type=float, default=2.0, help="serving: arrivals per second")
-    ap.add_argument("--only", choices=["si", "listrank", "batch", "serve", "copy"], default=None,
+    ap.add_argument("--only", choices=["si", "listrank", "batch", "serve", "bf", "copy"], default=None,
                     help="run one workload only (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -347,8 +400,11 @@ def main():
     if a.only == "serve":
         print(json.dumps({"serve": serve_bench(psi, a.requests, a.rate)}), flush=True)
         return
+    if a.only == "bf":
+        print(json.dumps({"bf": bf_bench(psi)}), flush=True)
+        return
     if a.only == "si":
-        a.no_listrank = a.no_cpu = a.no_batch = a.no_serve = True
+        a.no_listrank = a.no_cpu = a.no_batch = a.no_serve = a.no_bf = True
     g, s, it, br, wall, prof = si_bench(psi, a.n, a.steps, a.warmup, a.seed)
     value = g.n * br * a.steps / wall
     # Roofline of the dominant kernel: a pointer-jumping round reads and
@@ -389,6 +445,9 @@ def main():
         line["listrank"] = listrank_bench(psi)
     if not a.no_batch:
         line["batch"] = batch_bench(psi, a.batch_games)
+    if not a.no_bf:
+        log("[bench] solve_bf ...")
+        line["bf"] = bf_bench(psi)
     if not a.no_serve:
         log("[bench] serving ...")
         line["serve"] = serve_bench(psi, a.requests, a.rate)
