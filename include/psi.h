@@ -71,6 +71,19 @@ int psi_list_rank_host(const int* succ, int n, const int* heads, int n_heads, in
  * end) and the number of overflow sublists the level-0 walk created. */
 int psi_list_rank_status(int* error_bits, int* overflow_used);
 
+/* Euler-tour path sums over a forest of links: the valuation engine of
+ * GPUList (gpu_list_ranking.cu:52-92 d_init_list / d_build_list: the tour of
+ * the sink tree, one atomic exchange per vertex into its parent's list;
+ * :96-277 splitters, sublist walks, reduced list, broadcast: here psi_list_rank
+ * on the 2n+2-node tour and a scan in tour order), with a 64-bit weight per
+ * vertex instead of a dense count vector.  link[v] in [-1, n) or below -1:
+ * -1 = child of the root (the sink), below -1 = detached.  sum[v] = sum of w
+ * over v and its ancestors (mod 2^64), reach[v] = 1, for every v whose links
+ * reach the root; sum 0, reach 0 otherwise (cycles, detached chains).  Device
+ * pointers; method PSI_LR_AUTO, HJ or WALK; reach may be null. */
+int psi_tour_path_sums(const int* d_link, int n, const uint64_t* d_w, uint64_t* d_sum, int* d_reach,
+                       int method);
+
 /* Tuning knobs (the autotuner sets these). */
 #define PSI_LRT_LOG_D0 1        /* level-0 splitter density 2^-v                 */
 #define PSI_LRT_LOCAL_BITS 2    /* level-0 hop cap 2^v                           */

@@ -231,6 +231,21 @@ def list_rank_device(d_succ, n, d_heads, n_heads, d_rank, method=LR_AUTO):
           "psi_list_rank")
 
 
+_sig("psi_tour_path_sums", C.c_int, _vp, C.c_int, _vp, _vp, _vp, C.c_int)
+
+
+def tour_path_sums(link, w, method=LR_AUTO):
+    """Host wrapper of psi_tour_path_sums: (sum uint64[n], reach int32[n])."""
+    link = np.ascontiguousarray(link, np.int32)
+    w = np.ascontiguousarray(w, np.uint64)
+    n = link.shape[0]
+    d_link, d_w = DeviceArray.from_host(link), DeviceArray.from_host(w)
+    d_sum, d_reach = DeviceArray(8 * max(n, 1)), DeviceArray(4 * max(n, 1))
+    check(lib.psi_tour_path_sums(d_link.ptr, n, d_w.ptr, d_sum.ptr, d_reach.ptr, method),
+          "psi_tour_path_sums")
+    return d_sum.to_host(np.uint64, n), d_reach.to_host(np.int32, n)
+
+
 def list_rank_status():
     bits, over = C.c_int(), C.c_int()
     check(lib.psi_list_rank_status(C.byref(bits), C.byref(over)), "status")
