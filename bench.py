@@ -105,6 +105,42 @@ def listrank_bench(psi, reps=10):
     return out
 
 
+def tour_bench(psi, g, reps=5):
+    """Euler-tour path sums (the GPUList valuation engine: tour build, HJ
+    ranking of the 2n+2-node tour, scan) over the final strategy forest of the
+    4M game, weights = splitmix64(priority) (the valuation digests)."""
+    s = psi.Solver(g)
+    s.solve()
+    st, solved = s.strategy(), s.solved()
+    s.close()
+    link = np.where(solved == 1, -2, st).astype(np.int32)
+    x = g.priority.astype(np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    w = x ^ (x >> np.uint64(31))
+    n = g.n
+    d_link, d_w = psi.DeviceArray.from_host(link), psi.DeviceArray.from_host(w)
+    d_sum, d_reach = psi.DeviceArray(8 * n), psi.DeviceArray(4 * n)
+    run = lambda: psi.check(psi.lib.psi_tour_path_sums(d_link.ptr, n, d_w.ptr, d_sum.ptr,
+                                                        d_reach.ptr, psi.LR_AUTO), "tour")
+    run()
+    psi.synchronize()
+    timer = psi.Timer()
+    timer.start()
+    for _ in range(reps):
+        run()
+    timer.stop()
+    psi.synchronize()
+    ms = timer.ms() / reps
+    reach = int(d_reach.to_host(np.int32, n).sum())
+    # algorithmic bytes: links 4 + weights 8 + sums 8 per vertex, tour succ 8 + rank 8 per vertex
+    # (2 nodes), scan in/out 32 per vertex
+    alg = 68.0 * n
+    return {"n": n, "tour_nodes": 2 * n + 2, "reached": reach, "us": round(ms * 1e3, 1),
+            "alg_GBps": round(alg / (ms * 1e-3) / 1e9, 1),
+            "roofline_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def gen_games_parallel(psi, specs, threads=16):
     """specs: list of (kind, seed, n, pri_bound, deg_lo, deg_hi); generated on a
     thread pool (the generator releases the GIL inside the C call)."""
@@ -443,6 +479,7 @@ def main():
     s.close()
     if not a.no_listrank:
         line["listrank"] = listrank_bench(psi)
+        line["listrank"]["tour_path_sums_4M"] = tour_bench(psi, g)
     if not a.no_batch:
         line["batch"] = batch_bench(psi, a.batch_games)
     if not a.no_bf:

@@ -26,10 +26,11 @@ namespace {
 
 constexpr int kTB = 256;
 
-__global__ void k_tour_first(int n, int* __restrict__ first)
+__global__ void k_tour_first(int n, int* __restrict__ first, int* __restrict__ head)
 {
     int v = blockIdx.x * kTB + threadIdx.x;
     if (v <= n) first[v] = -1;
+    if (v == 0) *head = 2 * n;
 }
 
 __device__ __forceinline__ int tour_parent(const int* __restrict__ link, int n, int v)
@@ -129,15 +130,11 @@ extern "C" int psi_tour_path_sums(const int* d_link, int n, const uint64_t* d_w,
     uint64_t* sc = (uint64_t*)p; p += b_s;
     void* tmp = p; p += al(scan_bytes);
     int* head = (int*)p;
-    const int hv = 2 * n;
-    PSI_HIP(hipMemcpyAsync(head, &hv, sizeof(int), hipMemcpyHostToDevice, s));
-    k_tour_first<<<nblocks(n + 1, kTB), kTB, 0, s>>>(n, first);
+    k_tour_first<<<nblocks(n + 1, kTB), kTB, 0, s>>>(n, first, head);
     k_tour_push<<<nblocks(n, kTB), kTB, 0, s>>>(n, d_link, first, sib);
     k_tour_succ<<<nblocks(n + 1, kTB), kTB, 0, s>>>(n, d_link, first, sib, succ);
     k_tour_zero<uint64_t><<<nblocks(L, kTB), kTB, 0, s>>>(a, L);
     PSI_LAUNCH_CHECK();
-    // The head's constant must be in place before the ranking reads it.
-    PSI_HIP(hipStreamSynchronize(s));
     if (int rc = psi_list_rank(succ, (int)L, head, 1, rank, (method & 0xff) | PSI_LR_CHECKED)) return rc;
     k_tour_scatter<<<nblocks(L, kTB), kTB, 0, s>>>(n, rank, d_w, a);
     PSI_LAUNCH_CHECK();
