@@ -75,9 +75,16 @@ def si_bench(psi, n, steps, warmup, seed):
     return g, s, it, br, wall, prof
 
 
-def listrank_bench(psi, reps=10):
+def listrank_bench(psi, reps=10, tune=True):
     out = {}
     n = 1 << 24
+    if tune:
+        import autotune
+        t0 = time.perf_counter()
+        cfg, sec, trials = autotune.tune_list_rank(n)
+        out["autotuned"] = {"config": cfg, "trials": len(trials),
+                            "tuning_s": round(time.perf_counter() - t0, 2)}
+        log(f"[bench] list ranking autotuned: {cfg} ({len(trials)} trials)")
     succ, heads = psi.gen_list(1, n)
     cases = [("list_16M", succ, heads, psi.LR_HJ)]
     fs, fh, _ = psi.gen_forest(2, n, 1 << 20, 4096)
@@ -167,11 +174,19 @@ def batch_bench(psi, n_games=1024, seed=3, lo=1000, hi=1000000, check_alone=True
                                      for g, n in enumerate(sizes)])
     log(f"[bench] batch: generated {n_games} games, sum n={int(sizes.sum())} in "
         f"{time.time() - t0:.1f}s")
+    # launch shapes and steps per sync from the autotuner, on every 16th game
+    # (outside the timed regions; both forms below run with the result)
+    import autotune
+    t0 = time.perf_counter()
+    tuned, _, trials = autotune.tune_si(games[::16])
+    tuning_s = time.perf_counter() - t0
+    steps = tuned["steps_per_sync"]
+    log(f"[bench] batch: autotuned {tuned} ({len(trials)} trials, {tuning_s:.1f}s)")
     # batched: one handle, every game with its own convergence
     t = time.perf_counter()
     b = psi.Solver(games)
     t1 = time.perf_counter()
-    b.solve(steps_per_sync=4)
+    b.solve(steps_per_sync=steps)
     t2 = time.perf_counter()
     res_b = b.results()
     t3 = time.perf_counter()
@@ -190,7 +205,7 @@ def batch_bench(psi, n_games=1024, seed=3, lo=1000, hi=1000000, check_alone=True
     res_a = []
     for g in games:
         s = psi.Solver(g)
-        s.solve(steps_per_sync=4)
+        s.solve(steps_per_sync=steps)
         res_a.append((s.stats(), s.strategy(), s.labels()))
         s.close()
     t_alone = time.perf_counter() - t
@@ -205,7 +220,9 @@ def batch_bench(psi, n_games=1024, seed=3, lo=1000, hi=1000000, check_alone=True
             "games_per_s_one_at_a_time": round(n_games / t_alone, 1),
             "speedup": round(t_alone / t_batch, 2), "br_max": int(max(br)),
             "vertex_iterations": int(sum(int(sz) * b_ for sz, b_ in zip(sizes, br))),
-            "batched_equals_alone": bool(same)}
+            "batched_equals_alone": bool(same),
+            "autotuned": {"config": tuned, "trials": len(trials), "tuning_s": round(tuning_s, 2),
+                          "sample": "every 16th game of the batch"}}
 
 
 def serve_bench(psi, requests=96, rate=2.0, seed=4, budget_gib=64.0, steps=4, lookahead=64):

@@ -247,6 +247,16 @@ int psi_pool_result(psi_pool* p, int64_t ticket, int* iter, int* br, int* capped
 int psi_pool_info(psi_pool* p, int* running, int* queued, int64_t* vertex_cap, int64_t* edge_cap,
                   int64_t* vertices_free, int64_t* largest_free);
 
+/* ---- launch-shape tuning (the autotuner sets these) -----------------------
+ * Grid sizes (blocks of 256 threads) of the SI step kernels, which all run
+ * grid-stride loops; they apply to step graphs captured afterwards.  No
+ * reference counterpart (the reference hard-codes its launch shapes,
+ * gpu_parity.cu:10-14, gpu_list_ranking.cu:19). */
+#define PSI_SIT_JUMP_GRID 1  /* pointer-jumping rounds (default 2048)          */
+#define PSI_SIT_STEP_GRID 2  /* val_init / label / switch / mark (16384)       */
+#define PSI_SIT_POOL_GRID 3  /* the same kernels in a serving pool (4096)      */
+int psi_si_tune(int param, int value);
+
 /* ---- seeded generators (host only, no device calls) --------------------- */
 /* Random game: owner Bernoulli(1/2), priority U[0,pri_bound), out-degree
  * U[deg_lo,deg_hi] distinct uniform successors; CSR with offsets[n+1]. */

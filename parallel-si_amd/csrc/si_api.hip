@@ -152,6 +152,17 @@ extern "C" int psi_reset_opponent_strategy(psi_solver* s) { return core_reset_op
 
 static int check_game(psi_solver* s, int game);
 
+extern "C" int psi_si_tune(int param, int value)
+{
+    if (value < 64 || value > (1 << 20)) return fail(PSI_ERR_INVALID, "psi_si_tune: value out of range");
+    switch (param) {
+    case PSI_SIT_JUMP_GRID: g_si_tune.jump_grid = (unsigned)value; return PSI_OK;
+    case PSI_SIT_STEP_GRID: g_si_tune.step_grid = (unsigned)value; return PSI_OK;
+    case PSI_SIT_POOL_GRID: g_si_tune.pool_grid = (unsigned)value; return PSI_OK;
+    default: return fail(PSI_ERR_INVALID, "psi_si_tune: bad parameter");
+    }
+}
+
 extern "C" int psi_solve_bf(psi_solver* s, int max_rounds, int steps_per_sync)
 {
     Core& c = s->core;

@@ -115,6 +115,14 @@ struct Core {
     int graph_steps = 0;
 };
 
+// Launch shapes of the SI step kernels (grid-stride launches).
+struct SiTuning {
+    unsigned jump_grid = 2048;   // blocks of a pointer-jumping round
+    unsigned step_grid = 16384;  // blocks of val_init / label / switch / mark
+    unsigned pool_grid = 4096;   // the same in a serving pool
+};
+extern SiTuning g_si_tune;
+
 // One game's host CSR arrays (local vertex ids).
 struct GameView {
     int n;

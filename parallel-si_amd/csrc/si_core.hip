@@ -1312,18 +1312,13 @@ void core_destroy(Core& c)
 #define JGRID(n) std::min(nblocks((n), kBS), jump_grid_cap()), kBS, 0, c.s
 // Step kernels: capped grid-stride launches, so that a serving pool's large
 // capacity costs nothing above its occupied bound.
-#define SGRID(n) std::min(nblocks((n), kBS), c.pool ? 4096u : 16384u), kBS, 0, c.s
+#define SGRID(n) std::min(nblocks((n), kBS), c.pool ? g_si_tune.pool_grid : g_si_tune.step_grid), kBS, 0, c.s
 
-// Largest jump-round grid (PSI_JUMP_GRID overrides 2048 for tuning).
-static unsigned jump_grid_cap()
-{
-    static unsigned cap = [] {
-        const char* e = getenv("PSI_JUMP_GRID");
-        long v = e ? strtol(e, nullptr, 10) : 0;
-        return v >= 64 && v <= (1 << 20) ? (unsigned)v : 2048u;
-    }();
-    return cap;
-}
+// Launch-shape knobs (psi_si_tune; the autotuner sets them).  They apply to
+// graphs captured afterwards.
+SiTuning g_si_tune;
+
+static unsigned jump_grid_cap() { return g_si_tune.jump_grid; }
 
 static int maxn_of(const Core& c)
 {

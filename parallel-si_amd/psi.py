@@ -252,6 +252,14 @@ def list_rank_status():
     return bits.value, over.value
 
 
+SIT_JUMP_GRID, SIT_STEP_GRID, SIT_POOL_GRID = 1, 2, 3
+_sig("psi_si_tune", C.c_int, C.c_int, C.c_int)
+
+
+def si_tune(param, value):
+    check(lib.psi_si_tune(param, value), "psi_si_tune")
+
+
 def list_rank_tune(param, value):
     check(lib.psi_list_rank_tune(param, value), "psi_list_rank_tune")
 

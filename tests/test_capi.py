@@ -71,3 +71,15 @@ def test_size_lists_are_log_uniform_bounds():
     import psi
     s = psi.gen_sizes(11, 1024, 1000, 1000000)
     assert s.min() >= 1000 and s.max() <= 1000000
+
+
+def test_autotune_descent_keeps_only_clear_wins():
+    """Coordinate descent of the autotuner: a knob value is kept only if it
+    is at least 2% faster than the best so far."""
+    import autotune
+    cost = {("a", 1): 1.0, ("a", 2): 0.99, ("a", 3): 0.5, ("b", 1): 1.0, ("b", 2): 1.2}
+
+    def measure(cfg):
+        return cost[("a", cfg["a"])] * cost[("b", cfg["b"])]
+    best, t, trials = autotune._descend({"a": 1, "b": 1}, {"a": [1, 2, 3], "b": [1, 2]}, measure)
+    assert best == {"a": 3, "b": 1} and t == 0.5 and len(trials) == 4
