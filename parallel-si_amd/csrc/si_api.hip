@@ -17,6 +17,7 @@ struct psi_game {
 struct psi_solver {
     Core core;
     bool trace = false;
+    hipStream_t s = nullptr;   // the handle's own stream: handles solve concurrently
 };
 
 extern "C" int psi_game_create(int n, int m, const int* owner, const int* priority,
@@ -70,10 +71,14 @@ static int create_from_views(int n_games, const GameView* views, int flags, int 
     if (int rc = psi_init()) return rc;
     auto* s = new psi_solver;
     s->trace = (flags & PSI_SOLVER_TRACE) != 0;
-    int rc = core_create(s->core, n_games, views, s->trace, trace_cap, stream());
+    hipError_t e = hipStreamCreateWithFlags(&s->s, hipStreamNonBlocking);
+    int rc = e == hipSuccess ? PSI_OK : hip_fail(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+    if (!rc) rc = core_create(s->core, n_games, views, s->trace, trace_cap, s->s);
     if (!rc) rc = core_init_strat(s->core);
     if (rc) {
+        if (s->s) (void)hipStreamSynchronize(s->s);
         core_destroy(s->core);
+        if (s->s) (void)hipStreamDestroy(s->s);
         delete s;
         return rc;
     }
@@ -118,8 +123,9 @@ extern "C" int psi_solver_create(const psi_game* g, int flags, int trace_cap, ps
 extern "C" int psi_solver_destroy(psi_solver* s)
 {
     if (!s) return PSI_OK;
-    (void)hipStreamSynchronize(s->core.s);
+    (void)hipStreamSynchronize(s->s);
     core_destroy(s->core);
+    (void)hipStreamDestroy(s->s);
     delete s;
     return PSI_OK;
 }

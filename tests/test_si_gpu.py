@@ -221,3 +221,27 @@ def test_deep_sink_trees(gpu, podd):
     print(f"deep ladder podd={podd}", flush=True)
     r, s = _solve_both(gpu, g)
     _assert_same(r, s)
+
+
+def test_handles_solve_concurrently(gpu):
+    """Distinct handles on distinct host threads (each handle has its own
+    stream and step graphs): every result equals the oracle's."""
+    import threading
+    games = [gpu.gen_random_game(500 + k, 20000 + 3000 * k, 500, 1, 5) for k in range(4)]
+    exp = [oracle.solve(g) for g in games]
+    out = [None] * len(games)
+
+    def run(k):
+        s = gpu.Solver(games[k])
+        for _ in range(3):
+            s.solve()
+        out[k] = (s.stats(), s.strategy(), s.labels())
+        s.close()
+    th = [threading.Thread(target=run, args=(k,)) for k in range(len(games))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for (st, strat, lab), r in zip(out, exp):
+        assert st[:2] == (r.iter, r.br)
+        assert np.array_equal(strat, r.strat) and np.array_equal(lab, r.inf)
