@@ -184,6 +184,12 @@ int psi_get_stats(psi_solver* s, int game, int* iter, int* br, int* capped);
 #define PSI_K_COUNT 12
 int psi_solve_profiled(psi_solver* s, int reset, int max_rounds, double* times_ms,
                        int64_t* counts);  /* both arrays have PSI_K_COUNT entries */
+/* The step program (PSI_SIT_PROGRAM) launched one launch at a time with HIP
+ * events around each: the same classes by the phase each launch ran (jump
+ * rounds after convergence are never launched; mark_solved(0) counts with
+ * switch(0); TOTAL counts steps).  Not on a serving pool's solver. */
+int psi_solve_program_profiled(psi_solver* s, int reset, int max_rounds, double* times_ms,
+                               int64_t* counts);
 
 /* Results of one game, local vertex ids.  get_winning: gpu_parity.cu:380-394. */
 int psi_get_winning(psi_solver* s, int game, int player, int* out, int* len);
@@ -201,7 +207,10 @@ int psi_get_digests(psi_solver* s, int game, uint64_t* out);
 int psi_get_trace(psi_solver* s, int game, uint64_t* hashes, int cap, int* count);
 /* Pointer-jumping statistics of the last valuation: out[2r] vertices that
  * reached a terminal in round r, out[2r+1] still jumping after it; *last =
- * last round that ran; *err = 1 if the launched rounds were not enough. */
+ * last round that ran; *err = 1 if the rounds were not enough.  The counts are
+ * those of valuations launched round by round (psi_compute_valuation,
+ * psi_solve_profiled, psi_solve with PSI_SIT_PROGRAM 0); the step program
+ * keeps only *last and *err. */
 int psi_get_jump_stats(psi_solver* s, int* out, int rounds, int* last, int* err);
 /* Loads strategy (-1 sink), solved flags and labels of solved vertices. */
 int psi_set_state(psi_solver* s, int game, const int* strat, const int* solved,
@@ -255,6 +264,11 @@ int psi_pool_info(psi_pool* p, int* running, int* queued, int64_t* vertex_cap, i
 #define PSI_SIT_JUMP_GRID 1  /* pointer-jumping rounds (default 2048)          */
 #define PSI_SIT_STEP_GRID 2  /* val_init / label / switch / mark (16384)       */
 #define PSI_SIT_POOL_GRID 3  /* the same kernels in a serving pool (4096)      */
+#define PSI_SIT_PHASE_GRID 4 /* step-program launches (2048)                   */
+/* PSI_SIT_PROGRAM: 1 (default) runs psi_solve's loop as a step program (each
+ * launch runs the next phase of the loop: no launch is spent on a jump round
+ * after convergence); 0 launches every phase of every step. */
+#define PSI_SIT_PROGRAM 5
 int psi_si_tune(int param, int value);
 
 /* ---- seeded generators (host only, no device calls) --------------------- */

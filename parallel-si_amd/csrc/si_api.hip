@@ -160,8 +160,14 @@ static int check_game(psi_solver* s, int game);
 
 extern "C" int psi_si_tune(int param, int value)
 {
+    if (param == PSI_SIT_PROGRAM) {
+        if (value != 0 && value != 1) return fail(PSI_ERR_INVALID, "psi_si_tune: program is 0 or 1");
+        g_si_tune.program = value;
+        return PSI_OK;
+    }
     if (value < 64 || value > (1 << 20)) return fail(PSI_ERR_INVALID, "psi_si_tune: value out of range");
     switch (param) {
+    case PSI_SIT_PHASE_GRID: g_si_tune.phase_grid = (unsigned)value; return PSI_OK;
     case PSI_SIT_JUMP_GRID: g_si_tune.jump_grid = (unsigned)value; return PSI_OK;
     case PSI_SIT_STEP_GRID: g_si_tune.step_grid = (unsigned)value; return PSI_OK;
     case PSI_SIT_POOL_GRID: g_si_tune.pool_grid = (unsigned)value; return PSI_OK;
@@ -213,6 +219,18 @@ extern "C" int psi_solve_profiled(psi_solver* s, int reset, int max_rounds, doub
     if (int rc = core_set_params(c, reset, max_rounds)) return rc;
     std::vector<long long> cn(PSI_K_COUNT);
     int rc = core_solve_profiled(c, times_ms, cn.data());
+    for (int k = 0; k < PSI_K_COUNT; k++) counts[k] = cn[k];
+    return rc;
+}
+
+extern "C" int psi_solve_program_profiled(psi_solver* s, int reset, int max_rounds, double* times_ms,
+                                          int64_t* counts)
+{
+    Core& c = s->core;
+    if (int rc = core_init_strat(c)) return rc;
+    if (int rc = core_set_params(c, reset, max_rounds)) return rc;
+    std::vector<long long> cn(PSI_K_COUNT);
+    int rc = core_solve_program_profiled(c, times_ms, cn.data());
     for (int k = 0; k < PSI_K_COUNT; k++) counts[k] = cn[k];
     return rc;
 }

@@ -66,6 +66,20 @@ struct JumpCnt {
     int err;        // 1: ran out of launched rounds
 };
 
+// Device state of the step program (the solve loop as a sequence of identical
+// launches, each running the next phase of the loop), indexed by launch number
+// i so that launch i+1 finds launch i's words without a dependent load:
+// st[i & 1] = the phase launch i ran (phase | round << 8); psum[i & 1] = its
+// jump-round decision word; pslot[i & 3] = its jump round's counters (zeroed
+// by launch i-2).  A graph of these launches holds a multiple of 4.
+struct StepProg {
+    int st[2];
+    int any_outer;  // some game takes the outer step (set by the control phase)
+    int live;       // games not done
+    unsigned long long psum[2];
+    unsigned long long pslot[4][kSlots];
+};
+
 struct Core {
     int G = 0;
     long long N = 0, M = 0;
@@ -113,6 +127,10 @@ struct Core {
     unsigned long long* prof_ties = nullptr;  // profiled solves: compares taking the exact path
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
+    StepProg* prog = nullptr;          // step program state (not in a serving pool)
+    void* pargs = nullptr;             // step program arguments (device copy)
+    hipGraphExec_t pgraph = nullptr;   // captured step-program launches
+    int pgraph_launches = 0;
 };
 
 // Launch shapes of the SI step kernels (grid-stride launches).
@@ -120,6 +138,8 @@ struct SiTuning {
     unsigned jump_grid = 2048;   // blocks of a pointer-jumping round
     unsigned step_grid = 16384;  // blocks of val_init / label / switch / mark
     unsigned pool_grid = 4096;   // the same in a serving pool
+    unsigned phase_grid = 2048;  // blocks of a step-program launch
+    int program = 1;             // solve loop as a step program (else fixed launches per step)
 };
 extern SiTuning g_si_tune;
 
@@ -163,6 +183,7 @@ int core_pool_admit(Core& c, int slot, long long vbase, long long ebase, const G
 int core_pool_localize(Core& c, int slot, hipStream_t s2);
 int core_pool_release(Core& c, int slot, hipStream_t s2);
 int core_solve_profiled(Core& c, double* times, long long* counts);
+int core_solve_program_profiled(Core& c, double* times, long long* counts);
 // solve_bf for every game of the core (after core_init_strat and
 // core_set_params(..., bf = 1)): the device loop, then the Odd strategy of
 // games with at least one relaxation replaced by succ (cpu_bf.cpp:156-165).

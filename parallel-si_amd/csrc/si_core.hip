@@ -131,6 +131,42 @@ __device__ __forceinline__ int decide_round(int r, const JumpCnt* jc, int maxn, 
     return (r < 30 && (1 << r) < min(*cyc, maxn)) ? 1 : 0;
 }
 
+static_assert(kSlots == kWave, "a wave reads one slot row");
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x)
+{
+    for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+// decide_round by a whole wave: sm = the summary word of round r-1, x = this
+// lane's slot word of round r-1 (lane k: slot k; kSlots == kWave), both loaded
+// together.  Every lane returns the same.
+__device__ __forceinline__ int decide_wave(int r, unsigned long long sm, unsigned long long x, int maxn,
+                                           int* stop, int* cyc)
+{
+    const int halt = (int)(sm >> 63);
+    int st = (int)((sm >> 32) & 0x7FFFFFFFu) - 1, cy = (int)(sm & 0xFFFFFFFFu);
+    if (r >= 1 && st < 0 && !halt) {               // uniform: sm is the same in every lane
+        const unsigned long long res = wave_sum_u64(x >> 32), act = wave_sum_u64(x & 0xFFFFFFFFu);
+        if (res == 0) {
+            st = r - 1;
+            cy = (int)act;
+        }
+    }
+    *stop = st;
+    *cyc = cy;
+    if (halt || r >= kMaxRounds) return 0;
+    if (r == 0 || st < 0) return 1;
+    return (r < 30 && (1 << r) < min(cy, maxn)) ? 1 : 0;
+}
+
+__device__ __forceinline__ unsigned long long summary_word(int run, int stop, int cyc)
+{
+    return ((unsigned long long)(run ? 0 : 1) << 63) | ((unsigned long long)(stop + 1) << 32) |
+           (unsigned)cyc;
+}
+
 // ---------------------------------------------------------------------------
 // init_strat (gpu_parity.cu:78-96; cpu_parity.cpp:317-327)
 __global__ void k_init_strat(long long N, const int* __restrict__ vinfo, const int* __restrict__ off,
@@ -174,7 +210,7 @@ __global__ void k_reset_opp(long long N, const int* __restrict__ vinfo, const in
 
 // Level-0 jump records.  S = (next or terminal code, max priority, count, -).
 // Terminal codes: -1 sink, -2 solved +inf, -3 solved -inf, -4 finished game.
-__global__ void __launch_bounds__(kBS) k_val_init(
+__device__ __forceinline__ void val_init_body(
     long long N, const int* __restrict__ vinfo, const int* __restrict__ off,
     const int* __restrict__ tgt, const int* __restrict__ gid, const GameCtl* __restrict__ gc,
     int* __restrict__ strat, const int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
@@ -243,37 +279,38 @@ __global__ void __launch_bounds__(kBS) k_val_init(
     }
 }
 
+__global__ void __launch_bounds__(kBS) k_val_init(
+    long long N, const int* __restrict__ vinfo, const int* __restrict__ off,
+    const int* __restrict__ tgt, const int* __restrict__ gid, const GameCtl* __restrict__ gc,
+    int* __restrict__ strat, const int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
+    int* __restrict__ par, int2* __restrict__ S0, unsigned long long* __restrict__ Z0,
+    JumpCnt* __restrict__ jc, uint8_t* __restrict__ blive, uint8_t* __restrict__ bact0,
+    uint8_t* __restrict__ blast, int trace, const int* __restrict__ hi)
+{
+    val_init_body(N, vinfo, off, tgt, gid, gc, strat, solved, lab, par, S0, Z0, jc, blive, bact0,
+                  blast, trace, hi);
+}
+
 // One pointer-jumping round over the blocks that still hold a jumping vertex
 // (grid-stride, so launches after convergence cost a few thousand one-word
 // reads).  A block that is not visited keeps its last written level; its
 // records there are final, which is where gathers and the label kernel read
 // them (level min(r, blast[block])).  Lifting compares only read a vertex's
 // levels below its resolution level, which its block always wrote.
-__global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int2* __restrict__ Sl,
-                                              unsigned long long* __restrict__ Zl,
-                                              JumpCnt* __restrict__ jc,
-                                              const uint8_t* __restrict__ blive,
-                                              const uint8_t* __restrict__ bact_in,
-                                              uint8_t* __restrict__ bact_out,
-                                              uint8_t* __restrict__ blast, int trace,
-                                              const int* __restrict__ hi)
+//
+// Round r of a valuation that runs it.  cnt: two zeroed shared counters;
+// the block's resolved/still-jumping counts go to slot_row.
+__device__ __forceinline__ void jump_body(int r, long long N, int2* __restrict__ Sl,
+                                          unsigned long long* __restrict__ Zl,
+                                          unsigned long long* __restrict__ slot_row,
+                                          const uint8_t* __restrict__ blive,
+                                          const uint8_t* __restrict__ bact_in,
+                                          uint8_t* __restrict__ bact_out,
+                                          uint8_t* __restrict__ blast, int trace,
+                                          const int* __restrict__ hi, unsigned int* cnt)
 {
-    __shared__ int run_s;
-    __shared__ unsigned int cnt_res, cnt_act;
-    if (threadIdx.x == 0) {
-        int stop, cyc, halt;
-        int run = decide_round(r, jc, maxn, &stop, &cyc, &halt);
-        run_s = run;
-        cnt_res = 0;
-        cnt_act = 0;
-        if (blockIdx.x == 0) {
-            jc->summary[r & 1] = ((unsigned long long)(run ? 0 : 1) << 63) |
-                                 ((unsigned long long)(stop + 1) << 32) | (unsigned)cyc;
-            if (run) jc->last = r;
-        }
-    }
-    __syncthreads();
-    if (!run_s) return;
+    unsigned int& cnt_res = cnt[0];
+    unsigned int& cnt_act = cnt[1];
     const int2* Sin = Sl + (size_t)level_slot(r) * N;
     int2* Sout = Sl + (size_t)level_slot(r + 1) * N;
     const int span = r < 28 ? 1 << r : 0;         // vertices of an unfinished level-r record
@@ -329,14 +366,46 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int2
     }
     __syncthreads();
     if (threadIdx.x == 0 && (cnt_res | cnt_act))
-        atomicAdd(&jc->slot[r][blockIdx.x & (kSlots - 1)],
-                  ((unsigned long long)cnt_res << 32) | cnt_act);
+        atomicAdd(&slot_row[blockIdx.x & (kSlots - 1)], ((unsigned long long)cnt_res << 32) | cnt_act);
+}
+
+__global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int2* __restrict__ Sl,
+                                              unsigned long long* __restrict__ Zl,
+                                              JumpCnt* __restrict__ jc,
+                                              const uint8_t* __restrict__ blive,
+                                              const uint8_t* __restrict__ bact_in,
+                                              uint8_t* __restrict__ bact_out,
+                                              uint8_t* __restrict__ blast, int trace,
+                                              const int* __restrict__ hi)
+{
+    __shared__ int run_s;
+    __shared__ unsigned int cnt[2];
+    if (threadIdx.x < kWave) {                    // the first wave decides
+        const unsigned long long sm = r >= 1 ? jc->summary[(r - 1) & 1] : 0ull;
+        const unsigned long long x = r >= 1 ? jc->slot[r - 1][threadIdx.x] : 0ull;
+        int stop, cyc;
+        const int run = decide_wave(r, sm, x, maxn, &stop, &cyc);
+        if (threadIdx.x == 0) {
+            run_s = run;
+            cnt[0] = cnt[1] = 0;
+            if (blockIdx.x == 0) {
+                jc->summary[r & 1] = summary_word(run, stop, cyc);
+                if (run) jc->last = r;
+            }
+        }
+    }
+    __syncthreads();
+    if (!run_s) return;
+    jump_body(r, N, Sl, Zl, jc->slot[r], blive, bact_in, bact_out, blast, trace, hi, cnt);
 }
 
 // Labels (d_mark_infinite role), compare keys, trace hashes; in the loop form
 // also mark_solved(1) (cpu_parity.cpp:306-315), which solve_si calls right
 // after every valuation.
-__global__ void __launch_bounds__(kBS) k_label(
+//
+// check_rounds: flag the valuation if a round at lmax would still have run
+// (the launch sequence has lmax rounds).
+__device__ __forceinline__ void label_body(
     long long N, int maxn, int lmax, const int2* __restrict__ Sl,
     const unsigned long long* __restrict__ Zl, const uint8_t* __restrict__ blast,
     const int* __restrict__ gid, GameCtl* __restrict__ gc, const int* __restrict__ strat,
@@ -344,11 +413,11 @@ __global__ void __launch_bounds__(kBS) k_label(
     int* __restrict__ key, unsigned long long* __restrict__ dig, unsigned long long* __restrict__ tr,
     int trace_cap, int loop_form, int trace, JumpCnt* __restrict__ jc,
     const uint8_t* __restrict__ blive, const int* __restrict__ hi,
-    unsigned long long* __restrict__ chg, unsigned long long* __restrict__ chg0)
+    unsigned long long* __restrict__ chg, unsigned long long* __restrict__ chg0, int check_rounds)
 {
     const int last = jc->last;
     const int F = last + 1;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (check_rounds && blockIdx.x == 0 && threadIdx.x == 0) {
         int stop, cyc, halt;
         if (decide_round(lmax, jc, maxn, &stop, &cyc, &halt)) jc->err = 1;
     }
@@ -435,6 +504,20 @@ __global__ void __launch_bounds__(kBS) k_label(
     }
 }
 
+__global__ void __launch_bounds__(kBS) k_label(
+    long long N, int maxn, int lmax, const int2* __restrict__ Sl,
+    const unsigned long long* __restrict__ Zl, const uint8_t* __restrict__ blast,
+    const int* __restrict__ gid, GameCtl* __restrict__ gc, const int* __restrict__ strat,
+    int8_t* __restrict__ solved, int8_t* __restrict__ lab, int* __restrict__ dep,
+    int* __restrict__ key, unsigned long long* __restrict__ dig, unsigned long long* __restrict__ tr,
+    int trace_cap, int loop_form, int trace, JumpCnt* __restrict__ jc,
+    const uint8_t* __restrict__ blive, const int* __restrict__ hi,
+    unsigned long long* __restrict__ chg, unsigned long long* __restrict__ chg0)
+{
+    label_body(N, maxn, lmax, Sl, Zl, blast, gid, gc, strat, solved, lab, dep, key, dig, tr, trace_cap,
+               loop_form, trace, jc, blive, hi, chg, chg0, 1);
+}
+
 // Exact compare of two finite vertices whose path maxima are equal.  The jump
 // records of every level are binary-lifting tables (level j: 2^j-th
 // successor and the largest priority over those 2^j vertices), so the lowest
@@ -443,7 +526,7 @@ __global__ void __launch_bounds__(kBS) k_label(
 // otherwise 2, with the lengths of the private segments (u's first la
 // vertices and w's first lb vertices, before the common part), whose
 // multisets coop_scan compares.
-__device__ __noinline__ int cmp_lca(int u, int w, const int* __restrict__ dep,
+__device__ __forceinline__ int cmp_lca(int u, int w, const int* __restrict__ dep,
                                     const int2* __restrict__ Sl, const int* __restrict__ vinfo,
                                     long long N, int F, int* la_out, int* lb_out)
 {
@@ -517,7 +600,7 @@ __device__ __forceinline__ int wave_sum(int x)
 // walks 1/64 of each segment (its start found through the lifting tables), and
 // each step of the top-down compare is a wave reduction.  Every lane returns
 // the result.
-__device__ __noinline__ int coop_scan(int u, int la, int w, int lb, const int* __restrict__ par,
+__device__ __forceinline__ int coop_scan(int u, int la, int w, int lb, const int* __restrict__ par,
                                       const int* __restrict__ vinfo, const int2* __restrict__ Sl,
                                       long long N, int F, unsigned long long* __restrict__ prof)
 {
@@ -594,15 +677,18 @@ __device__ __forceinline__ int key_of(const int* __restrict__ key, int u)
 // keep the current successor unless one is strictly better; the first
 // strictly-better-than-current edge in CSR order wins each step of the scan.
 // Loop form with player 0 also applies mark_solved(0) first.
-__global__ void __launch_bounds__(kBS) k_switch(
+// Successors compared per batch of loads (registers for targets and keys).
+constexpr int kSwBatch = 5;
+
+__device__ __forceinline__ void switch_body(
     int player, long long N, const int* __restrict__ own, long long own_n,
     const int* __restrict__ vinfo, const int* __restrict__ off,
     const int* __restrict__ tgt, const int* __restrict__ gid, GameCtl* __restrict__ gc,
-    int* __restrict__ strat, int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
-    const int* __restrict__ key, const int* __restrict__ dep, const int* __restrict__ par,
-    const int2* __restrict__ Sl, const JumpCnt* __restrict__ jc, int loop_form,
-    int* __restrict__ flag, unsigned long long* __restrict__ ties, const int* __restrict__ hi,
-    const unsigned long long* __restrict__ chg, const unsigned long long* __restrict__ chg0)
+    int* __restrict__ strat, const int* __restrict__ key, const int* __restrict__ dep,
+    const int* __restrict__ par, const int2* __restrict__ Sl, const JumpCnt* __restrict__ jc,
+    int loop_form, int* __restrict__ flag, unsigned long long* __restrict__ ties,
+    const int* __restrict__ hi, const unsigned long long* __restrict__ chg,
+    const unsigned long long* __restrict__ chg0)
 {
     // Loop form: a vertex left at the scan's fixed point stays there unless
     // the valuation of its current successor or of a candidate changed since
@@ -615,10 +701,13 @@ __global__ void __launch_bounds__(kBS) k_switch(
     const int F = jc->last + 1;                   // levels of the last valuation
     // A serving pool's owner lists are indexed like its vertices (holes are
     // -1), so nothing at or above the occupied bound is a vertex.
-    const long long lim = min(own_n, (long long)*hi);
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long ib = (long long)blockIdx.x * blockDim.x; ib < lim; ib += stride) {
-        const long long i = ib + threadIdx.x;
+    // Waves take runs of 64 owner-list entries, grid-stride.
+    const int lim = (int)min(own_n, (long long)*hi);
+    const int nruns = (lim + kWave - 1) / kWave;
+    const int nw = (int)(gridDim.x * (blockDim.x / kWave));
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    for (int w = (int)(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); w < nruns; w += nw) {
+        const int i = w * kWave + lane;
         bool sw = false, live = false;
         int g = -1;
         const int v = i < lim ? own[i] : -1;
@@ -641,7 +730,7 @@ __global__ void __launch_bounds__(kBS) k_switch(
                 live = dirty;
             }
         }
-        // Successors in batches of 8: the target and key loads of a batch are
+        // Successors in batches of kSwBatch: the target and key loads of a batch are
         // independent, so they are in flight together; the scan itself is
         // sequential in CSR order (the reference's tie-break).  The loop runs
         // for the whole wave, so that exact compares whose segment maxima tie
@@ -651,38 +740,41 @@ __global__ void __launch_bounds__(kBS) k_switch(
         int e = live ? off[v] : 0;
         const int e1 = live ? off[v + 1] : 0;
         while (__any(e < e1)) {
-            const int cnt = e < e1 ? min(8, e1 - e) : 0;
-            int us[8], ks[8];
+            const int cnt = e < e1 ? min(kSwBatch, e1 - e) : 0;
+            int us[kSwBatch], ks[kSwBatch];
 #pragma unroll
-            for (int k = 0; k < 8; k++) us[k] = k < cnt ? tgt[e + k] : -1;
+            for (int k = 0; k < kSwBatch; k++) us[k] = k < cnt ? tgt[e + k] : -1;
 #pragma unroll
-            for (int k = 0; k < 8; k++) ks[k] = k < cnt ? key[us[k]] : 0;
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
+            for (int k = 0; k < kSwBatch; k++) ks[k] = k < cnt ? key[us[k]] : 0;
+            // One compare at a time (a rolled loop, so the exact-compare code
+            // exists once; k is uniform, so us[k] and ks[k] are register reads).
+#pragma unroll 1
+            for (int k = 0; k < kSwBatch && __any(k < cnt); k++) {
+                const int u = us[k], ku = ks[k];
                 int c = 0, la = 0, lb = 0;
                 bool tie = false;
                 if (k < cnt) {
-                    c = cmp_fast(us[k], ks[k], cur, kcur, &tie);
+                    c = cmp_fast(u, ku, cur, kcur, &tie);
                     if (tie) {
                         if (ties) atomicAdd(ties, 1ull);
-                        c = cmp_lca(us[k], cur, dep, Sl, vinfo, N, F, &la, &lb);
+                        c = cmp_lca(u, cur, dep, Sl, vinfo, N, F, &la, &lb);
                     }
                 }
                 unsigned long long need = __ballot(c == 2);
                 while (need) {
                     const int L = __ffsll((long long)need) - 1;
-                    const int r = coop_scan(__shfl(us[k], L), __shfl(la, L), __shfl(cur, L),
+                    const int r = coop_scan(__shfl(u, L), __shfl(la, L), __shfl(cur, L),
                                             __shfl(lb, L), par, vinfo, Sl, N, F, ties);
                     if ((int)(threadIdx.x & (kWave - 1)) == L) c = r;
                     need &= need - 1;
                 }
                 if (k < cnt && (player == 0 ? c > 0 : c < 0)) {
-                    cur = us[k];
-                    kcur = ks[k];
+                    cur = u;
+                    kcur = ku;
                     sw = true;
                 }
             }
-            e += 8;
+            e += kSwBatch;
         }
         if (sw) strat[v] = cur;
         // one store per wave and game: the flag only has to become nonzero
@@ -709,14 +801,29 @@ __global__ void __launch_bounds__(kBS) k_switch(
     }
 }
 
+__global__ void __launch_bounds__(kBS) k_switch(
+    int player, long long N, const int* __restrict__ own, long long own_n,
+    const int* __restrict__ vinfo, const int* __restrict__ off,
+    const int* __restrict__ tgt, const int* __restrict__ gid, GameCtl* __restrict__ gc,
+    int* __restrict__ strat, const int* __restrict__ key, const int* __restrict__ dep,
+    const int* __restrict__ par, const int2* __restrict__ Sl, const JumpCnt* __restrict__ jc,
+    int loop_form, int* __restrict__ flag, unsigned long long* __restrict__ ties,
+    const int* __restrict__ hi, const unsigned long long* __restrict__ chg,
+    const unsigned long long* __restrict__ chg0)
+{
+    switch_body(player, N, own, own_n, vinfo, off, tgt, gid, gc, strat, key, dep, par, Sl, jc,
+                loop_form, flag, ties, hi, chg, chg0);
+}
+
 // mark_solved(0) of the games that take the outer step (fused form of
 // cpu_parity.cpp:306-315 as solve_si calls it, gpumain.cpp:44).
 // solve_bf games after a relaxation also record succ: the chosen successor of
 // every unsolved Odd vertex with a finite valuation (cpu_bf.cpp:120-147).
-__global__ void k_mark0_outer(long long N, const int* __restrict__ gid, const GameCtl* __restrict__ gc,
-                              const int8_t* __restrict__ lab, int8_t* __restrict__ solved,
-                              const int* __restrict__ hi, const int* __restrict__ vinfo,
-                              const int* __restrict__ strat, int* __restrict__ succ)
+__device__ __forceinline__ void mark0_body(long long N, const int* __restrict__ gid,
+                                           const GameCtl* __restrict__ gc,
+                                           const int8_t* __restrict__ lab, int8_t* __restrict__ solved,
+                                           const int* __restrict__ hi, const int* __restrict__ vinfo,
+                                           const int* __restrict__ strat, int* __restrict__ succ)
 {
     const long long lim = min(N, (long long)*hi);
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -728,6 +835,14 @@ __global__ void k_mark0_outer(long long N, const int* __restrict__ gid, const Ga
             succ[v] = strat[v];
         if (l == 1) solved[v] = 1;
     }
+}
+
+__global__ void k_mark0_outer(long long N, const int* __restrict__ gid, const GameCtl* __restrict__ gc,
+                              const int8_t* __restrict__ lab, int8_t* __restrict__ solved,
+                              const int* __restrict__ hi, const int* __restrict__ vinfo,
+                              const int* __restrict__ strat, int* __restrict__ succ)
+{
+    mark0_body(N, gid, gc, lab, solved, hi, vinfo, strat, succ);
 }
 
 // succ starts at the game's vertex 0 (cpu_bf.cpp:17).
@@ -760,47 +875,195 @@ __global__ void k_mark_solved(int player, long long N, const int8_t* __restrict_
 }
 
 // After switch_strategy(1): count the round; the inner loop ends when no
-// player-1 vertex switched (gpumain.cpp:34-38).
-__global__ void k_ctl1(int G, GameCtl* gc)
+// player-1 vertex switched (gpumain.cpp:34-38).  A game that does not take the
+// outer step is checked against its round cap here (switch(0) and its control
+// leave it unchanged).  pg (the step program, or null): games taking the outer
+// step are flagged, finished games counted down.
+__device__ __forceinline__ void ctl1_body(int G, GameCtl* __restrict__ gc, StepProg* __restrict__ pg)
 {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= G) return;
-    GameCtl c = gc[g];
-    if (c.done) return;
-    if (c.err) {                   // solve_bf: odd strategy cycle (the reference exits)
-        c.done = 1;
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
+        GameCtl c = gc[g];
+        if (c.done) continue;
+        if (c.err) {               // solve_bf: odd strategy cycle (the reference exits)
+            c.done = 1;
+            gc[g] = c;
+            if (pg) atomicSub(&pg->live, 1);
+            continue;
+        }
+        c.br++;
+        c.outer = c.sw1 == 0;
+        c.sw1 = 0;
+        c.reset_pending = 0;
+        c.clear0 = 0;
+        if (!c.outer && c.max_rounds > 0 && c.br >= c.max_rounds) {
+            c.capped = 1;
+            c.done = 1;
+            if (pg) atomicSub(&pg->live, 1);
+        }
+        if (pg && c.outer) atomicOr(&pg->any_outer, 1);
         gc[g] = c;
-        return;
     }
-    c.br++;
-    c.outer = c.sw1 == 0;
-    c.sw1 = 0;
-    c.reset_pending = 0;
-    c.clear0 = 0;
-    gc[g] = c;
 }
 
+__global__ void k_ctl1(int G, GameCtl* gc) { ctl1_body(G, gc, nullptr); }
+
 // After switch_strategy(0) (gpumain.cpp:44-50).
-__global__ void k_ctl0(int G, GameCtl* gc)
+__device__ __forceinline__ void ctl0_body(int G, GameCtl* __restrict__ gc, StepProg* __restrict__ pg)
 {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= G) return;
-    GameCtl c = gc[g];
-    if (c.done) return;
-    if (c.outer) {
-        c.iter++;
-        c.clear0 = 1;              // the next valuation restarts the switch(0) record
-        c.bf_init = 1;             // solve_bf: later best responses relax
-        if (c.sw0 == 0) c.done = 1;
-        else if (c.reset) c.reset_pending = 1;
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
+        GameCtl c = gc[g];
+        if (c.done) continue;
+        if (c.outer) {
+            c.iter++;
+            c.clear0 = 1;          // the next valuation restarts the switch(0) record
+            c.bf_init = 1;         // solve_bf: later best responses relax
+            if (c.sw0 == 0) c.done = 1;
+            else if (c.reset) c.reset_pending = 1;
+        }
+        c.sw0 = 0;
+        c.outer = 0;
+        if (!c.done && c.max_rounds > 0 && c.br >= c.max_rounds) {
+            c.capped = 1;
+            c.done = 1;
+        }
+        if (pg && c.done) atomicSub(&pg->live, 1);
+        gc[g] = c;
     }
-    c.sw0 = 0;
-    c.outer = 0;
-    if (!c.done && c.max_rounds > 0 && c.br >= c.max_rounds) {
-        c.capped = 1;
-        c.done = 1;
+}
+
+__global__ void k_ctl0(int G, GameCtl* gc) { ctl0_body(G, gc, nullptr); }
+
+// ---------------------------------------------------------------------------
+// Step program: the device-resident loop as identical launches of k_phase.
+// Launch i reads the phase launch i-1 ran and runs the next one:
+//   INIT (val_init) -> JUMP 0 -> JUMP 1 -> ... (while decide_round runs the
+//   round) -> LABEL -> SW1 -> CTL1 -> [MARK0 -> SW0 -> CTL0, only when some
+//   game takes the outer step] -> INIT ...
+// so no launch is spent on a round after convergence or on an outer-step
+// phase that no game takes; a graph of these launches can stop anywhere and
+// the next graph continues.  Once no game is live every launch exits at once.
+enum Phase { PH_START = 0, PH_INIT, PH_JUMP, PH_LABEL, PH_SW1, PH_CTL1, PH_MARK0, PH_SW0, PH_CTL0, PH_DONE };
+
+struct StepArgs {
+    long long N, nb, own_n0, own_n1;
+    int G, maxn, lmax, trace, trace_cap;
+    const int *vinfo, *off, *tgt, *gid, *own0, *own1, *hi;
+    GameCtl* gc;
+    int *strat, *par, *dep, *key, *flag, *succ;
+    int8_t *solved, *lab;
+    uint8_t *blive, *bact, *blast;
+    int2* Sl;
+    unsigned long long *Zl, *dig, *tr, *chg, *chg0;
+    JumpCnt* jc;
+};
+
+// The arguments live in device memory (one pointer argument): each phase
+// loads the fields it uses, which keeps scalar registers free.
+__global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
+{
+    const StepArgs& a = *ap;
+    __shared__ int ph_s, r_s;
+    __shared__ unsigned int cnt[2];
+    if (threadIdx.x < kWave) {                    // the first wave decides
+        const int lane = threadIdx.x;
+        // Launch i-1's phase, round summary and slot row, loaded together.
+        const int prev = pg->st[(i + 1) & 1];
+        const unsigned long long sm = pg->psum[(i + 1) & 1];
+        const unsigned long long x = pg->pslot[(i + 3) & 3][lane];
+        int ph = prev & 0xff, r = prev >> 8;
+        switch (ph) {
+        case PH_INIT: ph = PH_JUMP; r = 0; break;
+        case PH_JUMP: r++; break;
+        case PH_LABEL: ph = PH_SW1; break;
+        case PH_SW1: ph = PH_CTL1; break;
+        case PH_CTL1: ph = pg->any_outer ? PH_MARK0 : PH_INIT; break;
+        case PH_MARK0: ph = PH_SW0; break;
+        case PH_SW0: ph = PH_CTL0; break;
+        case PH_DONE: break;
+        default: ph = PH_INIT; break;             // PH_START, PH_CTL0
+        }
+        if (ph == PH_INIT) {
+            r = 0;
+            if (pg->live <= 0) ph = PH_DONE;
+        }
+        if (ph == PH_JUMP) {
+            // round r's decision from round r-1 (launch i-1); a round at lmax
+            // would mean the valuation needs more rounds than it has levels
+            int stop, cyc;
+            int run = decide_wave(r, r >= 1 ? sm : 0ull, x, a.maxn, &stop, &cyc);
+            if (run && r >= a.lmax) {
+                if (blockIdx.x == 0 && lane == 0) a.jc->err = 1;
+                run = 0;
+            }
+            if (blockIdx.x == 0 && lane == 0) {
+                pg->psum[i & 1] = summary_word(run, stop, cyc);
+                if (run) a.jc->last = r;
+            }
+            if (!run) ph = PH_LABEL;
+        }
+        if (blockIdx.x == 0) {
+            pg->pslot[(i + 2) & 3][lane] = 0;     // launch i+2 counts its round there
+            if (lane == 0) {
+                pg->st[i & 1] = ph | (r << 8);
+                if (ph == PH_SW1) pg->any_outer = 0;   // set again by CTL1
+            }
+        }
+        if (lane == 0) {
+            ph_s = ph;
+            r_s = r;
+            cnt[0] = cnt[1] = 0;
+        }
     }
-    gc[g] = c;
+    __syncthreads();
+    const int ph = ph_s, r = r_s;
+    switch (ph) {
+    case PH_INIT:
+        val_init_body(a.N, a.vinfo, a.off, a.tgt, a.gid, a.gc, a.strat, a.solved, a.lab, a.par, a.Sl,
+                      a.Zl, a.jc, a.blive, a.bact, a.blast, a.trace, a.hi);
+        break;
+    case PH_JUMP:
+        jump_body(r, a.N, a.Sl, a.Zl, pg->pslot[i & 3], a.blive, a.bact + (r & 1) * a.nb,
+                  a.bact + ((r + 1) & 1) * a.nb, a.blast, a.trace, a.hi, cnt);
+        break;
+    case PH_LABEL:
+        label_body(a.N, a.maxn, a.lmax, a.Sl, a.Zl, a.blast, a.gid, a.gc, a.strat, a.solved, a.lab,
+                   a.dep, a.key, a.dig, a.tr, a.trace_cap, 1, a.trace, a.jc, a.blive, a.hi, a.chg,
+                   a.chg0, 0);
+        break;
+    case PH_SW1:
+    case PH_SW0: {                                // one inlined copy of the switch
+        const int pl = ph == PH_SW1 ? 1 : 0;
+        switch_body(pl, a.N, pl ? a.own1 : a.own0, pl ? a.own_n1 : a.own_n0, a.vinfo, a.off, a.tgt,
+                    a.gid, a.gc, a.strat, a.key, a.dep, a.par, a.Sl, a.jc, 1, a.flag, nullptr, a.hi,
+                    a.chg, a.chg0);
+        break;
+    }
+    case PH_CTL1: ctl1_body(a.G, a.gc, pg); break;
+    case PH_MARK0: mark0_body(a.N, a.gid, a.gc, a.lab, a.solved, a.hi, a.vinfo, a.strat, a.succ); break;
+    case PH_CTL0: ctl0_body(a.G, a.gc, pg); break;
+    default: break;
+    }
+}
+
+// Step program start: the next launch runs INIT; live = unfinished games.
+__global__ void k_prog_reset(int G, const GameCtl* __restrict__ gc, StepProg* __restrict__ pg)
+{
+    __shared__ int acc;
+    if (threadIdx.x == 0) acc = 0;
+    __syncthreads();
+    int live = 0;
+    for (int g = threadIdx.x; g < G; g += blockDim.x) live += gc[g].done ? 0 : 1;
+    atomicAdd(&acc, live);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        pg->st[0] = PH_START;
+        pg->st[1] = PH_START;
+        pg->psum[0] = pg->psum[1] = 0;
+        pg->any_outer = 0;
+        pg->live = acc;
+    }
+    for (int k = threadIdx.x; k < 4 * kSlots; k += blockDim.x) pg->pslot[k / kSlots][k % kSlots] = 0;
 }
 
 __global__ void k_count_live(int G, const GameCtl* gc, int* out)
@@ -1026,6 +1289,7 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
     if (int rc = dalloc(&c.blast, nblocks(N, kBS))) return rc;
     if (int rc = dalloc(&c.Sl, N * c.levels)) return rc;
     c.chg_words = 4LL * nblocks(N, kBS);
+    if (int rc = dalloc(&c.prog, 1)) return rc;
     if (int rc = dalloc(&c.chg, 2 * c.chg_words)) return rc;
     if (trace) {
         if (int rc = dalloc(&c.Zl, N * c.levels)) return rc;
@@ -1299,9 +1563,10 @@ int core_pool_release(Core& c, int slot, hipStream_t s2)
 void core_destroy(Core& c)
 {
     if (c.graph) (void)hipGraphExecDestroy(c.graph);
+    if (c.pgraph) (void)hipGraphExecDestroy(c.pgraph);
     void* ptrs[] = {c.vinfo, c.off, c.tgt, c.gid, c.strat, c.par, c.dep, c.key, c.solved, c.lab, c.blive,
                     c.bact, c.blast, c.Sl, c.own[0], c.own[1], c.Zl, c.dig, c.tr, c.gctl, c.jc, c.flag,
-                    c.cnt_scratch, c.chg, c.succ};
+                    c.cnt_scratch, c.chg, c.succ, c.prog, c.pargs};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c = Core();
@@ -1355,9 +1620,9 @@ static void launch_switch(Core& c, int player, bool loop_form)
                                       c.strat, c.succ);
     if (c.own_n[player] == 0) return;
     k_switch<<<SGRID(c.own_n[player])>>>(player, c.N, c.own[player], c.own_n[player], c.vinfo,
-                                        c.off, c.tgt, c.gid, c.gctl, c.strat, c.solved, c.lab,
-                                        c.key, c.dep, c.par, c.Sl, c.jc, loop_form, c.flag,
-                                        c.prof_ties, c.flag + 2, c.chg, c.chg + c.chg_words);
+                                        c.off, c.tgt, c.gid, c.gctl, c.strat, c.key, c.dep,
+                                        c.par, c.Sl, c.jc, loop_form, c.flag, c.prof_ties,
+                                        c.flag + 2, c.chg, c.chg + c.chg_words);
 }
 
 int core_set_params(Core& c, int reset, int max_rounds, int bf)
@@ -1372,6 +1637,11 @@ int core_set_params(Core& c, int reset, int max_rounds, int bf)
     }
     if (bf && !c.succ) {
         if (int rc = dalloc(&c.succ, c.N)) return rc;
+        // captured launches hold the (null) succ they were captured with
+        if (c.graph) (void)hipGraphExecDestroy(c.graph);
+        if (c.pgraph) (void)hipGraphExecDestroy(c.pgraph);
+        c.graph = nullptr;
+        c.pgraph = nullptr;
     }
     if (bf) k_succ_init<<<GRID(c.N)>>>(c.N, c.gid, c.gctl, c.succ);
     PSI_HIP(hipMemcpyAsync(c.gctl, gc.data(), c.G * sizeof(GameCtl), hipMemcpyHostToDevice, c.s));
@@ -1563,13 +1833,163 @@ int core_launch_steps(Core& c, int steps_per_check)
     return PSI_OK;
 }
 
+static StepArgs step_args(Core& c)
+{
+    StepArgs a;
+    a.N = c.N;
+    a.nb = nblocks(c.N, kBS);
+    a.own_n0 = c.own_n[0];
+    a.own_n1 = c.own_n[1];
+    a.G = c.G;
+    a.maxn = maxn_of(c);
+    a.lmax = c.lmax;
+    a.trace = c.trace;
+    a.trace_cap = c.trace_cap;
+    a.vinfo = c.vinfo;
+    a.off = c.off;
+    a.tgt = c.tgt;
+    a.gid = c.gid;
+    a.own0 = c.own[0];
+    a.own1 = c.own[1];
+    a.hi = c.flag + 2;
+    a.gc = c.gctl;
+    a.strat = c.strat;
+    a.par = c.par;
+    a.dep = c.dep;
+    a.key = c.key;
+    a.flag = c.flag;
+    a.succ = c.succ;
+    a.solved = c.solved;
+    a.lab = c.lab;
+    a.blive = c.blive;
+    a.bact = c.bact;
+    a.blast = c.blast;
+    a.Sl = c.Sl;
+    a.Zl = c.Zl;
+    a.dig = c.dig;
+    a.tr = c.tr;
+    a.chg = c.chg;
+    a.chg0 = c.chg + c.chg_words;
+    a.jc = c.jc;
+    return a;
+}
+
+// Copies the step-program arguments to the device (before a capture).
+static int upload_step_args(Core& c, StepArgs** out)
+{
+    if (!c.pargs) {
+        StepArgs* p = nullptr;
+        if (int rc = dalloc(&p, 1)) return rc;
+        c.pargs = p;
+    }
+    const StepArgs a = step_args(c);
+    PSI_HIP(hipMemcpyAsync(c.pargs, &a, sizeof a, hipMemcpyHostToDevice, c.s));
+    PSI_HIP(hipStreamSynchronize(c.s));
+    *out = (StepArgs*)c.pargs;
+    return PSI_OK;
+}
+
+// Launches one graph of `launches` (a multiple of 4) step-program launches,
+// captured on first use.
+static int launch_program(Core& c, int launches)
+{
+    if (!c.pgraph || c.pgraph_launches != launches) {
+        if (c.pgraph) {
+            (void)hipGraphExecDestroy(c.pgraph);
+            c.pgraph = nullptr;
+        }
+        StepArgs* a = nullptr;
+        if (int rc = upload_step_args(c, &a)) return rc;
+        const unsigned grid = std::max(1u, std::min(nblocks(c.N, kBS), g_si_tune.phase_grid));
+        hipGraph_t gr;
+        PSI_HIP(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < launches; i++) k_phase<<<grid, kBS, 0, c.s>>>(i, a, c.prog);
+        hipError_t e = hipGetLastError();
+        hipError_t e2 = hipStreamEndCapture(c.s, &gr);
+        if (e != hipSuccess) return hip_fail(e, "k_phase launch", __FILE__, __LINE__);
+        if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture", __FILE__, __LINE__);
+        e = hipGraphInstantiate(&c.pgraph, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+        c.pgraph_launches = launches;
+    }
+    PSI_HIP(hipGraphLaunch(c.pgraph, c.s));
+    return PSI_OK;
+}
+
+// The step program launched one launch at a time with HIP events around each
+// (no graph), reading back which phase each launch ran: device milliseconds
+// and launch counts per phase class (the PSI_K_* classes of
+// core_solve_profiled; mark_solved(0) counts with switch(0)).
+int core_solve_program_profiled(Core& c, double* times, long long* counts)
+{
+    for (int k = 0; k < PSI_K_COUNT; k++) {
+        times[k] = 0;
+        counts[k] = 0;
+    }
+    if (c.pool || !c.prog) return fail(PSI_ERR_INVALID, "step program: not available on this handle");
+    StepArgs* a = nullptr;
+    if (int rc = upload_step_args(c, &a)) return rc;
+    const unsigned grid = std::max(1u, std::min(nblocks(c.N, kBS), g_si_tune.phase_grid));
+    hipEvent_t e0, e1;
+    PSI_HIP(hipEventCreate(&e0));
+    PSI_HIP(hipEventCreate(&e1));
+    k_prog_reset<<<1, 256, 0, c.s>>>(c.G, c.gctl, c.prog);
+    int rc = PSI_OK;
+    for (long long i = 0; i < (1LL << 26); i++) {
+        const int par = (int)(i & 1);
+        if (hipEventRecord(e0, c.s) != hipSuccess) { rc = fail(PSI_ERR_HIP, "hipEventRecord"); break; }
+        k_phase<<<grid, kBS, 0, c.s>>>((int)(i & 3), a, c.prog);
+        if (hipEventRecord(e1, c.s) != hipSuccess) { rc = fail(PSI_ERR_HIP, "hipEventRecord"); break; }
+        int st = 0;
+        if (hipMemcpyAsync(&st, &c.prog->st[par], sizeof st, hipMemcpyDeviceToHost, c.s) != hipSuccess ||
+            hipStreamSynchronize(c.s) != hipSuccess) {
+            rc = fail(PSI_ERR_HIP, "step program: launch failed");
+            break;
+        }
+        const int ph = st & 0xff;
+        if (ph == PH_DONE) break;
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        int cls = PSI_K_CTL;
+        switch (ph) {
+        case PH_INIT: cls = PSI_K_VAL_INIT; counts[PSI_K_TOTAL]++; break;
+        case PH_JUMP: cls = PSI_K_JUMP; break;
+        case PH_LABEL: cls = PSI_K_LABEL; break;
+        case PH_SW1: cls = PSI_K_SWITCH1; break;
+        case PH_MARK0: case PH_SW0: cls = PSI_K_SWITCH0; break;
+        default: break;
+        }
+        times[cls] += ms;
+        counts[cls]++;
+        times[PSI_K_TOTAL] += ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
 int core_solve(Core& c, int steps_per_check)
 {
-    for (;;) {
-        if (int rc = core_launch_steps(c, steps_per_check)) return rc;
-        int done = 0;
-        if (int rc = core_all_done(c, &done)) return rc;
-        if (done) break;
+    if (!c.pool && c.prog && g_si_tune.program) {
+        // ~16 launches per SI step on random games; an even count keeps the
+        // state slots alternating across graphs
+        const int launches = 16 * std::max(1, steps_per_check);   // a multiple of 4
+        k_prog_reset<<<1, 256, 0, c.s>>>(c.G, c.gctl, c.prog);
+        PSI_LAUNCH_CHECK();
+        for (;;) {
+            if (int rc = launch_program(c, launches)) return rc;
+            int done = 0;
+            if (int rc = core_all_done(c, &done)) return rc;
+            if (done) break;
+        }
+    } else {
+        for (;;) {
+            if (int rc = core_launch_steps(c, steps_per_check)) return rc;
+            int done = 0;
+            if (int rc = core_all_done(c, &done)) return rc;
+            if (done) break;
+        }
     }
     JumpCnt jc;
     PSI_HIP(hipMemcpyAsync(&jc, c.jc, sizeof jc, hipMemcpyDeviceToHost, c.s));

@@ -252,7 +252,7 @@ def list_rank_status():
     return bits.value, over.value
 
 
-SIT_JUMP_GRID, SIT_STEP_GRID, SIT_POOL_GRID = 1, 2, 3
+SIT_JUMP_GRID, SIT_STEP_GRID, SIT_POOL_GRID, SIT_PHASE_GRID, SIT_PROGRAM = 1, 2, 3, 4, 5
 _sig("psi_si_tune", C.c_int, C.c_int, C.c_int)
 
 
@@ -430,6 +430,19 @@ def solve_profiled(solver, reset=False, max_rounds=0):
     t = np.zeros(len(K_NAMES), np.float64)
     c = np.zeros(len(K_NAMES), np.int64)
     check(lib.psi_solve_profiled(solver.h, int(reset), max_rounds, t, c), "psi_solve_profiled")
+    return {k: (float(t[i]), int(c[i])) for i, k in enumerate(K_NAMES)}
+
+_sig("psi_solve_program_profiled", C.c_int, _vp, C.c_int, C.c_int,
+     np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS"), _i64p)
+
+
+def solve_program_profiled(solver, reset=False, max_rounds=0):
+    """The step program launch by launch: {class: (ms, launches)} as
+    solve_profiled (ties/scans not counted)."""
+    t = np.zeros(len(K_NAMES), np.float64)
+    c = np.zeros(len(K_NAMES), np.int64)
+    check(lib.psi_solve_program_profiled(solver.h, int(reset), max_rounds, t, c),
+          "psi_solve_program_profiled")
     return {k: (float(t[i]), int(c[i])) for i, k in enumerate(K_NAMES)}
 
 _sig("psi_get_jump_stats", C.c_int, _vp, _i32p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int))
