@@ -70,9 +70,15 @@ def si_bench(psi, n, steps, warmup, seed):
     dev_ms = timer.ms()
     it, br, capped = s.stats()
     log(f"[bench] {steps} solves: iter={it} br={br} wall={wall:.3f}s device={dev_ms:.1f}ms")
+    # Attribution: the loop launched kernel by kernel (k_jump etc., the
+    # roofline's source), then the step program launch by launch (the phases
+    # the timed solves ran).
     prof = psi.solve_profiled(s)
     log("[bench] per-kernel ms/launches: " + ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in prof.items()))
-    return g, s, it, br, wall, prof
+    pprof = psi.solve_program_profiled(s)
+    log("[bench] step-program phases ms/launches: " +
+        ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in pprof.items() if v[1]))
+    return g, s, it, br, wall, prof, pprof
 
 
 def listrank_bench(psi, reps=10, tune=True):
@@ -458,12 +464,14 @@ def main():
         return
     if a.only == "si":
         a.no_listrank = a.no_cpu = a.no_batch = a.no_serve = a.no_bf = True
-    g, s, it, br, wall, prof = si_bench(psi, a.n, a.steps, a.warmup, a.seed)
+    g, s, it, br, wall, prof, pprof = si_bench(psi, a.n, a.steps, a.warmup, a.seed)
     value = g.n * br * a.steps / wall
     # Roofline of the dominant kernel: a pointer-jumping round reads and
     # writes every vertex's 8-byte record (successor or terminal code and
     # count, largest priority) and gathers the 8-byte record of each
-    # still-jumping vertex's successor.
+    # still-jumping vertex's successor.  Timed on k_jump, launched round by
+    # round; the timed solves run the same round code as the step program's
+    # jump phase (program_avg_launch_us: its launches, timed the same way).
     jump_ms, jump_launches = prof["jump"]
     gathers = prof["gathers"][1]
     jump_bytes = (16.0 * g.n * jump_launches + 8.0 * gathers) / max(jump_launches, 1)
@@ -472,7 +480,8 @@ def main():
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(g.n, a.seed), "alg_bytes_per_launch": round(jump_bytes),
             "avg_launch_us": round(jump_ms / max(jump_launches, 1) * 1e3, 2),
-            "launches": jump_launches}
+            "launches": jump_launches,
+            "program_avg_launch_us": round(pprof["jump"][0] / max(pprof["jump"][1], 1) * 1e3, 2)}
     line = {
         "metric": "SI vertex-iterations/s on 4M-vertex game",
         "value": round(value, 1),
@@ -492,6 +501,8 @@ def main():
         "roofline": roof,
         "kernels_ms": {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in prof.items()
                        if k != "gathers"},
+        "program_phases_ms": {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in pprof.items()
+                              if v[1]},
     }
     s.close()
     if not a.no_listrank:

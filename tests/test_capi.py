@@ -83,3 +83,17 @@ def test_autotune_descent_keeps_only_clear_wins():
         return cost[("a", cfg["a"])] * cost[("b", cfg["b"])]
     best, t, trials = autotune._descend({"a": 1, "b": 1}, {"a": [1, 2, 3], "b": [1, 2]}, measure)
     assert best == {"a": 3, "b": 1} and t == 0.5 and len(trials) == 4
+
+
+def test_si_tune_rejects_bad_values():
+    """Launch-shape settings are host state (no device call)."""
+    import psi
+    with pytest.raises(psi.PsiError):
+        psi.si_tune(psi.SIT_PROGRAM, 2)
+    with pytest.raises(psi.PsiError):
+        psi.si_tune(psi.SIT_PHASE_GRID, 3)
+    with pytest.raises(psi.PsiError):
+        psi.si_tune(99, 2048)
+    psi.si_tune(psi.SIT_PROGRAM, 0)
+    psi.si_tune(psi.SIT_PROGRAM, 1)
+    psi.si_tune(psi.SIT_PHASE_GRID, 2048)

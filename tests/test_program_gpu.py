@@ -1,0 +1,141 @@
+"""The solve loop as a step program (identical launches, each running the next
+phase) against the same loop launched phase by phase, and both against the
+oracle: identical iteration and valuation-round counts, strategies, labels,
+solved sets and per-round trace hashes, whatever the number of launches per
+graph; the launch-by-launch profile of the program solves the same."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def modes(gpu):
+    yield gpu
+    gpu.si_tune(gpu.SIT_PROGRAM, 1)
+
+
+def _run(gpu, games, program, steps_per_sync=4, reset=False):
+    gpu.si_tune(gpu.SIT_PROGRAM, program)
+    s = gpu.Solver(games, trace=True, trace_cap=512)
+    s.solve(reset=reset, steps_per_sync=steps_per_sync)
+    G = len(games) if isinstance(games, list) else 1
+    out = [(s.stats(k), s.strategy(k), s.labels(k), s.solved(k), s.trace_hashes(k)) for k in range(G)]
+    s.close()
+    return out
+
+
+def _same(a, b):
+    assert len(a) == len(b)
+    for k, (x, y) in enumerate(zip(a, b)):
+        assert x[0] == y[0], k
+        for u, v in zip(x[1:], y[1:]):
+            assert np.array_equal(u, v), k
+
+
+def _games(gpu):
+    return [gpu.gen_random_game(11, 30000, 20000, 2, 8),
+            gpu.gen_random_game(12, 20000, 60, 1, 4),
+            gpu.gen_chain_game(13, 5000)]
+
+
+@pytest.mark.parametrize("idx", range(3))
+def test_program_equals_phase_launches(modes, idx):
+    gpu = modes
+    g = _games(gpu)[idx]
+    print(f"program vs phases, game {idx}", flush=True)
+    a = _run(gpu, g, 1)
+    b = _run(gpu, g, 0)
+    _same(a, b)
+    r = oracle.solve(g, trace=True)
+    assert a[0][0][:2] == (r.iter, r.br)
+    assert np.array_equal(a[0][1], r.strat) and np.array_equal(a[0][2], r.inf)
+    assert np.array_equal(a[0][4], r.hashes[: len(a[0][4])])
+
+
+def test_program_reset_schedule(modes):
+    gpu = modes
+    g = gpu.gen_random_game(21, 8000, 500, 1, 5)
+    r = oracle.solve(g, reset=True)
+    a = _run(gpu, g, 1, reset=True)
+    _same(a, _run(gpu, g, 0, reset=True))
+    assert a[0][0][:2] == (r.iter, r.br)
+    assert np.array_equal(a[0][1], r.strat)
+
+
+@pytest.mark.parametrize("steps", [1, 2, 3, 7])
+def test_program_any_graph_length(modes, steps):
+    """Graphs of 16 x steps launches stop anywhere in a step; the next graph
+    continues it."""
+    gpu = modes
+    g = _games(gpu)[0]
+    _same(_run(gpu, g, 1, steps_per_sync=steps), _run(gpu, g, 0))
+
+
+def test_program_batch(modes):
+    gpu = modes
+    sizes = [3, 17, 400, 2500, 10, 9000, 1, 64, 30000]
+    games = []
+    for i, n in enumerate(sizes):
+        n = max(n, 2)
+        games.append(gpu.gen_chain_game(i, n) if i % 3 == 2
+                     else gpu.gen_random_game(70 + i, n, max(1, n // 3), 1, 4))
+    _same(_run(gpu, games, 1), _run(gpu, games, 0))
+
+
+def test_program_round_cap(modes):
+    gpu = modes
+    g = gpu.gen_random_game(5, 3000, 100, 1, 4)
+    r = oracle.solve(g, max_rounds=3)
+    gpu.si_tune(gpu.SIT_PROGRAM, 1)
+    s = gpu.Solver(g)
+    s.solve(max_rounds=3)
+    it, br, capped = s.stats()
+    assert capped and br == 3 and it == r.iter
+    assert np.array_equal(s.strategy(), r.strat)
+
+
+def test_program_profiled(modes):
+    gpu = modes
+    g = _games(gpu)[1]
+    gpu.si_tune(gpu.SIT_PROGRAM, 1)
+    s = gpu.Solver(g)
+    s.solve()
+    st, strat, lab = s.stats(), s.strategy(), s.labels()
+    p = gpu.solve_program_profiled(s)
+    assert s.stats() == st
+    assert np.array_equal(s.strategy(), strat) and np.array_equal(s.labels(), lab)
+    assert p["total"][1] == st[1]                 # one INIT per valuation round
+    assert p["val_init"][1] == p["label"][1] == p["switch1"][1] == st[1]
+    assert p["jump_idle"][1] == 0 and p["jump"][1] >= st[1]
+    s.close()
+
+
+def test_solve_then_bf_on_one_handle(modes):
+    """solve_bf allocates its successor record after a plain solve captured
+    its graphs: the captured launches are rebuilt with it."""
+    gpu = modes
+    rng = np.random.default_rng(3)
+    found = 0
+    for k in range(200):
+        n = int(rng.integers(20, 300))
+        g = gpu.gen_random_game(4400 + k, n, 4 * n, 1, 4)
+        g.priority[:] = rng.permutation(n).astype(np.int32)
+        r = oracle.solve_bf(g)
+        if r.status != oracle.BF_OK or r.iter < 2:
+            continue
+        for program in (1, 0):
+            gpu.si_tune(gpu.SIT_PROGRAM, program)
+            s = gpu.Solver(g)
+            s.solve()
+            s.solve_bf()
+            assert s.stats()[0] == r.iter
+            assert np.array_equal(s.strategy(), r.strat)
+            assert np.array_equal(s.labels(), r.inf)
+            s.close()
+        found += 1
+        if found == 4:
+            break
+    assert found == 4
