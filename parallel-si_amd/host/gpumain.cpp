@@ -33,6 +33,28 @@ static bool ends_with(const std::string& s, const char* suf)
     return s.size() >= n && s.compare(s.size() - n, n, suf) == 0;
 }
 
+// bzip2's stdio-style reading interface (libbz2.so.1, the bzlib 1.0 API's
+// zlib-compatibility functions).  The image has the library but not its
+// header, so the three entry points used are declared by their documented
+// signatures.
+extern "C" {
+typedef void BZFILE;
+BZFILE* BZ2_bzopen(const char* path, const char* mode);
+int BZ2_bzread(BZFILE* b, void* buf, int len);
+void BZ2_bzclose(BZFILE* b);
+}
+
+static bool read_bz2(const std::string& path, std::string& out)
+{
+    BZFILE* f = BZ2_bzopen(path.c_str(), "rb");
+    if (!f) return false;
+    char buf[1 << 16];
+    int k;
+    while ((k = BZ2_bzread(f, buf, sizeof buf)) > 0) out.append(buf, k);
+    BZ2_bzclose(f);
+    return k == 0;
+}
+
 static bool read_gz(const std::string& path, std::string& out)
 {
     gzFile f = gzopen(path.c_str(), "rb");
@@ -107,9 +129,11 @@ int main(int argc, char** argv)
     }
     Game g;
     try {
-        if (ends_with(filename, ".gz")) {
+        // decompression by extension, as gpumain.cpp:118-120
+        if (ends_with(filename, ".gz") || ends_with(filename, ".bz2")) {
             std::string text;
-            if (!read_gz(filename, text)) throw "Unable to open file";
+            const bool ok = ends_with(filename, ".gz") ? read_gz(filename, text) : read_bz2(filename, text);
+            if (!ok) throw "Unable to open file";
             std::istringstream in(text);
             g.parse_pgsolver(in);
         } else {

@@ -1,6 +1,6 @@
 """gpuparsi, the command line of the reference's gpumain.cpp, on the GPU:
 '<seconds> <iter> <br>' plus winning sets and strategies equal the oracle's,
-for plain and gzip input, device loop and host-driven solve_si."""
+for plain, gzip and bzip2 input, device loop and host-driven solve_si."""
 import gzip
 import json
 import os
@@ -53,6 +53,25 @@ def test_cli_plain_output_and_gzip(tmp_path):
     sec, it, br = out.split()
     r = oracle.solve(g)
     assert float(sec) > 0 and (int(it), int(br)) == (r.iter, r.br)
+
+
+def test_cli_bzip2(tmp_path):
+    """.bz2 input (gpumain.cpp:118) gives the plain file's result; a damaged
+    stream is a parsing error."""
+    import bz2
+    g = psi.gen_random_game(7, 3000, 80, 1, 4)
+    path = tmp_path / "g.gm.bz2"
+    path.write_bytes(bz2.compress(g.to_gm().encode()))
+    rc, out, err = run(["--json", "--winning", "gpu", str(path)])
+    assert rc == 0, err
+    res = json.loads(out)
+    r = oracle.solve(g)
+    assert (res["iter"], res["br"]) == (r.iter, r.br)
+    assert res["w0"] == [v for v in range(g.n) if r.inf[v] == 1]
+    bad = tmp_path / "bad.gm.bz2"
+    bad.write_bytes(path.read_bytes()[:40])
+    rc, out, err = run(["gpu", str(bad)])
+    assert rc == 255 and "parsing error" in err
 
 
 def test_cli_errors(tmp_path):
