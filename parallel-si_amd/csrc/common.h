@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "../../include/psi.h"
@@ -44,6 +45,11 @@ hipStream_t stream();
 
 // Grows *buf to at least `bytes`; contents are not preserved.
 int ensure(void** buf, size_t* cap, size_t bytes);
+// Held while a stream is captured into a graph, and around the creation and
+// destruction of solver handles and pools (allocation, frees, streams,
+// synchronous copies): HIP invalidates a capture that another thread's such
+// call overlaps.
+std::recursive_mutex& setup_mutex();
 
 __host__ __device__ inline uint32_t fmix32(uint32_t h)
 {

@@ -29,6 +29,12 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line)
     return PSI_ERR_HIP;
 }
 
+std::recursive_mutex& setup_mutex()
+{
+    static std::recursive_mutex m;
+    return m;
+}
+
 hipStream_t stream()
 {
     std::call_once(g_stream_once, [] {

@@ -69,6 +69,7 @@ static int create_from_views(int n_games, const GameView* views, int flags, int 
     *out = nullptr;
     if (n_games <= 0) return fail(PSI_ERR_INVALID, "no games");
     if (int rc = psi_init()) return rc;
+    std::lock_guard<std::recursive_mutex> lk(setup_mutex());
     auto* s = new psi_solver;
     s->trace = (flags & PSI_SOLVER_TRACE) != 0;
     hipError_t e = hipStreamCreateWithFlags(&s->s, hipStreamNonBlocking);
@@ -124,6 +125,7 @@ extern "C" int psi_solver_destroy(psi_solver* s)
 {
     if (!s) return PSI_OK;
     (void)hipStreamSynchronize(s->s);
+    std::lock_guard<std::recursive_mutex> lk(setup_mutex());
     core_destroy(s->core);
     (void)hipStreamDestroy(s->s);
     delete s;
@@ -183,7 +185,8 @@ extern "C" int psi_solve_bf(psi_solver* s, int max_rounds, int steps_per_sync)
     if (steps_per_sync <= 0) steps_per_sync = 4;
     if (int rc = core_solve_bf(c, steps_per_sync)) return rc;
     std::vector<GameCtl> gc(c.G);
-    PSI_HIP(hipMemcpy(gc.data(), c.gctl, c.G * sizeof(GameCtl), hipMemcpyDeviceToHost));
+    PSI_HIP(hipMemcpyAsync(gc.data(), c.gctl, c.G * sizeof(GameCtl), hipMemcpyDeviceToHost, c.s));
+    PSI_HIP(hipStreamSynchronize(c.s));
     for (int g = 0; g < c.G; g++)
         if (gc[g].err)
             return fail(PSI_ERR_INVALID, "game " + std::to_string(g) +

@@ -317,47 +317,54 @@ __device__ __forceinline__ void jump_body(int r, long long N, int2* __restrict__
     const unsigned long long* Zin = Zl + (size_t)level_slot(r) * N;
     unsigned long long* Zout = Zl + (size_t)level_slot(r + 1) * N;
     unsigned my_res = 0, my_act = 0;
-    const long long stride = (long long)gridDim.x * blockDim.x;
     const long long lim = min(N, (long long)*hi);
-    for (long long base = (long long)blockIdx.x * blockDim.x; base < lim; base += stride) {
-        const long long b = base / kBS;
-        if (!blive[b] || !bact_in[b]) {           // uniform per block
-            if (threadIdx.x == 0) bact_out[b] = 0;
-            continue;
-        }
-        long long v = base + threadIdx.x;
-        bool res = false, act = false;
-        if (v < N) {
-            int2 s = Sin[v];
-            unsigned long long z = trace ? Zin[v] : 0;
-            if (s.x >= 0) {
-                // The successor's record at level r, loaded together with its
-                // block's last level; a block that stopped earlier (its
-                // records final) is read again at that level.
-                const int bl = blast[s.x / kBS];
-                const size_t at_r = (size_t)level_slot(r) * N + s.x;
-                int2 t = Sl[at_r];
-                size_t at = at_r;
-                if (bl < r) {
-                    at = (size_t)level_slot(bl) * N + s.x;
-                    t = Sl[at];
+    const long long nbk = (lim + kBS - 1) / kBS;  // blocks below the bound
+    const long long G = gridDim.x;
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    // This workgroup's blocks are b0, b0 + G, ...; their flags are read 64 at
+    // a time (one per lane, every wave alike), so blocks with nothing to do
+    // cost no serial round trip.
+    for (long long b0 = blockIdx.x; b0 < nbk; b0 += (long long)kWave * G) {
+        const long long bq = b0 + (long long)lane * G;
+        const bool on = bq < nbk && blive[bq] && bact_in[bq];
+        if (threadIdx.x < kWave && bq < nbk && !on) bact_out[bq] = 0;
+        for (unsigned long long m = __ballot(on); m; m &= m - 1) {
+            const long long b = b0 + (long long)(__ffsll((long long)m) - 1) * G;
+            const long long base = b * kBS;
+            long long v = base + threadIdx.x;
+            bool res = false, act = false;
+            if (v < N) {
+                int2 s = Sin[v];
+                unsigned long long z = trace ? Zin[v] : 0;
+                if (s.x >= 0) {
+                    // The successor's record at level r, loaded together with its
+                    // block's last level; a block that stopped earlier (its
+                    // records final) is read again at that level.
+                    const int bl = blast[s.x / kBS];
+                    const size_t at_r = (size_t)level_slot(r) * N + s.x;
+                    int2 t = Sl[at_r];
+                    size_t at = at_r;
+                    if (bl < r) {
+                        at = (size_t)level_slot(bl) * N + s.x;
+                        t = Sl[at];
+                    }
+                    if (trace) z += Zl[at];
+                    s.x = t.x >= 0 ? t.x : term_code(term_kind(t.x), span + term_count(t.x));
+                    s.y = max(s.y & kPriMask, t.y & kPriMask) | ((s.y | t.y) & kChangedBit);
+                    res = s.x < 0;
+                    act = !res;
                 }
-                if (trace) z += Zl[at];
-                s.x = t.x >= 0 ? t.x : term_code(term_kind(t.x), span + term_count(t.x));
-                s.y = max(s.y & kPriMask, t.y & kPriMask) | ((s.y | t.y) & kChangedBit);
-                res = s.x < 0;
-                act = !res;
+                Sout[v] = s;
+                if (trace) Zout[v] = z;
             }
-            Sout[v] = s;
-            if (trace) Zout[v] = z;
-        }
-        unsigned long long mr = __ballot(res), ma = __ballot(act);
-        my_res += (unsigned)__popcll(mr);
-        my_act += (unsigned)__popcll(ma);
-        int any = __syncthreads_or(act);
-        if (threadIdx.x == 0) {
-            bact_out[b] = (uint8_t)any;
-            blast[b] = (uint8_t)(r + 1);
+            unsigned long long mr = __ballot(res), ma = __ballot(act);
+            my_res += (unsigned)__popcll(mr);
+            my_act += (unsigned)__popcll(ma);
+            int any = __syncthreads_or(act);
+            if (threadIdx.x == 0) {
+                bact_out[b] = (uint8_t)any;
+                blast[b] = (uint8_t)(r + 1);
+            }
         }
     }
     if ((threadIdx.x & (kWave - 1)) == 0 && (my_res | my_act)) {
@@ -422,84 +429,90 @@ __device__ __forceinline__ void label_body(
         if (decide_round(lmax, jc, maxn, &stop, &cyc, &halt)) jc->err = 1;
     }
     const long long lim = min(N, (long long)*hi);
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long vb = (long long)blockIdx.x * blockDim.x; vb < lim; vb += stride) {
-        const long long b = vb / kBS;
-        if (!blive[b]) continue;                  // uniform: the whole block is finished
-        // this block's final level (it may have stopped jumping earlier)
-        const int Fb = min(F, (int)blast[b]);
-        const int2* SF = Sl + (size_t)level_slot(Fb) * N;
-        const unsigned long long* ZF = Zl + (size_t)level_slot(Fb) * N;
-        const long long v = vb + threadIdx.x;
-        int g = -1, l = 0, sol = 0, loc = 0;
-        unsigned long long d = 0;
-        bool live = false, ch = false;
-        if (v < N) {
-            g = gid[v];
-            live = !gc[g].done;
-        }
-        if (live) {
-            sol = solved[v];
-            loc = (int)(v - gc[g].vbase);
-            const int lold = lab[v];
-            if (sol) {
-                l = lold;
-                key[v] = (l + 1) << 30;
-                dep[v] = 0;
-            } else {
-                int2 s = SF[v];
-                const int kind = s.x < 0 ? term_kind(s.x) : -1;
-                if (kind == 0) {                   // reaches the sink: finite
-                    l = 0;
-                    dep[v] = term_count(s.x);
-                    key[v] = (1 << 30) | (s.y & kPriMask);
-                    ch = (s.y & kChangedBit) != 0; // a link on the path changed
-                    if (trace) d = ZF[v];
-                } else {
-                    if (kind == 1) l = 1;
-                    else if (kind == 2) l = -1;
-                    else if (kind < 0) {           // largest priority on the cycle
-                        const int Fy = min(F, (int)blast[s.x / kBS]);
-                        l = (Sl[(size_t)level_slot(Fy) * N + s.x].y & 1) ? -1 : 1;
-                        // solve_bf: the relaxation never closes a cycle (such
-                        // vertices stay infinite, cpu_bf.cpp:156-160); the
-                        // first best response stops on an odd one.
-                        if (gc[g].bf) {
-                            if (gc[g].bf_init) l = 1;
-                            else if (l == -1) gc[g].err = 1;
-                        }
-                    } else {
-                        l = 1;                     // kind 3 (finished game): never on a live path
-                    }
+    const long long nbk = (lim + kBS - 1) / kBS;
+    const long long G = gridDim.x;
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    // blocks b0, b0 + G, ... with their live flags read 64 at a time
+    for (long long b0 = blockIdx.x; b0 < nbk; b0 += (long long)kWave * G) {
+        const long long bq = b0 + (long long)lane * G;
+        for (unsigned long long m = __ballot(bq < nbk && blive[bq]); m; m &= m - 1) {
+            const long long b = b0 + (long long)(__ffsll((long long)m) - 1) * G;
+            const long long vb = b * kBS;
+            // this block's final level (it may have stopped jumping earlier)
+            const int Fb = min(F, (int)blast[b]);
+            const int2* SF = Sl + (size_t)level_slot(Fb) * N;
+            const unsigned long long* ZF = Zl + (size_t)level_slot(Fb) * N;
+            const long long v = vb + threadIdx.x;
+            int g = -1, l = 0, sol = 0, loc = 0;
+            unsigned long long d = 0;
+            bool live = false, ch = false;
+            if (v < N) {
+                g = gid[v];
+                live = !gc[g].done;
+            }
+            if (live) {
+                sol = solved[v];
+                loc = (int)(v - gc[g].vbase);
+                const int lold = lab[v];
+                if (sol) {
+                    l = lold;
                     key[v] = (l + 1) << 30;
                     dep[v] = 0;
+                } else {
+                    int2 s = SF[v];
+                    const int kind = s.x < 0 ? term_kind(s.x) : -1;
+                    if (kind == 0) {                   // reaches the sink: finite
+                        l = 0;
+                        dep[v] = term_count(s.x);
+                        key[v] = (1 << 30) | (s.y & kPriMask);
+                        ch = (s.y & kChangedBit) != 0; // a link on the path changed
+                        if (trace) d = ZF[v];
+                    } else {
+                        if (kind == 1) l = 1;
+                        else if (kind == 2) l = -1;
+                        else if (kind < 0) {           // largest priority on the cycle
+                            const int Fy = min(F, (int)blast[s.x / kBS]);
+                            l = (Sl[(size_t)level_slot(Fy) * N + s.x].y & 1) ? -1 : 1;
+                            // solve_bf: the relaxation never closes a cycle (such
+                            // vertices stay infinite, cpu_bf.cpp:156-160); the
+                            // first best response stops on an odd one.
+                            if (gc[g].bf) {
+                                if (gc[g].bf_init) l = 1;
+                                else if (l == -1) gc[g].err = 1;
+                            }
+                        } else {
+                            l = 1;                     // kind 3 (finished game): never on a live path
+                        }
+                        key[v] = (l + 1) << 30;
+                        dep[v] = 0;
+                    }
+                    ch = ch || l != lold;
+                    lab[v] = (int8_t)l;
+                    if (loop_form && l == -1) solved[v] = 1;
                 }
-                ch = ch || l != lold;
-                lab[v] = (int8_t)l;
-                if (loop_form && l == -1) solved[v] = 1;
+                if (trace) dig[v] = d;
             }
-            if (trace) dig[v] = d;
-        }
-        // Valuation changed since the previous one (one word per wave), and
-        // since the last switch(0) pass of the vertex's game: that word is
-        // cleared for games that took the outer step in the previous step.
-        const unsigned long long cm = __ballot(ch);
-        const int restart = live ? gc[g].clear0 : 0;
-        const unsigned long long clr = __ballot(restart != 0);
-        if ((threadIdx.x & (kWave - 1)) == 0) {
-            chg[v >> 6] = cm;
-            chg0[v >> 6] = (chg0[v >> 6] & ~clr) | cm;
-        }
-        if (trace) {
-            int br = live ? gc[g].br : 0;
-            bool rec = live && br < trace_cap;
-            unsigned long long* base = rec ? tr + ((size_t)g * trace_cap + br) * 4 : nullptr;
-            int sv = rec ? strat[v] : -1;
-            if (sv >= 0) sv -= gc[g].vbase;              // local ids, as a game alone
-            wave_add_u64(rec ? base + 0 : nullptr, rec ? hmix_i32(loc, sv) : 0);
-            wave_add_u64(rec ? base + 1 : nullptr, rec ? hmix_i32(loc, sol) : 0);
-            wave_add_u64(rec ? base + 2 : nullptr, rec ? hmix_i32(loc, l) : 0);
-            wave_add_u64(rec ? base + 3 : nullptr, rec ? hmix_u64(loc, d) : 0);
+            // Valuation changed since the previous one (one word per wave), and
+            // since the last switch(0) pass of the vertex's game: that word is
+            // cleared for games that took the outer step in the previous step.
+            const unsigned long long cm = __ballot(ch);
+            const int restart = live ? gc[g].clear0 : 0;
+            const unsigned long long clr = __ballot(restart != 0);
+            if ((threadIdx.x & (kWave - 1)) == 0) {
+                chg[v >> 6] = cm;
+                chg0[v >> 6] = (chg0[v >> 6] & ~clr) | cm;
+            }
+            if (trace) {
+                int br = live ? gc[g].br : 0;
+                bool rec = live && br < trace_cap;
+                unsigned long long* base = rec ? tr + ((size_t)g * trace_cap + br) * 4 : nullptr;
+                int sv = rec ? strat[v] : -1;
+                if (sv >= 0) sv -= gc[g].vbase;              // local ids, as a game alone
+                wave_add_u64(rec ? base + 0 : nullptr, rec ? hmix_i32(loc, sv) : 0);
+                wave_add_u64(rec ? base + 1 : nullptr, rec ? hmix_i32(loc, sol) : 0);
+                wave_add_u64(rec ? base + 2 : nullptr, rec ? hmix_i32(loc, l) : 0);
+                wave_add_u64(rec ? base + 3 : nullptr, rec ? hmix_u64(loc, d) : 0);
+            }
         }
     }
 }
@@ -1104,6 +1117,7 @@ template <class T> __global__ void k_fill(T* p, long long n, T v)
 
 template <class T> static int dalloc(T** p, long long n)
 {
+    std::lock_guard<std::recursive_mutex> lk(setup_mutex());
     PSI_HIP(hipMalloc((void**)p, (size_t)std::max(n, 1LL) * sizeof(T)));
     return PSI_OK;
 }
@@ -1335,7 +1349,8 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
         // count of the last block, recomputed on the host side from vinfo tail
         long long lo = (long long)(nb - 1) * kBS;
         std::vector<int> tail(N - lo);
-        PSI_HIP(hipMemcpy(tail.data(), c.vinfo + lo, (N - lo) * 4, hipMemcpyDeviceToHost));
+        PSI_HIP(hipMemcpyAsync(tail.data(), c.vinfo + lo, (N - lo) * 4, hipMemcpyDeviceToHost, s));
+        PSI_HIP(hipStreamSynchronize(s));
         for (int x : tail) last_cnt += (x >> 30) & 1;
     }
     c.own_n[1] = (long long)last_pre + last_cnt;
@@ -1638,6 +1653,7 @@ int core_set_params(Core& c, int reset, int max_rounds, int bf)
     if (bf && !c.succ) {
         if (int rc = dalloc(&c.succ, c.N)) return rc;
         // captured launches hold the (null) succ they were captured with
+        std::lock_guard<std::recursive_mutex> lk(setup_mutex());
         if (c.graph) (void)hipGraphExecDestroy(c.graph);
         if (c.pgraph) (void)hipGraphExecDestroy(c.pgraph);
         c.graph = nullptr;
@@ -1731,7 +1747,10 @@ int core_solve_profiled(Core& c, double* times, long long* counts)
     };
     int rc = PSI_OK;
     unsigned long long* ties = nullptr;
-    PSI_HIP(hipMalloc(&ties, 24));
+    {
+        std::lock_guard<std::recursive_mutex> lk(setup_mutex());
+        PSI_HIP(hipMalloc(&ties, 24));
+    }
     PSI_HIP(hipMemsetAsync(ties, 0, 24, c.s));
     c.prof_ties = ties;
     for (int step = 0; step < (1 << 20); step++) {
@@ -1756,7 +1775,8 @@ int core_solve_profiled(Core& c, double* times, long long* counts)
         PSI_LAUNCH_CHECK();
         PSI_HIP(hipStreamSynchronize(c.s));
         JumpCnt jc;
-        PSI_HIP(hipMemcpy(&jc, c.jc, sizeof jc, hipMemcpyDeviceToHost));
+        PSI_HIP(hipMemcpyAsync(&jc, c.jc, sizeof jc, hipMemcpyDeviceToHost, c.s));
+        PSI_HIP(hipStreamSynchronize(c.s));
         int e = 0;
         times[PSI_K_VAL_INIT] += span(e, e + 1);
         counts[PSI_K_VAL_INIT]++;
@@ -1788,12 +1808,16 @@ int core_solve_profiled(Core& c, double* times, long long* counts)
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     unsigned long long h_ties[3] = {0, 0, 0};
-    (void)hipMemcpy(h_ties, ties, 24, hipMemcpyDeviceToHost);
+    (void)hipMemcpyAsync(h_ties, ties, 24, hipMemcpyDeviceToHost, c.s);
+    (void)hipStreamSynchronize(c.s);
     counts[PSI_K_TIES] = (long long)h_ties[0];
     counts[PSI_K_SCANS] = (long long)h_ties[1];
     counts[PSI_K_SCAN_STEPS] = (long long)h_ties[2];
     c.prof_ties = nullptr;
-    (void)hipFree(ties);
+    {
+        std::lock_guard<std::recursive_mutex> lk(setup_mutex());
+        (void)hipFree(ties);
+    }
     return rc;
 }
 
@@ -1819,6 +1843,7 @@ int core_launch_steps(Core& c, int steps_per_check)
             c.graph = nullptr;
         }
         hipGraph_t gr;
+        std::lock_guard<std::recursive_mutex> lk(setup_mutex());
         PSI_HIP(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
         int rc = core_enqueue_steps(c, steps_per_check);
         hipError_t e = hipStreamEndCapture(c.s, &gr);
@@ -1902,6 +1927,7 @@ static int launch_program(Core& c, int launches)
         if (int rc = upload_step_args(c, &a)) return rc;
         const unsigned grid = std::max(1u, std::min(nblocks(c.N, kBS), g_si_tune.phase_grid));
         hipGraph_t gr;
+        std::lock_guard<std::recursive_mutex> lk(setup_mutex());
         PSI_HIP(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < launches; i++) k_phase<<<grid, kBS, 0, c.s>>>(i, a, c.prog);
         hipError_t e = hipGetLastError();

@@ -209,6 +209,7 @@ extern "C" int psi_pool_create(int64_t hbm_budget, double edges_per_vertex, int 
     p->slots = max_games;
     p->max_n = max_n;
     int rc = psi_init();
+    std::unique_lock<std::recursive_mutex> lk(setup_mutex());
     if (rc == PSI_OK) {
         hipError_t e = hipStreamCreateWithFlags(&p->s1, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->s2, hipStreamNonBlocking);
@@ -218,6 +219,7 @@ extern "C" int psi_pool_create(int64_t hbm_budget, double edges_per_vertex, int 
         if (e != hipSuccess) rc = hip_fail(e, "pool streams and pinned buffers", __FILE__, __LINE__);
     }
     if (rc == PSI_OK) rc = core_create_pool(p->core, vcap, ecap, max_games, max_n, p->s1);
+    lk.unlock();
     if (rc) {
         psi_pool_destroy(p);
         return rc;
@@ -236,6 +238,7 @@ extern "C" int psi_pool_destroy(psi_pool* p)
     if (!p) return PSI_OK;
     if (p->s1) (void)hipStreamSynchronize(p->s1);
     if (p->s2) (void)hipStreamSynchronize(p->s2);
+    std::lock_guard<std::recursive_mutex> lk(setup_mutex());
     core_destroy(p->core);
     if (p->s1) (void)hipStreamDestroy(p->s1);
     if (p->s2) (void)hipStreamDestroy(p->s2);
