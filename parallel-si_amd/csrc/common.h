@@ -4,8 +4,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 
@@ -50,6 +53,21 @@ int ensure(void** buf, size_t* cap, size_t bytes);
 // synchronous copies): HIP invalidates a capture that another thread's such
 // call overlaps.
 std::recursive_mutex& setup_mutex();
+
+// f(lo, hi) over [0, n) split into one range per worker of a persistent host
+// pool (up to 16 threads); calls do not nest.
+void parallel_for(long long n, const std::function<void(long long, long long)>& f);
+// Large transfers through two pinned chunks, host side on the pool.
+struct HostPart {
+    const void* src;
+    size_t bytes;
+};
+// The concatenated parts to device dst, ordered on stream s (the sources
+// may be released on return).
+int staged_upload(hipStream_t s, void* dst, const HostPart* parts, int nparts);
+// Device src to consume(chunk, byte offset, length), chunk by chunk in order.
+int staged_download(hipStream_t s, const void* src, size_t total,
+                    const std::function<void(const char*, size_t, size_t)>& consume);
 
 __host__ __device__ inline uint32_t fmix32(uint32_t h)
 {

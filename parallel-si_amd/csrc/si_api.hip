@@ -409,6 +409,45 @@ extern "C" int psi_get_jump_stats(psi_solver* s, int* out, int rounds, int* last
 extern "C" int psi_get_results(psi_solver* s, int* strat_all, int* labels_all, int* stats_all)
 {
     Core& c = s->core;
+    if (c.N * 4 >= (32LL << 20)) {
+        // large: through pinned chunks, localised and widened on the host pool
+        if (strat_all) {
+            int rc = staged_download(c.s, c.strat, (size_t)c.N * 4, [&](const char* buf, size_t off, size_t len) {
+                const long long v0 = (long long)(off / 4), nv = (long long)(len / 4);
+                const int* in = (const int*)buf;
+                parallel_for(nv, [&](long long lo, long long hi) {
+                    long long v = v0 + lo;
+                    int g = (int)(std::upper_bound(c.h_vbase.begin(), c.h_vbase.end(), (int)v) - c.h_vbase.begin()) - 1;
+                    for (long long k = lo; k < hi; k++, v++) {
+                        while (g + 1 < c.G && v >= c.h_vbase[g + 1]) g++;
+                        const int x = in[k];
+                        strat_all[v] = x >= 0 ? x - c.h_vbase[g] : x;
+                    }
+                });
+            });
+            if (rc) return rc;
+        }
+        if (labels_all) {
+            int rc = staged_download(c.s, c.lab, (size_t)c.N, [&](const char* buf, size_t off, size_t len) {
+                const int8_t* in = (const int8_t*)buf;
+                parallel_for((long long)len, [&](long long lo, long long hi) {
+                    for (long long k = lo; k < hi; k++) labels_all[off + k] = in[k];
+                });
+            });
+            if (rc) return rc;
+        }
+        if (stats_all) {
+            std::vector<GameCtl> gc(c.G);
+            PSI_HIP(hipMemcpyAsync(gc.data(), c.gctl, c.G * sizeof(GameCtl), hipMemcpyDeviceToHost, c.s));
+            PSI_HIP(hipStreamSynchronize(c.s));
+            for (int g = 0; g < c.G; g++) {
+                stats_all[3 * g] = gc[g].iter;
+                stats_all[3 * g + 1] = gc[g].br;
+                stats_all[3 * g + 2] = gc[g].capped;
+            }
+        }
+        return PSI_OK;
+    }
     if (strat_all) {
         PSI_HIP(hipMemcpyAsync(strat_all, c.strat, (size_t)c.N * 4, hipMemcpyDeviceToHost, c.s));
     }

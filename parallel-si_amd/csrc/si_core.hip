@@ -1320,14 +1320,30 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
     if (int rc = dalloc(&d_ebase, G + 1)) return rc;
     if (int rc = dalloc(&d_cnt, nb)) return rc;
     pt.mark("alloc");
-    // Each game's arrays go straight to their place; owner and priority are
-    // staged in par/dep, which the first valuation overwrites.
-    for (int g = 0; g < G; g++) {
-        const GameView& gv = games[g];
-        PSI_HIP(hipMemcpyAsync(c.par + vbase[g], gv.owner, (size_t)gv.n * 4, hipMemcpyHostToDevice, s));
-        PSI_HIP(hipMemcpyAsync(c.dep + vbase[g], gv.pri, (size_t)gv.n * 4, hipMemcpyHostToDevice, s));
-        PSI_HIP(hipMemcpyAsync(c.off + vbase[g], gv.off, (size_t)gv.n * 4, hipMemcpyHostToDevice, s));
-        PSI_HIP(hipMemcpyAsync(c.tgt + ebase[g], gv.tgt, (size_t)gv.m * 4, hipMemcpyHostToDevice, s));
+    // Each game's arrays go straight to their place (large totals through
+    // pinned chunks); owner and priority are staged in par/dep, which the
+    // first valuation overwrites.
+    if ((size_t)(3 * N + M) * 4 >= (64ull << 20)) {
+        std::vector<HostPart> own(G), pri(G), offs(G), tg(G);
+        for (int g = 0; g < G; g++) {
+            const GameView& gv = games[g];
+            own[g] = HostPart{gv.owner, (size_t)gv.n * 4};
+            pri[g] = HostPart{gv.pri, (size_t)gv.n * 4};
+            offs[g] = HostPart{gv.off, (size_t)gv.n * 4};
+            tg[g] = HostPart{gv.tgt, (size_t)gv.m * 4};
+        }
+        if (int rc = staged_upload(s, c.par, own.data(), G)) return rc;
+        if (int rc = staged_upload(s, c.dep, pri.data(), G)) return rc;
+        if (int rc = staged_upload(s, c.off, offs.data(), G)) return rc;
+        if (int rc = staged_upload(s, c.tgt, tg.data(), G)) return rc;
+    } else {
+        for (int g = 0; g < G; g++) {
+            const GameView& gv = games[g];
+            PSI_HIP(hipMemcpyAsync(c.par + vbase[g], gv.owner, (size_t)gv.n * 4, hipMemcpyHostToDevice, s));
+            PSI_HIP(hipMemcpyAsync(c.dep + vbase[g], gv.pri, (size_t)gv.n * 4, hipMemcpyHostToDevice, s));
+            PSI_HIP(hipMemcpyAsync(c.off + vbase[g], gv.off, (size_t)gv.n * 4, hipMemcpyHostToDevice, s));
+            PSI_HIP(hipMemcpyAsync(c.tgt + ebase[g], gv.tgt, (size_t)gv.m * 4, hipMemcpyHostToDevice, s));
+        }
     }
     int Mi = (int)M;
     PSI_HIP(hipMemcpyAsync(c.off + N, &Mi, 4, hipMemcpyHostToDevice, s));
