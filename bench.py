@@ -75,10 +75,11 @@ def si_bench(psi, n, steps, warmup, seed):
     # the timed solves ran).
     prof = psi.solve_profiled(s)
     log("[bench] per-kernel ms/launches: " + ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in prof.items()))
+    prounds = psi.profile_rounds(s)
     pprof = psi.solve_program_profiled(s)
     log("[bench] step-program phases ms/launches: " +
         ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in pprof.items() if v[1]))
-    return g, s, it, br, wall, prof, pprof
+    return g, s, it, br, wall, prof, pprof, prounds
 
 
 def listrank_bench(psi, reps=10, tune=True):
@@ -464,7 +465,7 @@ def main():
         return
     if a.only == "si":
         a.no_listrank = a.no_cpu = a.no_batch = a.no_serve = a.no_bf = True
-    g, s, it, br, wall, prof, pprof = si_bench(psi, a.n, a.steps, a.warmup, a.seed)
+    g, s, it, br, wall, prof, pprof, prounds = si_bench(psi, a.n, a.steps, a.warmup, a.seed)
     value = g.n * br * a.steps / wall
     # Roofline of the dominant kernel: a pointer-jumping round reads and
     # writes every vertex's 8-byte record (successor or terminal code and
@@ -481,7 +482,8 @@ def main():
             "traffic": measured_traffic(g.n, a.seed), "alg_bytes_per_launch": round(jump_bytes),
             "avg_launch_us": round(jump_ms / max(jump_launches, 1) * 1e3, 2),
             "launches": jump_launches,
-            "program_avg_launch_us": round(pprof["jump"][0] / max(pprof["jump"][1], 1) * 1e3, 2)}
+            "program_avg_launch_us": round(pprof["jump"][0] / max(pprof["jump"][1], 1) * 1e3, 2),
+            "rounds_per_valuation": prounds}
     line = {
         "metric": "SI vertex-iterations/s on 4M-vertex game",
         "value": round(value, 1),

@@ -184,6 +184,9 @@ int psi_get_stats(psi_solver* s, int game, int* iter, int* br, int* capped);
 #define PSI_K_COUNT 12
 int psi_solve_profiled(psi_solver* s, int reset, int max_rounds, double* times_ms,
                        int64_t* counts);  /* both arrays have PSI_K_COUNT entries */
+/* Jump rounds that ran in each valuation of the last psi_solve_profiled, in
+ * order (its launches per valuation: lmax rounds, these first). */
+int psi_get_profile_rounds(psi_solver* s, int* out, int cap, int* count);
 /* The step program (PSI_SIT_PROGRAM) launched one launch at a time with HIP
  * events around each: the same classes by the phase each launch ran (jump
  * rounds after convergence are never launched; mark_solved(0) counts with

@@ -226,6 +226,14 @@ extern "C" int psi_solve_profiled(psi_solver* s, int reset, int max_rounds, doub
     return rc;
 }
 
+extern "C" int psi_get_profile_rounds(psi_solver* s, int* out, int cap, int* count)
+{
+    const std::vector<int>& r = s->core.prof_rounds;
+    *count = (int)r.size();
+    for (int k = 0; k < cap && k < (int)r.size(); k++) out[k] = r[k];
+    return PSI_OK;
+}
+
 extern "C" int psi_solve_program_profiled(psi_solver* s, int reset, int max_rounds, double* times_ms,
                                           int64_t* counts)
 {

@@ -125,6 +125,7 @@ struct Core {
     std::vector<int> h_n, h_vbase;
     std::vector<long long> h_m;
     unsigned long long* prof_ties = nullptr;  // profiled solves: compares taking the exact path
+    std::vector<int> prof_rounds;             // last profiled solve: jump rounds run per valuation
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     StepProg* prog = nullptr;          // step program state (not in a serving pool)

@@ -1769,6 +1769,7 @@ int core_solve_profiled(Core& c, double* times, long long* counts)
     }
     PSI_HIP(hipMemsetAsync(ties, 0, 24, c.s));
     c.prof_ties = ties;
+    c.prof_rounds.clear();
     for (int step = 0; step < (1 << 20); step++) {
         int k = 0;
         PSI_HIP(hipEventRecord(ev[k++], c.s));
@@ -1794,6 +1795,7 @@ int core_solve_profiled(Core& c, double* times, long long* counts)
         PSI_HIP(hipMemcpyAsync(&jc, c.jc, sizeof jc, hipMemcpyDeviceToHost, c.s));
         PSI_HIP(hipStreamSynchronize(c.s));
         int e = 0;
+        c.prof_rounds.push_back(jc.last + 1);
         times[PSI_K_VAL_INIT] += span(e, e + 1);
         counts[PSI_K_VAL_INIT]++;
         e++;

@@ -432,6 +432,16 @@ def solve_profiled(solver, reset=False, max_rounds=0):
     check(lib.psi_solve_profiled(solver.h, int(reset), max_rounds, t, c), "psi_solve_profiled")
     return {k: (float(t[i]), int(c[i])) for i, k in enumerate(K_NAMES)}
 
+_sig("psi_get_profile_rounds", C.c_int, _vp, _i32p, C.c_int, C.POINTER(C.c_int))
+
+
+def profile_rounds(solver, cap=1 << 16):
+    """Jump rounds run per valuation in the last solve_profiled."""
+    out = np.zeros(cap, np.int32)
+    k = C.c_int()
+    check(lib.psi_get_profile_rounds(solver.h, out, cap, C.byref(k)), "psi_get_profile_rounds")
+    return out[: min(k.value, cap)].tolist()
+
 _sig("psi_solve_program_profiled", C.c_int, _vp, C.c_int, C.c_int,
      np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS"), _i64p)
 
