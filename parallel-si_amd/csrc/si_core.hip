@@ -722,6 +722,11 @@ __device__ __forceinline__ void switch_body(
     for (int w = (int)(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); w < nruns; w += nw) {
         const int i = w * kWave + lane;
         bool sw = false, live = false;
+        // Only the successors whose valuation changed can beat a current
+        // successor whose valuation did not (the others were no better than it
+        // at the last scan, which left it a maximum), so such a vertex compares
+        // just those, in CSR order: the result is the full scan's.
+        bool only_changed = false;
         int g = -1;
         const int v = i < lim ? own[i] : -1;
         if (v >= 0) {
@@ -730,6 +735,7 @@ __device__ __forceinline__ void switch_body(
             if (live && incremental && !(player == 1 && gc[g].reset_pending)) {
                 const int c0 = strat[v];
                 bool dirty = c0 >= 0 && ((cb[c0 >> 6] >> (c0 & 63)) & 1);
+                only_changed = !dirty;
                 const int e1 = off[v + 1];
                 for (int e0 = off[v]; e0 < e1 && !dirty; e0 += 8) {
                     const int cnt = min(8, e1 - e0);
@@ -757,8 +763,13 @@ __device__ __forceinline__ void switch_body(
             int us[kSwBatch], ks[kSwBatch];
 #pragma unroll
             for (int k = 0; k < kSwBatch; k++) us[k] = k < cnt ? tgt[e + k] : -1;
+            if (only_changed) {                   // -2: unchanged, skipped
 #pragma unroll
-            for (int k = 0; k < kSwBatch; k++) ks[k] = k < cnt ? key[us[k]] : 0;
+                for (int k = 0; k < kSwBatch; k++)
+                    if (us[k] >= 0 && !((cb[us[k] >> 6] >> (us[k] & 63)) & 1)) us[k] = -2;
+            }
+#pragma unroll
+            for (int k = 0; k < kSwBatch; k++) ks[k] = us[k] >= 0 ? key[us[k]] : 0;
             // One compare at a time (a rolled loop, so the exact-compare code
             // exists once; k is uniform, so us[k] and ks[k] are register reads).
 #pragma unroll 1
@@ -766,7 +777,7 @@ __device__ __forceinline__ void switch_body(
                 const int u = us[k], ku = ks[k];
                 int c = 0, la = 0, lb = 0;
                 bool tie = false;
-                if (k < cnt) {
+                if (u >= 0) {
                     c = cmp_fast(u, ku, cur, kcur, &tie);
                     if (tie) {
                         if (ties) atomicAdd(ties, 1ull);
@@ -781,7 +792,7 @@ __device__ __forceinline__ void switch_body(
                     if ((int)(threadIdx.x & (kWave - 1)) == L) c = r;
                     need &= need - 1;
                 }
-                if (k < cnt && (player == 0 ? c > 0 : c < 0)) {
+                if (u >= 0 && (player == 0 ? c > 0 : c < 0)) {
                     cur = u;
                     kcur = ku;
                     sw = true;
