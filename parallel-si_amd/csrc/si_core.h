@@ -51,6 +51,7 @@ struct GameCtl {
     int bf;         // solve_bf (main.cpp:57-77): later best responses by relaxation
     int bf_init;    // solve_bf: the first best response (an SI inner loop) is done
     int err;        // solve_bf: odd strategy cycle in the first best response (cpu_bv.cpp:73-84)
+    int activate;   // serving pool, step program: admitted (done until the next step boundary)
 };
 
 // Jump-round bookkeeping (device memory), rewritten by every valuation.
@@ -76,6 +77,8 @@ struct StepProg {
     int st[2];
     int any_outer;  // some game takes the outer step (set by the control phase)
     int live;       // games not done
+    int host_seq;   // serving pool: admissions written by the host (between graphs)
+    int dev_seq[2]; // by launch parity: admissions activated as of that launch
     unsigned long long psum[2];
     unsigned long long pslot[4][kSlots];
 };
@@ -183,6 +186,9 @@ int core_pool_admit(Core& c, int slot, long long vbase, long long ebase, const G
                     hipStream_t s2);
 int core_pool_localize(Core& c, int slot, hipStream_t s2);
 int core_pool_release(Core& c, int slot, hipStream_t s2);
+// Serving pool under the step program: launches one graph of 16 x steps
+// program launches (admitted games join at the next step boundary).
+int core_pool_launch_program(Core& c, int steps);
 int core_solve_profiled(Core& c, double* times, long long* counts);
 int core_solve_program_profiled(Core& c, double* times, long long* counts);
 // solve_bf for every game of the core (after core_init_strat and

@@ -966,7 +966,23 @@ __global__ void k_ctl0(int G, GameCtl* gc) { ctl0_body(G, gc, nullptr); }
 // so no launch is spent on a round after convergence or on an outer-step
 // phase that no game takes; a graph of these launches can stop anywhere and
 // the next graph continues.  Once no game is live every launch exits at once.
-enum Phase { PH_START = 0, PH_INIT, PH_JUMP, PH_LABEL, PH_SW1, PH_CTL1, PH_MARK0, PH_SW0, PH_CTL0, PH_DONE };
+enum Phase {
+    PH_START = 0, PH_INIT, PH_JUMP, PH_LABEL, PH_SW1, PH_CTL1, PH_MARK0, PH_SW0, PH_CTL0, PH_DONE, PH_ACT
+};
+
+// Serving pool: games the host admitted (done, activate set) become live,
+// at a step boundary; the occupied bound grows to cover them.
+__device__ __forceinline__ void act_body(int G, GameCtl* __restrict__ gc, StepProg* __restrict__ pg,
+                                         int* __restrict__ hi)
+{
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
+        if (!gc[g].activate) continue;
+        gc[g].activate = 0;
+        gc[g].done = 0;
+        atomicAdd(&pg->live, 1);
+        atomicMax(hi, gc[g].vbase + gc[g].n + 1);
+    }
+}
 
 struct StepArgs {
     long long N, nb, own_n0, own_n1;
@@ -1005,8 +1021,13 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         case PH_MARK0: ph = PH_SW0; break;
         case PH_SW0: ph = PH_CTL0; break;
         case PH_DONE: break;
-        default: ph = PH_INIT; break;             // PH_START, PH_CTL0
+        default: ph = PH_INIT; break;             // PH_START, PH_CTL0, PH_ACT
         }
+        // at a step boundary: admitted games first (a serving pool), then
+        // the next step if any game is live
+        const int hseq = pg->host_seq;
+        const int dseq = pg->dev_seq[(i + 1) & 1];
+        if ((ph == PH_INIT || ph == PH_DONE) && hseq != dseq) ph = PH_ACT;
         if (ph == PH_INIT) {
             r = 0;
             if (pg->live <= 0) ph = PH_DONE;
@@ -1031,6 +1052,7 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
             if (lane == 0) {
                 pg->st[i & 1] = ph | (r << 8);
                 if (ph == PH_SW1) pg->any_outer = 0;   // set again by CTL1
+                pg->dev_seq[i & 1] = ph == PH_ACT ? hseq : dseq;
             }
         }
         if (lane == 0) {
@@ -1066,6 +1088,7 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
     case PH_CTL1: ctl1_body(a.G, a.gc, pg); break;
     case PH_MARK0: mark0_body(a.N, a.gid, a.gc, a.lab, a.solved, a.hi, a.vinfo, a.strat, a.succ); break;
     case PH_CTL0: ctl0_body(a.G, a.gc, pg); break;
+    case PH_ACT: act_body(a.G, a.gc, pg, a.flag + 2); break;
     default: break;
     }
 }
@@ -1083,6 +1106,7 @@ __global__ void k_prog_reset(int G, const GameCtl* __restrict__ gc, StepProg* __
     if (threadIdx.x == 0) {
         pg->st[0] = PH_START;
         pg->st[1] = PH_START;
+        pg->host_seq = pg->dev_seq[0] = pg->dev_seq[1] = 0;
         pg->psum[0] = pg->psum[1] = 0;
         pg->any_outer = 0;
         pg->live = acc;
@@ -1540,6 +1564,7 @@ int core_create_pool(Core& c, long long vcap, long long ecap, int slots, int max
     if (int rc = dalloc(&c.jc, 1)) return rc;
     if (int rc = dalloc(&c.flag, 4)) return rc;
     if (int rc = dalloc(&c.cnt_scratch, nblocks((long long)maxn + 1, kBS))) return rc;
+    if (int rc = dalloc(&c.prog, 1)) return rc;
     c.own_n[0] = c.own_n[1] = N;
     std::vector<GameCtl> gc(G);
     for (auto& x : gc) {
@@ -1553,6 +1578,7 @@ int core_create_pool(Core& c, long long vcap, long long ecap, int slots, int max
     k_fill<int><<<1, 64, 0, s>>>(c.flag, 4, 0);
     const long long jn = (long long)(sizeof(JumpCnt) / 4);
     k_fill<int><<<nblocks(jn, kBS), kBS, 0, s>>>((int*)c.jc, jn, 0);
+    k_prog_reset<<<1, 256, 0, s>>>(G, c.gctl, c.prog);
     PSI_LAUNCH_CHECK();
     PSI_HIP(hipStreamSynchronize(s));
     return PSI_OK;
@@ -2022,6 +2048,11 @@ int core_solve_program_profiled(Core& c, double* times, long long* counts)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return rc;
+}
+
+int core_pool_launch_program(Core& c, int steps)
+{
+    return launch_program(c, 16 * std::max(1, steps));
 }
 
 int core_solve(Core& c, int steps_per_check)

@@ -101,6 +101,9 @@ struct psi_pool {
     GameCtl* hctl = nullptr;              // pinned: control blocks being activated
     GameCtl* hread = nullptr;             // pinned: control blocks read back
     int* herr = nullptr;                  // pinned: the jump rounds' error flag
+    int* hseq = nullptr;                  // pinned: admissions written (step program)
+    bool program = false;                 // steps as the step program (fixed at creation)
+    int admitted = 0;
     std::deque<Request> queue;
     int head_bypassed = 0;
     std::unordered_map<int64_t, int> state;      // 0 queued, 1 running, 2 finished
@@ -137,8 +140,19 @@ static int pool_admit(psi_pool* p, const Request& r, bool* admitted)
     g.n = r.gv.n;
     g.reset = r.reset;
     g.max_rounds = r.max_rounds;
-    // Ordered after the step batch in flight: the game joins the next one.
+    // Ordered after the step batch in flight: the game joins the next one
+    // (the step program activates it at its next step boundary, since a
+    // batch of its launches may end inside a step).
+    if (p->program) {
+        g.done = 1;
+        g.activate = 1;
+    }
     PSI_HIP(hipMemcpyAsync(p->core.gctl + slot, &g, sizeof g, hipMemcpyHostToDevice, p->core.s));
+    if (p->program) {
+        *p->hseq = ++p->admitted;
+        PSI_HIP(hipMemcpyAsync(&p->core.prog->host_seq, p->hseq, sizeof(int), hipMemcpyHostToDevice,
+                               p->core.s));
+    }
     p->slot_ticket[slot] = r.ticket;
     p->slot_ebase[slot] = eb;
     p->state[r.ticket] = 1;
@@ -216,6 +230,8 @@ extern "C" int psi_pool_create(int64_t hbm_budget, double edges_per_vertex, int 
         if (e == hipSuccess) e = hipHostMalloc((void**)&p->hctl, sizeof(GameCtl) * (max_games + 1));
         if (e == hipSuccess) e = hipHostMalloc((void**)&p->hread, sizeof(GameCtl) * (max_games + 1));
         if (e == hipSuccess) e = hipHostMalloc((void**)&p->herr, sizeof(int));
+        if (e == hipSuccess) e = hipHostMalloc((void**)&p->hseq, sizeof(int));
+        p->program = g_si_tune.program != 0;
         if (e != hipSuccess) rc = hip_fail(e, "pool streams and pinned buffers", __FILE__, __LINE__);
     }
     if (rc == PSI_OK) rc = core_create_pool(p->core, vcap, ecap, max_games, max_n, p->s1);
@@ -245,6 +261,7 @@ extern "C" int psi_pool_destroy(psi_pool* p)
     if (p->hctl) (void)hipHostFree(p->hctl);
     if (p->hread) (void)hipHostFree(p->hread);
     if (p->herr) (void)hipHostFree(p->herr);
+    if (p->hseq) (void)hipHostFree(p->hseq);
     delete p;
     return PSI_OK;
 }
@@ -283,10 +300,11 @@ extern "C" int psi_pool_step(psi_pool* p, int steps, int64_t* done_tickets, int 
     if (*p->herr) return fail(PSI_ERR_INVALID, "internal: jump rounds exhausted");
     std::vector<int> fin;
     for (int k = 0; k < p->slots; k++)
-        if (p->slot_ticket[k] >= 0 && p->hread[k].done) fin.push_back(k);
+        if (p->slot_ticket[k] >= 0 && p->hread[k].done && !p->hread[k].activate) fin.push_back(k);
     // 2. next batch, if any admitted game is unfinished
     if (p->running > (int)fin.size()) {
-        if (int rc = core_launch_steps(c, steps)) return rc;
+        if (int rc = p->program ? core_pool_launch_program(c, steps) : core_launch_steps(c, steps))
+            return rc;
     }
     // 3. while it runs: copy out finished games and free their ranges
     for (int k : fin) {

@@ -49,12 +49,21 @@ def test_pool_single_equals_solver(gpu):
     p.close()
 
 
+@pytest.fixture(params=[1, 0], ids=["program", "fixed"])
+def pool_mode(gpu, request):
+    """Pools step as the step program (admitted games join at a step
+    boundary) or with the fixed launch sequence; fixed at pool creation."""
+    gpu.si_tune(gpu.SIT_PROGRAM, request.param)
+    yield request.param
+    gpu.si_tune(gpu.SIT_PROGRAM, 1)
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
-def test_pool_mix_equals_alone(gpu, seed):
+def test_pool_mix_equals_alone(gpu, pool_mode, seed):
     """Requests arrive between steps into a pool too small to hold them all:
     they queue, are admitted as others finish, and each result equals the
     oracle's solve of that game alone."""
-    print(f"pool mix seed={seed}", flush=True)
+    print(f"pool mix seed={seed} program={pool_mode}", flush=True)
     reqs = _requests(gpu, seed, 40, 6000)
     bpv = gpu.pool_bytes_per_vertex(8192)
     p = gpu.Pool((bpv + 20) * 20000, edges_per_vertex=5.0, max_games=12, max_n=8192)
@@ -74,7 +83,7 @@ def test_pool_mix_equals_alone(gpu, seed):
     p.close()
 
 
-def test_pool_mix_equals_batch_and_solo(gpu):
+def test_pool_mix_equals_batch_and_solo(gpu, pool_mode):
     """The same games through the pool, as one batch handle and one at a time
     on the GPU: identical results (mid-size games, beyond the oracle's quick
     range at scale)."""
@@ -98,7 +107,7 @@ def test_pool_mix_equals_batch_and_solo(gpu):
     p.close()
 
 
-def test_pool_many_tiny_reuse_slots(gpu):
+def test_pool_many_tiny_reuse_slots(gpu, pool_mode):
     reqs = [(gpu.gen_random_game(500 + k, 2 + k % 50, 8, 1, 3), k % 7 == 0) for k in range(300)]
     p = gpu.Pool(8 << 20, max_games=8, max_n=64)
     tickets = [p.submit(g, reset=r) for g, r in reqs]
@@ -108,7 +117,7 @@ def test_pool_many_tiny_reuse_slots(gpu):
     p.close()
 
 
-def test_pool_round_cap(gpu):
+def test_pool_round_cap(gpu, pool_mode):
     g = gpu.gen_random_game(5, 3000, 100, 1, 4)
     r = oracle.solve(g, max_rounds=3)
     p = gpu.Pool(32 << 20, max_games=2, max_n=4096)

@@ -74,6 +74,21 @@ def test_program_any_graph_length(modes, steps):
     _same(_run(gpu, g, 1, steps_per_sync=steps), _run(gpu, g, 0))
 
 
+def test_program_recaptures_on_one_handle(modes):
+    """One handle solving again with other graph lengths (a new capture each
+    time) and after the fixed-sequence form: the same results every time."""
+    gpu = modes
+    g = gpu.gen_random_game(31, 20000, 3000, 1, 5)
+    r = oracle.solve(g)
+    s = gpu.Solver(g)
+    for program, steps in [(1, 4), (1, 1), (0, 4), (1, 3), (1, 4)]:
+        gpu.si_tune(gpu.SIT_PROGRAM, program)
+        s.solve(steps_per_sync=steps)
+        assert s.stats()[:2] == (r.iter, r.br), (program, steps)
+        assert np.array_equal(s.strategy(), r.strat) and np.array_equal(s.labels(), r.inf)
+    s.close()
+
+
 def test_program_batch(modes):
     gpu = modes
     sizes = [3, 17, 400, 2500, 10, 9000, 1, 64, 30000]
