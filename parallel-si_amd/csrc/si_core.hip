@@ -235,21 +235,24 @@ __device__ __forceinline__ void val_init_body(
         const long long v = base + threadIdx.x;
         int live = 0, jumping = 0;
         if (v < N) {
-            int g = gid[v];
+            // the vertex's words are loaded together with its game's control
+            // block (no load waits on another)
+            const int g = gid[v];
+            const int vi = vinfo[v];
+            int s = strat[v];
+            const int old = par[v];
+            const int sv = solved[v], lv = lab[v];
             live = !gc[g].done;
             if (live) {
-                int vi = vinfo[v];
-                int s = strat[v];
                 if (gc[g].reset_pending && owner_of(vi) == 1) {   // main.cpp:25-26
                     s = tgt[off[v]];
                     strat[v] = s;
                 }
                 // w = 1 if this vertex's link differs from the previous
                 // valuation's (the OR along a path says the path changed).
-                const int old = par[v];
                 const int first = gc[g].br == 0;
-                if (solved[v]) {
-                    const int code = term_code(lab[v] == 1 ? 1 : 2, 0);
+                if (sv) {
+                    const int code = term_code(lv == 1 ? 1 : 2, 0);
                     S0[v] = make_int2(code, (first | (old != code)) ? kChangedBit : 0);
                     par[v] = code;
                     if (trace) Z0[v] = 0;
@@ -443,23 +446,24 @@ __device__ __forceinline__ void label_body(
             const int2* SF = Sl + (size_t)level_slot(Fb) * N;
             const unsigned long long* ZF = Zl + (size_t)level_slot(Fb) * N;
             const long long v = vb + threadIdx.x;
-            int g = -1, l = 0, sol = 0, loc = 0;
+            int g = -1, l = 0, sol = 0, loc = 0, lold = 0;
             unsigned long long d = 0;
             bool live = false, ch = false;
-            if (v < N) {
+            int2 s = make_int2(0, 0);
+            if (v < N) {                          // loaded together (no load waits on another)
                 g = gid[v];
+                sol = solved[v];
+                lold = lab[v];
+                s = SF[v];
                 live = !gc[g].done;
             }
             if (live) {
-                sol = solved[v];
                 loc = (int)(v - gc[g].vbase);
-                const int lold = lab[v];
                 if (sol) {
                     l = lold;
                     key[v] = (l + 1) << 30;
                     dep[v] = 0;
                 } else {
-                    int2 s = SF[v];
                     const int kind = s.x < 0 ? term_kind(s.x) : -1;
                     if (kind == 0) {                   // reaches the sink: finite
                         l = 0;
