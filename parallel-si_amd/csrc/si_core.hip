@@ -731,17 +731,19 @@ __device__ __forceinline__ void switch_body(
         // at the last scan, which left it a maximum), so such a vertex compares
         // just those, in CSR order: the result is the full scan's.
         bool only_changed = false;
-        int g = -1;
+        int g = -1, c0 = -1, eb = 0, ee = 0;
         const int v = i < lim ? own[i] : -1;
         if (v >= 0) {
             g = gid[v];
+            c0 = strat[v];                        // loaded together with the game's control
+            eb = off[v];
+            ee = off[v + 1];
             live = !gc[g].done && (!loop_form || player == 1 || gc[g].outer);
             if (live && incremental && !(player == 1 && gc[g].reset_pending)) {
-                const int c0 = strat[v];
                 bool dirty = c0 >= 0 && ((cb[c0 >> 6] >> (c0 & 63)) & 1);
                 only_changed = !dirty;
-                const int e1 = off[v + 1];
-                for (int e0 = off[v]; e0 < e1 && !dirty; e0 += 8) {
+                const int e1 = ee;
+                for (int e0 = eb; e0 < e1 && !dirty; e0 += 8) {
                     const int cnt = min(8, e1 - e0);
                     int us[8];
 #pragma unroll
@@ -758,10 +760,10 @@ __device__ __forceinline__ void switch_body(
         // sequential in CSR order (the reference's tie-break).  The loop runs
         // for the whole wave, so that exact compares whose segment maxima tie
         // are done by all lanes together (coop_scan), one at a time.
-        int cur = live ? strat[v] : -1;
+        int cur = live ? c0 : -1;
         int kcur = live ? key_of(key, cur) : 0;
-        int e = live ? off[v] : 0;
-        const int e1 = live ? off[v + 1] : 0;
+        int e = live ? eb : 0;
+        const int e1 = live ? ee : 0;
         while (__any(e < e1)) {
             const int cnt = e < e1 ? min(kSwBatch, e1 - e) : 0;
             int us[kSwBatch], ks[kSwBatch];
