@@ -167,8 +167,10 @@ int psi_solve(psi_solver* s, int reset, int max_rounds, int steps_per_sync);
 int psi_solve_bf(psi_solver* s, int max_rounds, int steps_per_sync);
 int psi_get_bf_status(psi_solver* s, int game, int* status);   /* 1 = odd cycle */
 int psi_get_stats(psi_solver* s, int game, int* iter, int* br, int* capped);
-/* The same solve launched kernel by kernel with HIP events between launches
- * (no graph): device milliseconds and launch counts per kernel class. */
+/* Profiling (no reference counterpart; the reference times solve_si as a
+ * whole, gpumain.cpp:56-71).  The same solve launched kernel by kernel with
+ * HIP events between launches (no graph): device milliseconds and launch
+ * counts per kernel class. */
 #define PSI_K_VAL_INIT 0
 #define PSI_K_JUMP 1      /* pointer-jumping rounds that ran               */
 #define PSI_K_JUMP_IDLE 2 /* rounds launched after convergence (no-ops)    */
