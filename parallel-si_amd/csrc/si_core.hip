@@ -46,6 +46,7 @@
 namespace psi {
 
 constexpr int kBS = 256;
+constexpr int kRows = kBS / kWave;   // rows of 64 vertices in a block
 constexpr int kPriMask = 0x3FFFFFFF;
 // Jump levels 0..kLiftCap are kept as binary-lifting tables; later rounds
 // (only needed to cover long cycles) rotate through two more buffers.
@@ -324,48 +325,88 @@ __device__ __forceinline__ void jump_body(int r, long long N, int2* __restrict__
     const long long nbk = (lim + kBS - 1) / kBS;  // blocks below the bound
     const long long G = gridDim.x;
     const int lane = (int)(threadIdx.x & (kWave - 1));
+    const int wid = (int)(threadIdx.x / kWave);
     // This workgroup's blocks are b0, b0 + G, ...; their flags are read 64 at
     // a time (one per lane, every wave alike), so blocks with nothing to do
-    // cost no serial round trip.
+    // cost no serial round trip.  The blocks with work are dealt to the
+    // workgroup's waves in turn: a wave runs a whole block, the loads of its
+    // kRows rows in flight together, and the waves' blocks overlap (no
+    // barrier between them).
     for (long long b0 = blockIdx.x; b0 < nbk; b0 += (long long)kWave * G) {
         const long long bq = b0 + (long long)lane * G;
-        const bool on = bq < nbk && blive[bq] && bact_in[bq];
+        const long long bc = min(bq, nbk - 1);    // both flags loaded together
+        const int f_live = blive[bc], f_act = bact_in[bc];
+        const bool on = bq < nbk && f_live && f_act;
         if (threadIdx.x < kWave && bq < nbk && !on) bact_out[bq] = 0;
-        for (unsigned long long m = __ballot(on); m; m &= m - 1) {
+        unsigned long long m = __ballot(on);
+        for (int k = 0; m; m &= m - 1, k++) {
+            if ((k & (kRows - 1)) != wid) continue;
             const long long b = b0 + (long long)(__ffsll((long long)m) - 1) * G;
-            const long long base = b * kBS;
-            long long v = base + threadIdx.x;
-            bool res = false, act = false;
-            if (v < N) {
-                int2 s = Sin[v];
-                unsigned long long z = trace ? Zin[v] : 0;
-                if (s.x >= 0) {
-                    // The successor's record at level r, loaded together with its
-                    // block's last level; a block that stopped earlier (its
-                    // records final) is read again at that level.
-                    const int bl = blast[s.x / kBS];
-                    const size_t at_r = (size_t)level_slot(r) * N + s.x;
-                    int2 t = Sl[at_r];
-                    size_t at = at_r;
-                    if (bl < r) {
-                        at = (size_t)level_slot(bl) * N + s.x;
-                        t = Sl[at];
+            const long long base = b * kBS + lane;
+            if (trace) {
+                // trace digests: the same gathers, row by row, before the
+                // records below replace level r + 1
+#pragma unroll 1
+                for (int q = 0; q < kRows; q++) {
+                    const long long v = base + q * kWave;
+                    if (v >= N) continue;
+                    const int2 s0 = Sin[v];
+                    unsigned long long z = Zin[v];
+                    if (s0.x >= 0) {
+                        const int bz = min((int)blast[s0.x / kBS], r);
+                        z += Zl[(size_t)level_slot(bz) * N + s0.x];
                     }
-                    if (trace) z += Zl[at];
-                    s.x = t.x >= 0 ? t.x : term_code(term_kind(t.x), span + term_count(t.x));
-                    s.y = max(s.y & kPriMask, t.y & kPriMask) | ((s.y | t.y) & kChangedBit);
-                    res = s.x < 0;
+                    Zout[v] = z;
+                }
+            }
+            // Every load below is unconditional, with clamped indices (rows past
+            // N read vertex N - 1, non-jumping rows read vertex 0), so that a
+            // row's loads are all in flight before any result is waited for.
+            int2 s[kRows], t[kRows];
+            int bl[kRows];
+#pragma unroll
+            for (int q = 0; q < kRows; q++) {
+                const long long v = base + q * kWave;
+                s[q] = Sin[min(v, N - 1)];
+                if (v >= N) s[q].x = -1;
+            }
+            // The successor's record at level r, loaded together with its
+            // block's last level; a block that stopped earlier (its records
+            // final) is read again at that level.
+#pragma unroll
+            for (int q = 0; q < kRows; q++) {
+                const int x = max(s[q].x, 0);
+                bl[q] = blast[x / kBS];
+                t[q] = Sl[(size_t)level_slot(r) * N + x];
+            }
+#pragma unroll
+            for (int q = 0; q < kRows; q++) {
+                const bool again = s[q].x >= 0 && bl[q] < r;
+                if (__any(again)) {
+                    const int2 u = Sl[(size_t)level_slot(again ? bl[q] : r) * N + (again ? s[q].x : 0)];
+                    if (again) t[q] = u;
+                }
+            }
+            unsigned long long any = 0;
+#pragma unroll
+            for (int q = 0; q < kRows; q++) {
+                const long long v = base + q * kWave;
+                bool res = false, act = false;
+                if (s[q].x >= 0) {
+                    const int2 u = t[q];
+                    s[q].x = u.x >= 0 ? u.x : term_code(term_kind(u.x), span + term_count(u.x));
+                    s[q].y = max(s[q].y & kPriMask, u.y & kPriMask) | ((s[q].y | u.y) & kChangedBit);
+                    res = s[q].x < 0;
                     act = !res;
                 }
-                Sout[v] = s;
-                if (trace) Zout[v] = z;
+                if (v < N) Sout[v] = s[q];
+                const unsigned long long mr = __ballot(res), ma = __ballot(act);
+                my_res += (unsigned)__popcll(mr);
+                my_act += (unsigned)__popcll(ma);
+                any |= ma;
             }
-            unsigned long long mr = __ballot(res), ma = __ballot(act);
-            my_res += (unsigned)__popcll(mr);
-            my_act += (unsigned)__popcll(ma);
-            int any = __syncthreads_or(act);
-            if (threadIdx.x == 0) {
-                bact_out[b] = (uint8_t)any;
+            if (lane == 0) {
+                bact_out[b] = (uint8_t)(any != 0);
                 blast[b] = (uint8_t)(r + 1);
             }
         }
@@ -1003,6 +1044,17 @@ struct StepArgs {
     JumpCnt* jc;
 };
 
+// A pointer field of the arguments, read as a pointer to global memory:
+// loads and stores through it (after inlining, also through what is derived
+// from it) are then global instructions; a pointer read from memory is
+// otherwise generic, and flat instructions need 64-bit vector addresses and
+// wait on two counters.
+template <class T> __device__ __forceinline__ T* glb(T* const& field)
+{
+    using GlobalPtr = __attribute__((address_space(1))) T*;
+    return (T*)*reinterpret_cast<const GlobalPtr*>(&field);
+}
+
 // The arguments live in device memory (one pointer argument): each phase
 // loads the fields it uses, which keeps scalar registers free.
 __global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8, 8)))
@@ -1071,30 +1123,37 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
     const int ph = ph_s, r = r_s;
     switch (ph) {
     case PH_INIT:
-        val_init_body(a.N, a.vinfo, a.off, a.tgt, a.gid, a.gc, a.strat, a.solved, a.lab, a.par, a.Sl,
-                      a.Zl, a.jc, a.blive, a.bact, a.blast, a.trace, a.hi);
+        val_init_body(a.N, glb(a.vinfo), glb(a.off), glb(a.tgt), glb(a.gid), glb(a.gc), glb(a.strat),
+                      glb(a.solved), glb(a.lab), glb(a.par), glb(a.Sl), glb(a.Zl), glb(a.jc),
+                      glb(a.blive), glb(a.bact), glb(a.blast), a.trace, glb(a.hi));
         break;
     case PH_JUMP:
-        jump_body(r, a.N, a.Sl, a.Zl, pg->pslot[i & 3], a.blive, a.bact + (r & 1) * a.nb,
-                  a.bact + ((r + 1) & 1) * a.nb, a.blast, a.trace, a.hi, cnt);
+        jump_body(r, a.N, glb(a.Sl), glb(a.Zl), pg->pslot[i & 3], glb(a.blive),
+                  glb(a.bact) + (r & 1) * a.nb, glb(a.bact) + ((r + 1) & 1) * a.nb, glb(a.blast),
+                  a.trace, glb(a.hi), cnt);
         break;
     case PH_LABEL:
-        label_body(a.N, a.maxn, a.lmax, a.Sl, a.Zl, a.blast, a.gid, a.gc, a.strat, a.solved, a.lab,
-                   a.dep, a.key, a.dig, a.tr, a.trace_cap, 1, a.trace, a.jc, a.blive, a.hi, a.chg,
-                   a.chg0, 0);
+        label_body(a.N, a.maxn, a.lmax, glb(a.Sl), glb(a.Zl), glb(a.blast), glb(a.gid), glb(a.gc),
+                   glb(a.strat), glb(a.solved), glb(a.lab), glb(a.dep), glb(a.key), glb(a.dig),
+                   glb(a.tr), a.trace_cap, 1, a.trace, glb(a.jc), glb(a.blive), glb(a.hi), glb(a.chg),
+                   glb(a.chg0), 0);
         break;
     case PH_SW1:
     case PH_SW0: {                                // one inlined copy of the switch
         const int pl = ph == PH_SW1 ? 1 : 0;
-        switch_body(pl, a.N, pl ? a.own1 : a.own0, pl ? a.own_n1 : a.own_n0, a.vinfo, a.off, a.tgt,
-                    a.gid, a.gc, a.strat, a.key, a.dep, a.par, a.Sl, a.jc, 1, a.flag, nullptr, a.hi,
-                    a.chg, a.chg0);
+        switch_body(pl, a.N, pl ? glb(a.own1) : glb(a.own0), pl ? a.own_n1 : a.own_n0, glb(a.vinfo),
+                    glb(a.off), glb(a.tgt), glb(a.gid), glb(a.gc), glb(a.strat), glb(a.key),
+                    glb(a.dep), glb(a.par), glb(a.Sl), glb(a.jc), 1, glb(a.flag), nullptr, glb(a.hi),
+                    glb(a.chg), glb(a.chg0));
         break;
     }
-    case PH_CTL1: ctl1_body(a.G, a.gc, pg); break;
-    case PH_MARK0: mark0_body(a.N, a.gid, a.gc, a.lab, a.solved, a.hi, a.vinfo, a.strat, a.succ); break;
-    case PH_CTL0: ctl0_body(a.G, a.gc, pg); break;
-    case PH_ACT: act_body(a.G, a.gc, pg, a.flag + 2); break;
+    case PH_CTL1: ctl1_body(a.G, glb(a.gc), pg); break;
+    case PH_MARK0:
+        mark0_body(a.N, glb(a.gid), glb(a.gc), glb(a.lab), glb(a.solved), glb(a.hi), glb(a.vinfo),
+                   glb(a.strat), a.succ ? glb(a.succ) : nullptr);
+        break;
+    case PH_CTL0: ctl0_body(a.G, glb(a.gc), pg); break;
+    case PH_ACT: act_body(a.G, glb(a.gc), pg, glb(a.flag) + 2); break;
     default: break;
     }
 }
