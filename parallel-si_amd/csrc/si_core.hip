@@ -47,6 +47,7 @@ namespace psi {
 
 constexpr int kBS = 256;
 constexpr int kRows = kBS / kWave;   // rows of 64 vertices in a block
+constexpr int kInitRows = 2;         // rows of a block initialised together
 constexpr int kPriMask = 0x3FFFFFFF;
 // Jump levels 0..kLiftCap are kept as binary-lifting tables; later rounds
 // (only needed to cover long cycles) rotate through two more buffers.
@@ -209,6 +210,15 @@ __global__ void k_reset_opp(long long N, const int* __restrict__ vinfo, const in
     if (owner_of(vinfo[v]) == 1) strat[v] = tgt[off[v]];
 }
 
+// Uses the values here (waiting for them): the loads that produced them stay
+// above this point, in flight together, rather than being moved by the
+// compiler next to their first use (under a branch, where each is waited
+// for on its own).
+__device__ __forceinline__ void issued(int a, int b, int c, int d, int e, int f)
+{
+    asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d), "v"(e), "v"(f));
+}
+
 // Level-0 jump records.  S = (next or terminal code, max priority, count, -).
 // Terminal codes: -1 sink, -2 solved +inf, -3 solved -inf, -4 finished game.
 __device__ __forceinline__ void val_init_body(
@@ -230,54 +240,90 @@ __device__ __forceinline__ void val_init_body(
     // Vertices at or above *hi belong to no unfinished game; no kernel of the
     // step visits their blocks.
     const long long lim = min(N, (long long)*hi);
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long base = (long long)blockIdx.x * blockDim.x; base < lim; base += stride) {
-        const long long b = base / kBS;
-        const long long v = base + threadIdx.x;
-        int live = 0, jumping = 0;
-        if (v < N) {
-            // the vertex's words are loaded together with its game's control
-            // block (no load waits on another)
-            const int g = gid[v];
-            const int vi = vinfo[v];
-            int s = strat[v];
-            const int old = par[v];
-            const int sv = solved[v], lv = lab[v];
-            live = !gc[g].done;
-            if (live) {
-                if (gc[g].reset_pending && owner_of(vi) == 1) {   // main.cpp:25-26
-                    s = tgt[off[v]];
-                    strat[v] = s;
+    const long long nbk = (lim + kBS - 1) / kBS;
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    const long long nwv = (long long)gridDim.x * (kBS / kWave);
+    // A wave takes a whole block, kInitRows rows of 64 vertices at a time
+    // with their loads in flight together: the loads are unconditional, with
+    // clamped indices (a load under a branch is waited for where the branch
+    // joins), and no barrier separates the waves.
+    for (long long b = (long long)blockIdx.x * (kBS / kWave) + threadIdx.x / kWave; b < nbk; b += nwv) {
+        const long long base = b * kBS + lane;
+        unsigned long long mlive[kRows];
+        unsigned long long any = 0, anyj = 0;
+#pragma unroll
+        for (int h = 0; h < kRows; h += kInitRows) {
+            int g[kInitRows], vi[kInitRows], s[kInitRows], old[kInitRows], sv[kInitRows], lv[kInitRows];
+#pragma unroll
+            for (int q = 0; q < kInitRows; q++) {
+                const long long vc = min(base + (h + q) * kWave, N - 1);
+                g[q] = gid[vc];
+                vi[q] = vinfo[vc];
+                s[q] = strat[vc];
+                old[q] = par[vc];
+                sv[q] = solved[vc];
+                lv[q] = lab[vc];
+            }
+#pragma unroll
+            for (int q = 0; q < kInitRows; q++) issued(g[q], vi[q], s[q], old[q], sv[q], lv[q]);
+            // then the games' control words
+            int done[kInitRows], rp[kInitRows], br[kInitRows];
+#pragma unroll
+            for (int q = 0; q < kInitRows; q++) {
+                done[q] = gc[g[q]].done;
+                rp[q] = gc[g[q]].reset_pending;
+                br[q] = gc[g[q]].br;
+            }
+#pragma unroll
+            for (int q = 0; q < kInitRows; q++) issued(done[q], rp[q], br[q], 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < kInitRows; q++) {
+                const long long v = base + (h + q) * kWave;
+                const bool live = v < N && !done[q];
+                bool jumping = false;
+                if (live) {
+                    int sq = s[q];
+                    if (rp[q] && owner_of(vi[q]) == 1) {   // main.cpp:25-26
+                        sq = tgt[off[v]];
+                        strat[v] = sq;
+                    }
+                    // w = 1 if this vertex's link differs from the previous
+                    // valuation's (the OR along a path says the path changed).
+                    const int first = br[q] == 0;
+                    if (sv[q]) {
+                        const int code = term_code(lv[q] == 1 ? 1 : 2, 0);
+                        S0[v] = make_int2(code, (first | (old[q] != code)) ? kChangedBit : 0);
+                        par[v] = code;
+                        if (trace) Z0[v] = 0;
+                    } else {
+                        par[v] = sq;
+                        S0[v] = make_int2(sq >= 0 ? sq : term_code(0, 1),
+                                          pri_of(vi[q]) | ((first | (old[q] != sq)) ? kChangedBit : 0));
+                        if (trace) Z0[v] = splitmix64((unsigned long long)(uint32_t)pri_of(vi[q]));
+                        jumping = sq >= 0;
+                    }
                 }
-                // w = 1 if this vertex's link differs from the previous
-                // valuation's (the OR along a path says the path changed).
-                const int first = gc[g].br == 0;
-                if (sv) {
-                    const int code = term_code(lv == 1 ? 1 : 2, 0);
-                    S0[v] = make_int2(code, (first | (old != code)) ? kChangedBit : 0);
-                    par[v] = code;
-                    if (trace) Z0[v] = 0;
-                } else {
-                    par[v] = s;
-                    S0[v] = make_int2(s >= 0 ? s : term_code(0, 1),
-                                      pri_of(vi) | ((first | (old != s)) ? kChangedBit : 0));
-                    if (trace) Z0[v] = splitmix64((unsigned long long)(uint32_t)pri_of(vi));
-                    jumping = s >= 0;
-                }
+                mlive[h + q] = __ballot(live);
+                any |= mlive[h + q];
+                anyj |= __ballot(jumping);
             }
         }
         // Blocks that hold no vertex of an unfinished game are skipped by the
         // jump and label kernels of this valuation; round 0 visits only
-        // blocks with a vertex that still has to jump.
-        int any = __syncthreads_or(live);
-        int anyj = __syncthreads_or(jumping);
-        // Other vertices of a visited block hold a terminal record, so that
-        // the jump rounds never follow what a finished game, or a free range
-        // of a serving pool, left in the level buffers.
-        if (any && !live && v < N) S0[v] = make_int2(term_code(3, 0), 0);
-        if (threadIdx.x == 0) {
-            blive[b] = (uint8_t)any;
-            bact0[b] = (uint8_t)anyj;
+        // blocks with a vertex that still has to jump.  Other vertices of a
+        // visited block hold a terminal record, so that the jump rounds never
+        // follow what a finished game, or a free range of a serving pool, left
+        // in the level buffers.
+        if (any) {
+#pragma unroll
+            for (int q = 0; q < kRows; q++) {
+                const long long v = base + q * kWave;
+                if (v < N && !((mlive[q] >> lane) & 1)) S0[v] = make_int2(term_code(3, 0), 0);
+            }
+        }
+        if (lane == 0) {
+            blive[b] = (uint8_t)(any != 0);
+            bact0[b] = (uint8_t)(anyj != 0);
             blast[b] = 0;
         }
     }
@@ -476,87 +522,131 @@ __device__ __forceinline__ void label_body(
     const long long nbk = (lim + kBS - 1) / kBS;
     const long long G = gridDim.x;
     const int lane = (int)(threadIdx.x & (kWave - 1));
-    // blocks b0, b0 + G, ... with their live flags read 64 at a time
+    const int wid = (int)(threadIdx.x / kWave);
+    // Blocks b0, b0 + G, ... with their live flags read 64 at a time; the
+    // live ones are dealt to the workgroup's waves in turn, a wave taking a
+    // whole block, kInitRows rows at a time with their loads in flight
+    // together (unconditional, clamped indices; no barrier between waves).
     for (long long b0 = blockIdx.x; b0 < nbk; b0 += (long long)kWave * G) {
         const long long bq = b0 + (long long)lane * G;
-        for (unsigned long long m = __ballot(bq < nbk && blive[bq]); m; m &= m - 1) {
+        const bool bl_live = blive[min(bq, nbk - 1)] != 0;
+        unsigned long long m = __ballot(bq < nbk && bl_live);
+        for (int k = 0; m; m &= m - 1, k++) {
+            if ((k & (kRows - 1)) != wid) continue;
             const long long b = b0 + (long long)(__ffsll((long long)m) - 1) * G;
-            const long long vb = b * kBS;
             // this block's final level (it may have stopped jumping earlier)
             const int Fb = min(F, (int)blast[b]);
             const int2* SF = Sl + (size_t)level_slot(Fb) * N;
             const unsigned long long* ZF = Zl + (size_t)level_slot(Fb) * N;
-            const long long v = vb + threadIdx.x;
-            int g = -1, l = 0, sol = 0, loc = 0, lold = 0;
-            unsigned long long d = 0;
-            bool live = false, ch = false;
-            int2 s = make_int2(0, 0);
-            if (v < N) {                          // loaded together (no load waits on another)
-                g = gid[v];
-                sol = solved[v];
-                lold = lab[v];
-                s = SF[v];
-                live = !gc[g].done;
-            }
-            if (live) {
-                loc = (int)(v - gc[g].vbase);
-                if (sol) {
-                    l = lold;
-                    key[v] = (l + 1) << 30;
-                    dep[v] = 0;
-                } else {
-                    const int kind = s.x < 0 ? term_kind(s.x) : -1;
-                    if (kind == 0) {                   // reaches the sink: finite
-                        l = 0;
-                        dep[v] = term_count(s.x);
-                        key[v] = (1 << 30) | (s.y & kPriMask);
-                        ch = (s.y & kChangedBit) != 0; // a link on the path changed
-                        if (trace) d = ZF[v];
-                    } else {
-                        if (kind == 1) l = 1;
-                        else if (kind == 2) l = -1;
-                        else if (kind < 0) {           // largest priority on the cycle
-                            const int Fy = min(F, (int)blast[s.x / kBS]);
-                            l = (Sl[(size_t)level_slot(Fy) * N + s.x].y & 1) ? -1 : 1;
-                            // solve_bf: the relaxation never closes a cycle (such
-                            // vertices stay infinite, cpu_bf.cpp:156-160); the
-                            // first best response stops on an odd one.
-                            if (gc[g].bf) {
-                                if (gc[g].bf_init) l = 1;
-                                else if (l == -1) gc[g].err = 1;
-                            }
-                        } else {
-                            l = 1;                     // kind 3 (finished game): never on a live path
-                        }
-                        key[v] = (l + 1) << 30;
-                        dep[v] = 0;
-                    }
-                    ch = ch || l != lold;
-                    lab[v] = (int8_t)l;
-                    if (loop_form && l == -1) solved[v] = 1;
+#pragma unroll
+            for (int h = 0; h < kRows; h += kInitRows) {
+                int g[kInitRows], sol[kInitRows], lold[kInitRows];
+                int2 s[kInitRows];
+                unsigned long long c0w[kInitRows];
+#pragma unroll
+                for (int q = 0; q < kInitRows; q++) {
+                    const long long v = b * kBS + (h + q) * kWave + lane;
+                    const long long vc = min(v, N - 1);
+                    g[q] = gid[vc];
+                    sol[q] = solved[vc];
+                    lold[q] = lab[vc];
+                    s[q] = SF[vc];
+                    c0w[q] = chg0[v >> 6];            // the row's word (every lane alike)
                 }
-                if (trace) dig[v] = d;
-            }
-            // Valuation changed since the previous one (one word per wave), and
-            // since the last switch(0) pass of the vertex's game: that word is
-            // cleared for games that took the outer step in the previous step.
-            const unsigned long long cm = __ballot(ch);
-            const int restart = live ? gc[g].clear0 : 0;
-            const unsigned long long clr = __ballot(restart != 0);
-            if ((threadIdx.x & (kWave - 1)) == 0) {
-                chg[v >> 6] = cm;
-                chg0[v >> 6] = (chg0[v >> 6] & ~clr) | cm;
-            }
-            if (trace) {
-                int br = live ? gc[g].br : 0;
-                bool rec = live && br < trace_cap;
-                unsigned long long* base = rec ? tr + ((size_t)g * trace_cap + br) * 4 : nullptr;
-                int sv = rec ? strat[v] : -1;
-                if (sv >= 0) sv -= gc[g].vbase;              // local ids, as a game alone
-                wave_add_u64(rec ? base + 0 : nullptr, rec ? hmix_i32(loc, sv) : 0);
-                wave_add_u64(rec ? base + 1 : nullptr, rec ? hmix_i32(loc, sol) : 0);
-                wave_add_u64(rec ? base + 2 : nullptr, rec ? hmix_i32(loc, l) : 0);
-                wave_add_u64(rec ? base + 3 : nullptr, rec ? hmix_u64(loc, d) : 0);
+#pragma unroll
+                for (int q = 0; q < kInitRows; q++) issued(g[q], sol[q], lold[q], s[q].x, s[q].y, (int)c0w[q]);
+                // then the games' control words
+                int done[kInitRows], vbase[kInitRows], clr0[kInitRows], bf[kInitRows], bfi[kInitRows];
+#pragma unroll
+                for (int q = 0; q < kInitRows; q++) {
+                    done[q] = gc[g[q]].done;
+                    vbase[q] = gc[g[q]].vbase;
+                    clr0[q] = gc[g[q]].clear0;
+                    bf[q] = gc[g[q]].bf;
+                    bfi[q] = gc[g[q]].bf_init;
+                }
+#pragma unroll
+                for (int q = 0; q < kInitRows; q++) issued(done[q], vbase[q], clr0[q], bf[q], bfi[q], 0);
+#pragma unroll
+                for (int q = 0; q < kInitRows; q++) {
+                    const long long v = b * kBS + (h + q) * kWave + lane;
+                    const bool live = v < N && !done[q];
+                    const int kind = s[q].x < 0 ? term_kind(s[q].x) : -1;
+                    // A cycle-bound vertex: its 2^F-th successor is on the
+                    // cycle, whose largest priority is that successor's
+                    // window maximum at its block's final level (loads for
+                    // the whole row, other lanes reading their own vertex).
+                    int cyc_pri = 0;
+                    const bool cyc = live && !sol[q] && kind < 0;
+                    if (__any(cyc)) {
+                        const int x = cyc ? s[q].x : (int)min(v, N - 1);
+                        const int Fy = min(F, (int)blast[x / kBS]);
+                        cyc_pri = Sl[(size_t)level_slot(Fy) * N + x].y;
+                    }
+                    int l = 0, loc = 0;
+                    unsigned long long d = 0;
+                    bool ch = false;
+                    if (live) {
+                        loc = (int)(v - vbase[q]);
+                        if (sol[q]) {
+                            l = lold[q];
+                            key[v] = (l + 1) << 30;
+                            dep[v] = 0;
+                        } else {
+                            if (kind == 0) {                   // reaches the sink: finite
+                                l = 0;
+                                dep[v] = term_count(s[q].x);
+                                key[v] = (1 << 30) | (s[q].y & kPriMask);
+                                ch = (s[q].y & kChangedBit) != 0; // a link on the path changed
+                                if (trace) d = ZF[v];
+                            } else {
+                                if (kind == 1) l = 1;
+                                else if (kind == 2) l = -1;
+                                else if (kind < 0) {           // largest priority on the cycle
+                                    l = (cyc_pri & 1) ? -1 : 1;
+                                    // solve_bf: the relaxation never closes a cycle (such
+                                    // vertices stay infinite, cpu_bf.cpp:156-160); the
+                                    // first best response stops on an odd one.
+                                    if (bf[q]) {
+                                        if (bfi[q]) l = 1;
+                                        else if (l == -1) gc[g[q]].err = 1;
+                                    }
+                                } else {
+                                    l = 1;                     // kind 3 (finished game): never on a live path
+                                }
+                                key[v] = (l + 1) << 30;
+                                dep[v] = 0;
+                            }
+                            ch = ch || l != lold[q];
+                            lab[v] = (int8_t)l;
+                            if (loop_form && l == -1) solved[v] = 1;
+                        }
+                        if (trace) dig[v] = d;
+                    }
+                    // Valuation changed since the previous one (one word per
+                    // row), and since the last switch(0) pass of the vertex's
+                    // game: that word is cleared for games that took the outer
+                    // step in the previous step.
+                    const unsigned long long cm = __ballot(ch);
+                    const unsigned long long clr = __ballot(live && clr0[q] != 0);
+                    if (lane == 0) {
+                        chg[v >> 6] = cm;
+                        chg0[v >> 6] = (c0w[q] & ~clr) | cm;
+                    }
+                    if (trace) {
+                        const int gq = live ? g[q] : -1;
+                        int br = live ? gc[gq].br : 0;
+                        bool rec = live && br < trace_cap;
+                        unsigned long long* base = rec ? tr + ((size_t)gq * trace_cap + br) * 4 : nullptr;
+                        int sv = rec ? strat[v] : -1;
+                        if (sv >= 0) sv -= vbase[q];              // local ids, as a game alone
+                        const int so = live ? sol[q] : 0;
+                        wave_add_u64(rec ? base + 0 : nullptr, rec ? hmix_i32(loc, sv) : 0);
+                        wave_add_u64(rec ? base + 1 : nullptr, rec ? hmix_i32(loc, so) : 0);
+                        wave_add_u64(rec ? base + 2 : nullptr, rec ? hmix_i32(loc, l) : 0);
+                        wave_add_u64(rec ? base + 3 : nullptr, rec ? hmix_u64(loc, d) : 0);
+                    }
+                }
             }
         }
     }
@@ -1052,7 +1142,7 @@ struct StepArgs {
 template <class T> __device__ __forceinline__ T* glb(T* const& field)
 {
     using GlobalPtr = __attribute__((address_space(1))) T*;
-    return (T*)*reinterpret_cast<const GlobalPtr*>(&field);
+    return (T*)__builtin_bit_cast(GlobalPtr, field);
 }
 
 // The arguments live in device memory (one pointer argument): each phase
