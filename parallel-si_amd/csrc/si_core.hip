@@ -1145,6 +1145,14 @@ template <class T> __device__ __forceinline__ T* glb(T* const& field)
     return (T*)__builtin_bit_cast(GlobalPtr, field);
 }
 
+// the head of StepProg as 32-bit words: st[2], any_outer, live, host_seq,
+// dev_seq[2], (padding), psum[2]
+constexpr int kProgHeadWords = 12;
+static_assert(offsetof(StepProg, any_outer) == 8 && offsetof(StepProg, live) == 12 &&
+                  offsetof(StepProg, host_seq) == 16 && offsetof(StepProg, dev_seq) == 20 &&
+                  offsetof(StepProg, psum) == 32 && offsetof(StepProg, pslot) == 4 * kProgHeadWords,
+              "StepProg head layout");
+
 // The arguments live in device memory (one pointer argument): each phase
 // loads the fields it uses, which keeps scalar registers free.
 __global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8, 8)))
@@ -1155,17 +1163,29 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
     __shared__ unsigned int cnt[2];
     if (threadIdx.x < kWave) {                    // the first wave decides
         const int lane = threadIdx.x;
-        // Launch i-1's phase, round summary and slot row, loaded together.
-        const int prev = pg->st[(i + 1) & 1];
-        const unsigned long long sm = pg->psum[(i + 1) & 1];
+        // Every word launch i-1 left (phase, round summary, control words)
+        // and its slot row, in two vector loads issued together: lane k < 12
+        // holds word k of the state's head, every lane its slot word.
+        const int hw = reinterpret_cast<const int*>(pg)[min(lane, kProgHeadWords - 1)];
         const unsigned long long x = pg->pslot[(i + 3) & 3][lane];
+        const int maxn = a.maxn, lmax = a.lmax;
+        // all these loads issued before any is waited for
+        asm volatile("" ::"v"(hw), "v"(x), "s"(maxn), "s"(lmax));
+        const int p = (i + 1) & 1;
+        // lane k's word as a scalar (the same in every lane)
+        auto head = [&](int k) { return __builtin_amdgcn_readlane(hw, k); };
+        const int prev = head(p);                                         // st[p]
+        const int any_outer = head(2), live = head(3);
+        const int hseq = head(4), dseq = head(5 + p);                     // host_seq, dev_seq[p]
+        const unsigned long long sm =
+            ((unsigned long long)(unsigned)head(9 + 2 * p) << 32) | (unsigned)head(8 + 2 * p);   // psum[p]
         int ph = prev & 0xff, r = prev >> 8;
         switch (ph) {
         case PH_INIT: ph = PH_JUMP; r = 0; break;
         case PH_JUMP: r++; break;
         case PH_LABEL: ph = PH_SW1; break;
         case PH_SW1: ph = PH_CTL1; break;
-        case PH_CTL1: ph = pg->any_outer ? PH_MARK0 : PH_INIT; break;
+        case PH_CTL1: ph = any_outer ? PH_MARK0 : PH_INIT; break;
         case PH_MARK0: ph = PH_SW0; break;
         case PH_SW0: ph = PH_CTL0; break;
         case PH_DONE: break;
@@ -1173,19 +1193,17 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         }
         // at a step boundary: admitted games first (a serving pool), then
         // the next step if any game is live
-        const int hseq = pg->host_seq;
-        const int dseq = pg->dev_seq[(i + 1) & 1];
         if ((ph == PH_INIT || ph == PH_DONE) && hseq != dseq) ph = PH_ACT;
         if (ph == PH_INIT) {
             r = 0;
-            if (pg->live <= 0) ph = PH_DONE;
+            if (live <= 0) ph = PH_DONE;
         }
         if (ph == PH_JUMP) {
             // round r's decision from round r-1 (launch i-1); a round at lmax
             // would mean the valuation needs more rounds than it has levels
             int stop, cyc;
-            int run = decide_wave(r, r >= 1 ? sm : 0ull, x, a.maxn, &stop, &cyc);
-            if (run && r >= a.lmax) {
+            int run = decide_wave(r, r >= 1 ? sm : 0ull, x, maxn, &stop, &cyc);
+            if (run && r >= lmax) {
                 if (blockIdx.x == 0 && lane == 0) a.jc->err = 1;
                 run = 0;
             }
