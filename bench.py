@@ -471,8 +471,10 @@ def main():
     # writes every vertex's 8-byte record (successor or terminal code and
     # count, largest priority) and gathers the 8-byte record of each
     # still-jumping vertex's successor.  Timed on k_jump, launched round by
-    # round; the timed solves run the same round code as the step program's
-    # jump phase (program_avg_launch_us: its launches, timed the same way).
+    # round, by each kernel's own start and end timestamps (as rocprofv3's
+    # kernel trace reads them); the timed solves run the same round code as
+    # the step program's jump phase (program_avg_launch_us: its launches,
+    # timed by stream events around each, so with the dispatch gap).
     jump_ms, jump_launches = prof["jump"]
     gathers = prof["gathers"][1]
     jump_bytes = (16.0 * g.n * jump_launches + 8.0 * gathers) / max(jump_launches, 1)

@@ -170,7 +170,9 @@ int psi_get_stats(psi_solver* s, int game, int* iter, int* br, int* capped);
 /* Profiling (no reference counterpart; the reference times solve_si as a
  * whole, gpumain.cpp:56-71).  The same solve launched kernel by kernel with
  * HIP events between launches (no graph): device milliseconds and launch
- * counts per kernel class. */
+ * counts per kernel class.  Jump rounds are timed by their kernels' own start
+ * and end (events attached to the dispatch, as a profiler reads them); the
+ * other classes by the events around each launch. */
 #define PSI_K_VAL_INIT 0
 #define PSI_K_JUMP 1      /* pointer-jumping rounds that ran               */
 #define PSI_K_JUMP_IDLE 2 /* rounds launched after convergence (no-ops)    */

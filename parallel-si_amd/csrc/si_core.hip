@@ -36,6 +36,8 @@
 // captured in a hipGraph) without any host round trip.
 #include "si_core.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -1847,6 +1849,18 @@ static void launch_jump(Core& c, int r, int maxn)
                            c.bact + ((r + 1) & 1) * nb, c.blast, c.trace, c.flag + 2);
 }
 
+// The same launch with events that take the kernel's own start and end
+// times (the dispatch packet's timestamps, as a profiler reads them), not
+// the stream's position around it.
+static void launch_jump_timed(Core& c, int r, int maxn, hipEvent_t start, hipEvent_t stop)
+{
+    const long long nb = nblocks(c.N, kBS);
+    hipExtLaunchKernelGGL(k_jump, dim3((unsigned)std::min(nblocks(c.N, kBS), jump_grid_cap())), dim3(kBS),
+                          0, c.s, start, stop, 0, r, c.N, maxn, c.Sl, c.Zl, c.jc, (const uint8_t*)c.blive,
+                          (const uint8_t*)(c.bact + (r & 1) * nb), c.bact + ((r + 1) & 1) * nb,
+                          c.blast, (int)c.trace, (const int*)(c.flag + 2));
+}
+
 static void launch_label(Core& c, int maxn, bool loop_form)
 {
     k_label<<<SGRID(c.N)>>>(c.N, maxn, c.lmax, c.Sl, c.Zl, c.blast, c.gid, c.gctl, c.strat,
@@ -1964,8 +1978,9 @@ int core_solve_profiled(Core& c, double* times, long long* counts)
     }
     int maxn = maxn_of(c);
     const int nev = c.lmax + 8;
-    std::vector<hipEvent_t> ev(nev);
+    std::vector<hipEvent_t> ev(nev), jev(2 * c.lmax);   // jev: each round's kernel start and end
     for (auto& e : ev) PSI_HIP(hipEventCreate(&e));
+    for (auto& e : jev) PSI_HIP(hipEventCreate(&e));
     auto span = [&](int a, int b) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, ev[a], ev[b]);
@@ -1986,7 +2001,7 @@ int core_solve_profiled(Core& c, double* times, long long* counts)
         launch_val_init(c);
         PSI_HIP(hipEventRecord(ev[k++], c.s));
         for (int r = 0; r < c.lmax; r++) {
-            launch_jump(c, r, maxn);
+            launch_jump_timed(c, r, maxn, jev[2 * r], jev[2 * r + 1]);
             PSI_HIP(hipEventRecord(ev[k++], c.s));
         }
         launch_label(c, maxn, true);
@@ -2012,7 +2027,9 @@ int core_solve_profiled(Core& c, double* times, long long* counts)
         unsigned long long prev_act = 0;
         for (int r = 0; r < c.lmax; r++, e++) {
             int cls = r <= jc.last ? PSI_K_JUMP : PSI_K_JUMP_IDLE;
-            times[cls] += span(e, e + 1);
+            float kms = 0;
+            (void)hipEventElapsedTime(&kms, jev[2 * r], jev[2 * r + 1]);
+            times[cls] += kms;
             counts[cls]++;
             unsigned long long res = 0, act = 0;
             if (r < kMaxRounds)
@@ -2035,6 +2052,7 @@ int core_solve_profiled(Core& c, double* times, long long* counts)
         if (done) break;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
+    for (auto& e : jev) (void)hipEventDestroy(e);
     unsigned long long h_ties[3] = {0, 0, 0};
     (void)hipMemcpyAsync(h_ties, ties, 24, hipMemcpyDeviceToHost, c.s);
     (void)hipStreamSynchronize(c.s);
