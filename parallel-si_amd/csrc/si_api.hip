@@ -167,6 +167,12 @@ extern "C" int psi_si_tune(int param, int value)
         g_si_tune.program = value;
         return PSI_OK;
     }
+    if (param == PSI_SIT_DOUBLE_ROUNDS) {
+        if (value < 0 || value > 16)
+            return fail(PSI_ERR_INVALID, "psi_si_tune: double rounds shift is 0..16");
+        g_si_tune.dshift = value;
+        return PSI_OK;
+    }
     if (value < 64 || value > (1 << 20)) return fail(PSI_ERR_INVALID, "psi_si_tune: value out of range");
     switch (param) {
     case PSI_SIT_PHASE_GRID: g_si_tune.phase_grid = (unsigned)value; return PSI_OK;

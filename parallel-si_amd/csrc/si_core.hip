@@ -468,6 +468,152 @@ __device__ __forceinline__ void jump_body(int r, long long N, int2* __restrict__
         atomicAdd(&slot_row[blockIdx.x & (kSlots - 1)], ((unsigned long long)cnt_res << 32) | cnt_act);
 }
 
+// A record extended by the record of its end vertex: s spans `span` vertices
+// and has not ended.
+__device__ __forceinline__ int2 join_rec(int2 s, int2 t, int span)
+{
+    int2 o;
+    o.x = t.x >= 0 ? t.x : term_code(term_kind(t.x), span + term_count(t.x));
+    o.y = max(s.y & kPriMask, t.y & kPriMask) | ((s.y | t.y) & kChangedBit);
+    return o;
+}
+
+// Two rounds in one launch, for sparse rounds (few vertices still jumping),
+// where the launch, not the gathers, is the cost: from level r, level r + 1
+// = s o rec(x1) and level r + 2 = (level r + 1) o rec(x2) o rec(x3), every
+// gathered record at level r (written by an earlier launch; x1, x2, x3 the
+// ends of the spans so far), with the same re-read at a stopped block's
+// level as one round.  Needs level r + 2 in a different buffer from level r
+// (r <= kLiftCap).  The counts are round r + 1's (resolved at level r + 2,
+// still jumping after it): a stop seen there is a stop at r or r + 1, and
+// only whether there was one is ever used.
+__device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict__ Sl,
+                                           unsigned long long* __restrict__ Zl,
+                                           unsigned long long* __restrict__ slot_row,
+                                           const uint8_t* __restrict__ blive,
+                                           const uint8_t* __restrict__ bact_in,
+                                           uint8_t* __restrict__ bact_out,
+                                           uint8_t* __restrict__ blast, int trace,
+                                           const int* __restrict__ hi, unsigned int* cnt)
+{
+    unsigned int& cnt_res = cnt[0];
+    unsigned int& cnt_act = cnt[1];
+    const int2* Sin = Sl + (size_t)level_slot(r) * N;
+    int2* Sout1 = Sl + (size_t)level_slot(r + 1) * N;
+    int2* Sout2 = Sl + (size_t)level_slot(r + 2) * N;
+    const int span = 1 << r;
+    const unsigned long long* Zin = Zl + (size_t)level_slot(r) * N;
+    unsigned long long* Zout1 = Zl + (size_t)level_slot(r + 1) * N;
+    unsigned long long* Zout2 = Zl + (size_t)level_slot(r + 2) * N;
+    unsigned my_res = 0, my_act = 0;
+    const long long lim = min(N, (long long)*hi);
+    const long long nbk = (lim + kBS - 1) / kBS;
+    const long long G = gridDim.x;
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    const int wid = (int)(threadIdx.x / kWave);
+    for (long long b0 = blockIdx.x; b0 < nbk; b0 += (long long)kWave * G) {
+        const long long bq = b0 + (long long)lane * G;
+        const long long bc = min(bq, nbk - 1);
+        const int f_live = blive[bc], f_act = bact_in[bc];
+        const bool on = bq < nbk && f_live && f_act;
+        if (threadIdx.x < kWave && bq < nbk && !on) bact_out[bq] = 0;
+        unsigned long long m = __ballot(on);
+        for (int k = 0; m; m &= m - 1, k++) {
+            if ((k & (kRows - 1)) != wid) continue;
+            const long long b = b0 + (long long)(__ffsll((long long)m) - 1) * G;
+            const long long base = b * kBS + lane;
+            if (trace) {
+                // trace digests, row by row, from level r alone (as below)
+#pragma unroll 1
+                for (int q = 0; q < kRows; q++) {
+                    const long long v = base + q * kWave;
+                    if (v >= N) continue;
+                    int2 c = Sin[v];
+                    unsigned long long z = Zin[v];
+#pragma unroll 1
+                    for (int step = 0; step < 3 && c.x >= 0; step++) {
+                        const int bz = min((int)blast[c.x / kBS], r);
+                        const size_t at = (size_t)level_slot(bz) * N + c.x;
+                        z += Zl[at];
+                        c = join_rec(c, Sl[at], (step == 2 ? 3 : step + 1) * span);
+                        if (step == 0) Zout1[v] = z;
+                    }
+                    if (Sin[v].x < 0) Zout1[v] = z;
+                    Zout2[v] = z;
+                }
+            }
+            // The rows' records, then three steps of gathers for all rows at
+            // once (unconditional, clamped indices, as in one round).
+            int2 s[kRows];
+            bool act1[kRows];                     // still jumping at level r + 1
+#pragma unroll
+            for (int q = 0; q < kRows; q++) {
+                const long long v = base + q * kWave;
+                s[q] = Sin[min(v, N - 1)];
+                if (v >= N) s[q].x = -1;
+            }
+#pragma unroll 1
+            for (int step = 0; step < 3; step++) {
+                int2 t[kRows];
+                int bl[kRows];
+#pragma unroll
+                for (int q = 0; q < kRows; q++) {
+                    const int x = max(s[q].x, 0);
+                    bl[q] = blast[x / kBS];
+                    t[q] = Sl[(size_t)level_slot(r) * N + x];
+                }
+#pragma unroll
+                for (int q = 0; q < kRows; q++) {
+                    const bool again = s[q].x >= 0 && bl[q] < r;
+                    if (__any(again)) {
+                        const int2 u = Sl[(size_t)level_slot(again ? bl[q] : r) * N + (again ? s[q].x : 0)];
+                        if (again) t[q] = u;
+                    }
+                }
+                const int sp = (step == 2 ? 3 : step + 1) * span;   // vertices s spans so far
+#pragma unroll
+                for (int q = 0; q < kRows; q++)
+                    if (s[q].x >= 0) s[q] = join_rec(s[q], t[q], sp);
+                if (step == 0)
+#pragma unroll
+                    for (int q = 0; q < kRows; q++) {
+                        const long long v = base + q * kWave;
+                        if (v < N) Sout1[v] = s[q];           // level r + 1
+                        act1[q] = s[q].x >= 0;
+                    }
+                bool more = false;
+#pragma unroll
+                for (int q = 0; q < kRows; q++) more |= s[q].x >= 0;
+                if (!__any(more)) break;
+            }
+            unsigned long long any = 0;
+#pragma unroll
+            for (int q = 0; q < kRows; q++) {
+                const long long v = base + q * kWave;
+                if (v < N) Sout2[v] = s[q];                   // level r + 2
+                const bool act = s[q].x >= 0;
+                // resolved in round r + 1: jumping at level r + 1, not at r + 2
+                const bool res = act1[q] && !act;
+                const unsigned long long ma = __ballot(act);
+                my_res += (unsigned)__popcll(__ballot(res));
+                my_act += (unsigned)__popcll(ma);
+                any |= ma;
+            }
+            if (lane == 0) {
+                bact_out[b] = (uint8_t)(any != 0);
+                blast[b] = (uint8_t)(r + 2);
+            }
+        }
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0 && (my_res | my_act)) {
+        atomicAdd(&cnt_res, my_res);
+        atomicAdd(&cnt_act, my_act);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && (cnt_res | cnt_act))
+        atomicAdd(&slot_row[blockIdx.x & (kSlots - 1)], ((unsigned long long)cnt_res << 32) | cnt_act);
+}
+
 __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int2* __restrict__ Sl,
                                               unsigned long long* __restrict__ Zl,
                                               JumpCnt* __restrict__ jc,
@@ -1126,6 +1272,7 @@ __device__ __forceinline__ void act_body(int G, GameCtl* __restrict__ gc, StepPr
 struct StepArgs {
     long long N, nb, own_n0, own_n1;
     int G, maxn, lmax, trace, trace_cap;
+    int dshift;  // jump rounds with fewer than N >> dshift vertices jumping run two levels (0: never)
     const int *vinfo, *off, *tgt, *gid, *own0, *own1, *hi;
     GameCtl* gc;
     int *strat, *par, *dep, *key, *flag, *succ;
@@ -1161,7 +1308,7 @@ __global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8, 8))
 k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
 {
     const StepArgs& a = *ap;
-    __shared__ int ph_s, r_s;
+    __shared__ int ph_s, r_s, dbl_s, dpar_s;
     __shared__ unsigned int cnt[2];
     if (threadIdx.x < kWave) {                    // the first wave decides
         const int lane = threadIdx.x;
@@ -1170,9 +1317,9 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         // holds word k of the state's head, every lane its slot word.
         const int hw = reinterpret_cast<const int*>(pg)[min(lane, kProgHeadWords - 1)];
         const unsigned long long x = pg->pslot[(i + 3) & 3][lane];
-        const int maxn = a.maxn, lmax = a.lmax;
+        const int maxn = a.maxn, lmax = a.lmax, dshift = a.dshift;
         // all these loads issued before any is waited for
-        asm volatile("" ::"v"(hw), "v"(x), "s"(maxn), "s"(lmax));
+        asm volatile("" ::"v"(hw), "v"(x), "s"(maxn), "s"(lmax), "s"(dshift));
         const int p = (i + 1) & 1;
         // lane k's word as a scalar (the same in every lane)
         auto head = [&](int k) { return __builtin_amdgcn_readlane(hw, k); };
@@ -1181,9 +1328,10 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         const int hseq = head(4), dseq = head(5 + p);                     // host_seq, dev_seq[p]
         const unsigned long long sm =
             ((unsigned long long)(unsigned)head(9 + 2 * p) << 32) | (unsigned)head(8 + 2 * p);   // psum[p]
-        int ph = prev & 0xff, r = prev >> 8;
+        // st = phase | bact parity offset << 7 | round << 8
+        int ph = prev & 0x7f, r = prev >> 8, dbl = 0, dpar = (prev >> 7) & 1;
         switch (ph) {
-        case PH_INIT: ph = PH_JUMP; r = 0; break;
+        case PH_INIT: ph = PH_JUMP; r = 0; dpar = 0; break;
         case PH_JUMP: r++; break;
         case PH_LABEL: ph = PH_SW1; break;
         case PH_SW1: ph = PH_CTL1; break;
@@ -1209,16 +1357,22 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
                 if (blockIdx.x == 0 && lane == 0) a.jc->err = 1;
                 run = 0;
             }
+            // A sparse round (few vertices still jumping after round r - 1)
+            // runs rounds r and r + 1 in this launch (jump2_body); the next
+            // launch then takes round r + 2.
+            const unsigned long long act_prev = wave_sum_u64(x & 0xFFFFFFFFu);
+            dbl = run && dshift > 0 && r >= 1 && r <= kLiftCap && r + 2 <= lmax &&
+                  (act_prev << dshift) < (unsigned long long)a.N;
             if (blockIdx.x == 0 && lane == 0) {
                 pg->psum[i & 1] = summary_word(run, stop, cyc);
-                if (run) a.jc->last = r;
+                if (run) a.jc->last = r + dbl;
             }
             if (!run) ph = PH_LABEL;
         }
         if (blockIdx.x == 0) {
             pg->pslot[(i + 2) & 3][lane] = 0;     // launch i+2 counts its round there
             if (lane == 0) {
-                pg->st[i & 1] = ph | (r << 8);
+                pg->st[i & 1] = ph | ((dpar ^ dbl) << 7) | ((r + dbl) << 8);
                 if (ph == PH_SW1) pg->any_outer = 0;   // set again by CTL1
                 pg->dev_seq[i & 1] = ph == PH_ACT ? hseq : dseq;
             }
@@ -1226,6 +1380,8 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         if (lane == 0) {
             ph_s = ph;
             r_s = r;
+            dbl_s = dbl;
+            dpar_s = dpar;
             cnt[0] = cnt[1] = 0;
         }
     }
@@ -1237,11 +1393,20 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
                       glb(a.solved), glb(a.lab), glb(a.par), glb(a.Sl), glb(a.Zl), glb(a.jc),
                       glb(a.blive), glb(a.bact), glb(a.blast), a.trace, glb(a.hi));
         break;
-    case PH_JUMP:
-        jump_body(r, a.N, glb(a.Sl), glb(a.Zl), pg->pslot[i & 3], glb(a.blive),
-                  glb(a.bact) + (r & 1) * a.nb, glb(a.bact) + ((r + 1) & 1) * a.nb, glb(a.blast),
-                  a.trace, glb(a.hi), cnt);
+    case PH_JUMP: {
+        // block flags of round r: after an odd number of two-round launches
+        // in this valuation, the parity of the flag buffers is offset by one
+        const int pin = (r + dpar_s) & 1;
+        if (dbl_s)
+            jump2_body(r, a.N, glb(a.Sl), glb(a.Zl), pg->pslot[i & 3], glb(a.blive),
+                       glb(a.bact) + pin * a.nb, glb(a.bact) + (pin ^ 1) * a.nb, glb(a.blast),
+                       a.trace, glb(a.hi), cnt);
+        else
+            jump_body(r, a.N, glb(a.Sl), glb(a.Zl), pg->pslot[i & 3], glb(a.blive),
+                      glb(a.bact) + pin * a.nb, glb(a.bact) + (pin ^ 1) * a.nb, glb(a.blast),
+                      a.trace, glb(a.hi), cnt);
         break;
+    }
     case PH_LABEL:
         label_body(a.N, a.maxn, a.lmax, glb(a.Sl), glb(a.Zl), glb(a.blast), glb(a.gid), glb(a.gc),
                    glb(a.strat), glb(a.solved), glb(a.lab), glb(a.dep), glb(a.key), glb(a.dig),
@@ -2114,6 +2279,7 @@ static StepArgs step_args(Core& c)
     a.G = c.G;
     a.maxn = maxn_of(c);
     a.lmax = c.lmax;
+    a.dshift = g_si_tune.dshift;
     a.trace = c.trace;
     a.trace_cap = c.trace_cap;
     a.vinfo = c.vinfo;
@@ -2156,6 +2322,7 @@ static int upload_step_args(Core& c, StepArgs** out)
     const StepArgs a = step_args(c);
     PSI_HIP(hipMemcpyAsync(c.pargs, &a, sizeof a, hipMemcpyHostToDevice, c.s));
     PSI_HIP(hipStreamSynchronize(c.s));
+    c.pargs_dshift = a.dshift;
     *out = (StepArgs*)c.pargs;
     return PSI_OK;
 }
@@ -2164,6 +2331,10 @@ static int upload_step_args(Core& c, StepArgs** out)
 // captured on first use.
 static int launch_program(Core& c, int launches)
 {
+    if (c.pgraph && c.pargs_dshift != g_si_tune.dshift) {   // tuned since the capture
+        StepArgs* a = nullptr;
+        if (int rc = upload_step_args(c, &a)) return rc;
+    }
     if (!c.pgraph || c.pgraph_launches != launches) {
         if (c.pgraph) {
             (void)hipGraphExecDestroy(c.pgraph);
@@ -2219,7 +2390,7 @@ int core_solve_program_profiled(Core& c, double* times, long long* counts)
             rc = fail(PSI_ERR_HIP, "step program: launch failed");
             break;
         }
-        const int ph = st & 0xff;
+        const int ph = st & 0x7f;
         if (ph == PH_DONE) break;
         float ms = 0;
         (void)hipEventElapsedTime(&ms, e0, e1);

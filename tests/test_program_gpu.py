@@ -15,6 +15,7 @@ pytestmark = pytest.mark.gpu
 def modes(gpu):
     yield gpu
     gpu.si_tune(gpu.SIT_PROGRAM, 1)
+    gpu.si_tune(gpu.SIT_DOUBLE_ROUNDS, 1)
 
 
 def _run(gpu, games, program, steps_per_sync=4, reset=False):
@@ -154,3 +155,31 @@ def test_solve_then_bf_on_one_handle(modes):
         if found == 4:
             break
     assert found == 4
+
+
+@pytest.mark.parametrize("shift", [0, 1, 4, 16])
+@pytest.mark.parametrize("idx", range(3))
+def test_program_double_rounds(modes, idx, shift):
+    """Sparse jump rounds advancing two levels in one launch (any threshold,
+    shift 1 makes nearly every round after the first a double one): the same
+    strategies, labels, solved sets, counts and per-round trace hashes as the
+    rounds launched one by one."""
+    gpu = modes
+    g = _games(gpu)[idx]
+    print(f"double rounds shift {shift}, game {idx}", flush=True)
+    gpu.si_tune(gpu.SIT_DOUBLE_ROUNDS, shift)
+    _same(_run(gpu, g, 1), _run(gpu, g, 0))
+
+
+@pytest.mark.parametrize("shift", [1, 4])
+def test_program_double_rounds_batch_reset(modes, shift):
+    gpu = modes
+    sizes = [3, 17, 400, 2500, 10, 9000, 1, 64, 30000]
+    games = []
+    for i, n in enumerate(sizes):
+        n = max(n, 2)
+        games.append(gpu.gen_chain_game(i, n) if i % 3 == 2
+                     else gpu.gen_random_game(90 + i, n, max(1, n // 3), 1, 4))
+    gpu.si_tune(gpu.SIT_DOUBLE_ROUNDS, shift)
+    _same(_run(gpu, games, 1, reset=True), _run(gpu, games, 0, reset=True))
+    _same(_run(gpu, games, 1, steps_per_sync=3), _run(gpu, games, 0))
