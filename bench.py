@@ -217,6 +217,7 @@ def batch_bench(psi, n_games=1024, seed=3, lo=1000, hi=1000000, check_alone=True
         s.close()
     t_alone = time.perf_counter() - t
     log(f"[bench] batch: one-at-a-time {t_alone:.2f}s")
+    autotune.apply_si(autotune.SI_DEFAULT)        # later legs run with the defaults
     same = all(x[0] == y[0] and np.array_equal(x[1], y[1]) and np.array_equal(x[2], y[2])
                for x, y in zip(res_b, res_a))
     n_tot = int(sizes.sum())
