@@ -276,11 +276,10 @@ int psi_pool_info(psi_pool* p, int* running, int* queued, int64_t* vertex_cap, i
  * launch runs the next phase of the loop: no launch is spent on a jump round
  * after convergence); 0 launches every phase of every step. */
 #define PSI_SIT_PROGRAM 5
-/* PSI_SIT_DOUBLE_ROUNDS: in the step program, a pointer-jumping round with
- * fewer than n >> value vertices still jumping (n: vertices of the handle)
- * advances two levels in one launch; 0 never, 1..16 (default 1: while fewer
- * than half the vertices still jump). */
-#define PSI_SIT_DOUBLE_ROUNDS 6
+/* PSI_SIT_TWO_LEVELS: 1 (default) the step program's pointer-jumping
+ * launches advance two levels each (three dependent gathers per vertex),
+ * 0 one level; results are identical. */
+#define PSI_SIT_TWO_LEVELS 6
 int psi_si_tune(int param, int value);
 
 /* ---- seeded generators (host only, no device calls) --------------------- */

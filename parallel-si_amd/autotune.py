@@ -2,8 +2,7 @@
 
 Batched SI: times solves of a sample of the batch for each candidate
 step-program grid, steps per synchronisation (launches per graph: 16 per
-step) and two-level jump threshold (a round with fewer than n >> shift
-vertices still jumping runs two levels in one launch), one knob at a time
+step) and whether jump launches advance two levels, one knob at a time
 from the defaults (coordinate descent, keeping a
 change only if it is at least 2% faster).  List ranking: the same over the level-0 and
 reduced-level splitter densities and the level-0 hop cap, on a list of the
@@ -17,9 +16,9 @@ import numpy as np
 
 import psi
 
-SI_DEFAULT = {"phase_grid": 2048, "steps_per_sync": 4, "double_shift": 1}
+SI_DEFAULT = {"phase_grid": 2048, "steps_per_sync": 4, "two_levels": 1}
 SI_CANDIDATES = {"phase_grid": [1024, 1536, 2048], "steps_per_sync": [2, 4, 8],
-                 "double_shift": [0, 1, 2, 4]}
+                 "two_levels": [0, 1]}
 LR_DEFAULT = {"log_d0": 5, "local_bits": 12, "log_dk": 5}
 LR_CANDIDATES = {"log_d0": [4, 5, 6, 7], "local_bits": [10, 12, 14], "log_dk": [4, 5, 6]}
 _LR_PARAM = {"log_d0": psi.LRT_LOG_D0, "local_bits": psi.LRT_LOCAL_BITS, "log_dk": psi.LRT_LOG_DK}
@@ -27,7 +26,7 @@ _LR_PARAM = {"log_d0": psi.LRT_LOG_D0, "local_bits": psi.LRT_LOCAL_BITS, "log_dk
 
 def apply_si(cfg):
     psi.si_tune(psi.SIT_PHASE_GRID, int(cfg["phase_grid"]))
-    psi.si_tune(psi.SIT_DOUBLE_ROUNDS, int(cfg.get("double_shift", SI_DEFAULT["double_shift"])))
+    psi.si_tune(psi.SIT_TWO_LEVELS, int(cfg.get("two_levels", SI_DEFAULT["two_levels"])))
 
 
 def apply_lr(cfg):

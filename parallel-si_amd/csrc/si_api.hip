@@ -167,10 +167,9 @@ extern "C" int psi_si_tune(int param, int value)
         g_si_tune.program = value;
         return PSI_OK;
     }
-    if (param == PSI_SIT_DOUBLE_ROUNDS) {
-        if (value < 0 || value > 16)
-            return fail(PSI_ERR_INVALID, "psi_si_tune: double rounds shift is 0..16");
-        g_si_tune.dshift = value;
+    if (param == PSI_SIT_TWO_LEVELS) {
+        if (value != 0 && value != 1) return fail(PSI_ERR_INVALID, "psi_si_tune: two levels is 0 or 1");
+        g_si_tune.two_levels = value;
         return PSI_OK;
     }
     if (value < 64 || value > (1 << 20)) return fail(PSI_ERR_INVALID, "psi_si_tune: value out of range");

@@ -133,7 +133,7 @@ struct Core {
     int graph_steps = 0;
     StepProg* prog = nullptr;          // step program state (not in a serving pool)
     void* pargs = nullptr;             // step program arguments (device copy)
-    int pargs_dshift = -1;             // the two-round threshold in that copy
+    int pargs_two = -1;                // two_levels in that copy
     hipGraphExec_t pgraph = nullptr;   // captured step-program launches
     int pgraph_launches = 0;
 };
@@ -145,8 +145,7 @@ struct SiTuning {
     unsigned pool_grid = 4096;   // the same in a serving pool
     unsigned phase_grid = 2048;  // blocks of a step-program launch
     int program = 1;             // solve loop as a step program (else fixed launches per step)
-    int dshift = 1;              // step program: a jump round with fewer than N >> dshift
-                                 // vertices jumping runs two levels (0: never)
+    int two_levels = 1;          // step program: jump launches advance two levels
 };
 extern SiTuning g_si_tune;
 

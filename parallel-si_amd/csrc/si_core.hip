@@ -51,12 +51,14 @@ constexpr int kBS = 256;
 constexpr int kRows = kBS / kWave;   // rows of 64 vertices in a block
 constexpr int kInitRows = 2;         // rows of a block initialised together
 constexpr int kPriMask = 0x3FFFFFFF;
-// Jump levels 0..kLiftCap are kept as binary-lifting tables; later rounds
-// (only needed to cover long cycles) rotate through two more buffers.
+// Jump levels 0..kLiftCap are kept as binary-lifting tables; later levels
+// rotate through three more buffers (a launch may read level r while writing
+// levels r + 1 and r + 2).
 constexpr int kLiftCap = 12;
+constexpr int kRotating = 3;
 __host__ __device__ __forceinline__ int level_slot(int j)
 {
-    return j <= kLiftCap ? j : kLiftCap + 1 + (j & 1);
+    return j <= kLiftCap ? j : kLiftCap + 1 + (j - kLiftCap - 1) % kRotating;
 }
 
 __device__ __forceinline__ int pri_of(int vi) { return vi & kPriMask; }
@@ -478,13 +480,14 @@ __device__ __forceinline__ int2 join_rec(int2 s, int2 t, int span)
     return o;
 }
 
-// Two rounds in one launch, for sparse rounds (few vertices still jumping),
-// where the launch, not the gathers, is the cost: from level r, level r + 1
+// Two rounds in one launch: three dependent gather steps instead of two
+// launches (one gather step each, after the first round few vertices have
+// resolved).  From level r, level r + 1
 // = s o rec(x1) and level r + 2 = (level r + 1) o rec(x2) o rec(x3), every
 // gathered record at level r (written by an earlier launch; x1, x2, x3 the
 // ends of the spans so far), with the same re-read at a stopped block's
-// level as one round.  Needs level r + 2 in a different buffer from level r
-// (r <= kLiftCap).  The counts are round r + 1's (resolved at level r + 2,
+// level as one round (levels r, r + 1 and r + 2 are in different buffers).
+// The counts are round r + 1's (resolved at level r + 2,
 // still jumping after it): a stop seen there is a stop at r or r + 1, and
 // only whether there was one is ever used.
 __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict__ Sl,
@@ -1272,7 +1275,7 @@ __device__ __forceinline__ void act_body(int G, GameCtl* __restrict__ gc, StepPr
 struct StepArgs {
     long long N, nb, own_n0, own_n1;
     int G, maxn, lmax, trace, trace_cap;
-    int dshift;  // jump rounds with fewer than N >> dshift vertices jumping run two levels (0: never)
+    int two_levels;  // jump launches advance two levels (else one)
     const int *vinfo, *off, *tgt, *gid, *own0, *own1, *hi;
     GameCtl* gc;
     int *strat, *par, *dep, *key, *flag, *succ;
@@ -1317,9 +1320,9 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         // holds word k of the state's head, every lane its slot word.
         const int hw = reinterpret_cast<const int*>(pg)[min(lane, kProgHeadWords - 1)];
         const unsigned long long x = pg->pslot[(i + 3) & 3][lane];
-        const int maxn = a.maxn, lmax = a.lmax, dshift = a.dshift;
+        const int maxn = a.maxn, lmax = a.lmax, two = a.two_levels;
         // all these loads issued before any is waited for
-        asm volatile("" ::"v"(hw), "v"(x), "s"(maxn), "s"(lmax), "s"(dshift));
+        asm volatile("" ::"v"(hw), "v"(x), "s"(maxn), "s"(lmax), "s"(two));
         const int p = (i + 1) & 1;
         // lane k's word as a scalar (the same in every lane)
         auto head = [&](int k) { return __builtin_amdgcn_readlane(hw, k); };
@@ -1357,12 +1360,9 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
                 if (blockIdx.x == 0 && lane == 0) a.jc->err = 1;
                 run = 0;
             }
-            // A sparse round (few vertices still jumping after round r - 1)
-            // runs rounds r and r + 1 in this launch (jump2_body); the next
-            // launch then takes round r + 2.
-            const unsigned long long act_prev = wave_sum_u64(x & 0xFFFFFFFFu);
-            dbl = run && dshift > 0 && r >= 1 && r <= kLiftCap && r + 2 <= lmax &&
-                  (act_prev << dshift) < (unsigned long long)a.N;
+            // Rounds r and r + 1 in this launch (jump2_body) while level
+            // r + 2 exists; the next launch then takes round r + 2.
+            dbl = run && two && r + 2 <= lmax;
             if (blockIdx.x == 0 && lane == 0) {
                 pg->psum[i & 1] = summary_word(run, stop, cyc);
                 if (run) a.jc->last = r + dbl;
@@ -1650,7 +1650,7 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
     c.trace_cap = trace ? std::max(trace_cap, 1) : 0;
     c.max_pri = *std::max_element(mp.begin(), mp.end());
     c.lmax = ceil_log2(maxn) + 2;
-    c.levels = std::min(c.lmax + 1, kLiftCap + 3);
+    c.levels = std::min(c.lmax + 1, kLiftCap + 1 + kRotating);
     c.h_n.resize(G);
     c.h_m.resize(G);
     c.h_vbase.resize(G);
@@ -1856,7 +1856,7 @@ __global__ void k_pool_release(long long vbase, long long cnt, int sentinel, int
 long long core_pool_bytes_per_vertex(int maxn)
 {
     const int lmax = ceil_log2(maxn) + 2;
-    const int levels = std::min(lmax + 1, kLiftCap + 3);
+    const int levels = std::min(lmax + 1, kLiftCap + 1 + kRotating);
     // vinfo off gid strat par dep key own0 own1 (ints), solved lab, records
     return 9 * 4 + 2 + 8LL * levels + 1;           // + block flags, change bits, rounded up
 }
@@ -1878,7 +1878,7 @@ int core_create_pool(Core& c, long long vcap, long long ecap, int slots, int max
     c.trace = false;
     c.trace_cap = 0;
     c.lmax = ceil_log2(maxn) + 2;
-    c.levels = std::min(c.lmax + 1, kLiftCap + 3);
+    c.levels = std::min(c.lmax + 1, kLiftCap + 1 + kRotating);
     c.h_n.assign(G, 0);
     c.h_m.assign(G, 0);
     c.h_vbase.assign(G, 0);
@@ -2279,7 +2279,7 @@ static StepArgs step_args(Core& c)
     a.G = c.G;
     a.maxn = maxn_of(c);
     a.lmax = c.lmax;
-    a.dshift = g_si_tune.dshift;
+    a.two_levels = g_si_tune.two_levels;
     a.trace = c.trace;
     a.trace_cap = c.trace_cap;
     a.vinfo = c.vinfo;
@@ -2322,7 +2322,7 @@ static int upload_step_args(Core& c, StepArgs** out)
     const StepArgs a = step_args(c);
     PSI_HIP(hipMemcpyAsync(c.pargs, &a, sizeof a, hipMemcpyHostToDevice, c.s));
     PSI_HIP(hipStreamSynchronize(c.s));
-    c.pargs_dshift = a.dshift;
+    c.pargs_two = a.two_levels;
     *out = (StepArgs*)c.pargs;
     return PSI_OK;
 }
@@ -2331,7 +2331,7 @@ static int upload_step_args(Core& c, StepArgs** out)
 // captured on first use.
 static int launch_program(Core& c, int launches)
 {
-    if (c.pgraph && c.pargs_dshift != g_si_tune.dshift) {   // tuned since the capture
+    if (c.pgraph && c.pargs_two != g_si_tune.two_levels) {   // tuned since the capture
         StepArgs* a = nullptr;
         if (int rc = upload_step_args(c, &a)) return rc;
     }

@@ -253,7 +253,7 @@ def list_rank_status():
 
 
 SIT_JUMP_GRID, SIT_STEP_GRID, SIT_POOL_GRID, SIT_PHASE_GRID, SIT_PROGRAM = 1, 2, 3, 4, 5
-SIT_DOUBLE_ROUNDS = 6
+SIT_TWO_LEVELS = 6
 _sig("psi_si_tune", C.c_int, C.c_int, C.c_int)
 
 

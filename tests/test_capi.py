@@ -95,11 +95,11 @@ def test_si_tune_rejects_bad_values():
     with pytest.raises(psi.PsiError):
         psi.si_tune(99, 2048)
     with pytest.raises(psi.PsiError):
-        psi.si_tune(psi.SIT_DOUBLE_ROUNDS, 17)
+        psi.si_tune(psi.SIT_TWO_LEVELS, 2)
     with pytest.raises(psi.PsiError):
-        psi.si_tune(psi.SIT_DOUBLE_ROUNDS, -1)
+        psi.si_tune(psi.SIT_TWO_LEVELS, -1)
     psi.si_tune(psi.SIT_PROGRAM, 0)
     psi.si_tune(psi.SIT_PROGRAM, 1)
     psi.si_tune(psi.SIT_PHASE_GRID, 2048)
-    psi.si_tune(psi.SIT_DOUBLE_ROUNDS, 0)
-    psi.si_tune(psi.SIT_DOUBLE_ROUNDS, 1)
+    psi.si_tune(psi.SIT_TWO_LEVELS, 0)
+    psi.si_tune(psi.SIT_TWO_LEVELS, 1)

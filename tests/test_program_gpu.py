@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 def modes(gpu):
     yield gpu
     gpu.si_tune(gpu.SIT_PROGRAM, 1)
-    gpu.si_tune(gpu.SIT_DOUBLE_ROUNDS, 1)
+    gpu.si_tune(gpu.SIT_TWO_LEVELS, 1)
 
 
 def _run(gpu, games, program, steps_per_sync=4, reset=False):
@@ -157,22 +157,20 @@ def test_solve_then_bf_on_one_handle(modes):
     assert found == 4
 
 
-@pytest.mark.parametrize("shift", [0, 1, 4, 16])
+@pytest.mark.parametrize("two", [0, 1])
 @pytest.mark.parametrize("idx", range(3))
-def test_program_double_rounds(modes, idx, shift):
-    """Sparse jump rounds advancing two levels in one launch (any threshold,
-    shift 1 makes nearly every round after the first a double one): the same
-    strategies, labels, solved sets, counts and per-round trace hashes as the
-    rounds launched one by one."""
+def test_program_two_levels(modes, idx, two):
+    """Jump launches advancing one or two levels: the same strategies, labels,
+    solved sets, counts and per-round trace hashes as the rounds launched one
+    by one."""
     gpu = modes
     g = _games(gpu)[idx]
-    print(f"double rounds shift {shift}, game {idx}", flush=True)
-    gpu.si_tune(gpu.SIT_DOUBLE_ROUNDS, shift)
+    print(f"two levels {two}, game {idx}", flush=True)
+    gpu.si_tune(gpu.SIT_TWO_LEVELS, two)
     _same(_run(gpu, g, 1), _run(gpu, g, 0))
 
 
-@pytest.mark.parametrize("shift", [1, 4])
-def test_program_double_rounds_batch_reset(modes, shift):
+def test_program_two_levels_batch_reset(modes):
     gpu = modes
     sizes = [3, 17, 400, 2500, 10, 9000, 1, 64, 30000]
     games = []
@@ -180,6 +178,21 @@ def test_program_double_rounds_batch_reset(modes, shift):
         n = max(n, 2)
         games.append(gpu.gen_chain_game(i, n) if i % 3 == 2
                      else gpu.gen_random_game(90 + i, n, max(1, n // 3), 1, 4))
-    gpu.si_tune(gpu.SIT_DOUBLE_ROUNDS, shift)
+    gpu.si_tune(gpu.SIT_TWO_LEVELS, 1)
     _same(_run(gpu, games, 1, reset=True), _run(gpu, games, 0, reset=True))
     _same(_run(gpu, games, 1, steps_per_sync=3), _run(gpu, games, 0))
+
+
+def test_program_two_levels_long_cycles(modes):
+    """Valuations whose cycle-bound rounds run past the lifting levels (into
+    the rotating buffers, two levels per launch): a 4096-vertex single cycle
+    with tails, and chains long enough for deep rounds."""
+    gpu = modes
+    games = [gpu.gen_chain_game(41, 70000), gpu.gen_random_game(42, 200000, 20, 1, 2)]
+    for g in games:
+        r = oracle.solve(g)
+        gpu.si_tune(gpu.SIT_TWO_LEVELS, 1)
+        a = _run(gpu, g, 1)
+        assert a[0][0][:2] == (r.iter, r.br)
+        assert np.array_equal(a[0][1], r.strat) and np.array_equal(a[0][2], r.inf)
+        _same(a, _run(gpu, g, 0))
