@@ -297,10 +297,10 @@ __device__ __forceinline__ void val_init_body(
                     if (sv[q]) {
                         const int code = term_code(lv[q] == 1 ? 1 : 2, 0);
                         S0[v] = make_int2(code, (first | (old[q] != code)) ? kChangedBit : 0);
-                        par[v] = code;
+                        if (old[q] != code) par[v] = code;   // (unchanged links: no store)
                         if (trace) Z0[v] = 0;
                     } else {
-                        par[v] = sq;
+                        if (old[q] != sq) par[v] = sq;
                         S0[v] = make_int2(sq >= 0 ? sq : term_code(0, 1),
                                           pri_of(vi[q]) | ((first | (old[q] != sq)) ? kChangedBit : 0));
                         if (trace) Z0[v] = splitmix64((unsigned long long)(uint32_t)pri_of(vi[q]));
@@ -769,7 +769,7 @@ __device__ __forceinline__ void label_body(
                                 dep[v] = 0;
                             }
                             ch = ch || l != lold[q];
-                            lab[v] = (int8_t)l;
+                            if (l != lold[q]) lab[v] = (int8_t)l;
                             if (loop_form && l == -1) solved[v] = 1;
                         }
                         if (trace) dig[v] = d;
