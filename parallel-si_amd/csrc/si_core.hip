@@ -1248,8 +1248,9 @@ __global__ void k_ctl0(int G, GameCtl* gc) { ctl0_body(G, gc, nullptr); }
 // ---------------------------------------------------------------------------
 // Step program: the device-resident loop as identical launches of k_phase.
 // Launch i reads the phase launch i-1 ran and runs the next one:
-//   INIT (val_init) -> JUMP 0 -> JUMP 1 -> ... (while decide_round runs the
-//   round) -> LABEL -> SW1 -> CTL1 -> [MARK0 -> SW0 -> CTL0, only when some
+//   INIT (val_init) -> JUMP 0 -> JUMP 2 -> ... (rounds r and r + 1 per
+//   launch, while the decision runs round r) -> LABEL -> SW1 -> CTL1 ->
+//   [MARK0+SW0 -> CTL0, only when some
 //   game takes the outer step] -> INIT ...
 // so no launch is spent on a round after convergence or on an outer-step
 // phase that no game takes; a graph of these launches can stop anywhere and
@@ -1339,7 +1340,7 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         case PH_LABEL: ph = PH_SW1; break;
         case PH_SW1: ph = PH_CTL1; break;
         case PH_CTL1: ph = any_outer ? PH_MARK0 : PH_INIT; break;
-        case PH_MARK0: ph = PH_SW0; break;
+        case PH_MARK0: ph = PH_CTL0; break;   // switch(0) ran in the same launch
         case PH_SW0: ph = PH_CTL0; break;
         case PH_DONE: break;
         default: ph = PH_INIT; break;             // PH_START, PH_CTL0, PH_ACT
@@ -1413,6 +1414,14 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
                    glb(a.tr), a.trace_cap, 1, a.trace, glb(a.jc), glb(a.blive), glb(a.hi), glb(a.chg),
                    glb(a.chg0), 0);
         break;
+    case PH_MARK0:
+        // mark_solved(0), then switch(0) in the same launch: the switch reads
+        // nothing the marking writes (solved flags; succ of Odd's vertices,
+        // whose strategies switch(0) leaves alone), and the marking reads no
+        // strategy switch(0) writes
+        mark0_body(a.N, glb(a.gid), glb(a.gc), glb(a.lab), glb(a.solved), glb(a.hi), glb(a.vinfo),
+                   glb(a.strat), a.succ ? glb(a.succ) : nullptr);
+        [[fallthrough]];
     case PH_SW1:
     case PH_SW0: {                                // one inlined copy of the switch
         const int pl = ph == PH_SW1 ? 1 : 0;
@@ -1423,10 +1432,6 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         break;
     }
     case PH_CTL1: ctl1_body(a.G, glb(a.gc), pg); break;
-    case PH_MARK0:
-        mark0_body(a.N, glb(a.gid), glb(a.gc), glb(a.lab), glb(a.solved), glb(a.hi), glb(a.vinfo),
-                   glb(a.strat), a.succ ? glb(a.succ) : nullptr);
-        break;
     case PH_CTL0: ctl0_body(a.G, glb(a.gc), pg); break;
     case PH_ACT: act_body(a.G, glb(a.gc), pg, glb(a.flag) + 2); break;
     default: break;
