@@ -261,7 +261,7 @@ __device__ __forceinline__ void val_init_body(
 #pragma unroll
             for (int q = 0; q < kInitRows; q++) {
                 const long long vc = min(base + (h + q) * kWave, N - 1);
-                g[q] = gid[vc];
+                g[q] = gid ? gid[vc] : 0;        // no gid: one game
                 vi[q] = vinfo[vc];
                 s[q] = strat[vc];
                 old[q] = par[vc];
@@ -698,7 +698,7 @@ __device__ __forceinline__ void label_body(
                 for (int q = 0; q < kInitRows; q++) {
                     const long long v = b * kBS + (h + q) * kWave + lane;
                     const long long vc = min(v, N - 1);
-                    g[q] = gid[vc];
+                    g[q] = gid ? gid[vc] : 0;    // no gid: one game
                     sol[q] = solved[vc];
                     lold[q] = lab[vc];
                     s[q] = SF[vc];
@@ -1016,7 +1016,7 @@ __device__ __forceinline__ void switch_body(
         int g = -1, c0 = -1, eb = 0, ee = 0;
         const int v = i < lim ? own[i] : -1;
         if (v >= 0) {
-            g = gid[v];
+            g = gid ? gid[v] : 0;                 // no gid: one game
             c0 = strat[v];                        // loaded together with the game's control
             eb = off[v];
             ee = off[v + 1];
@@ -1388,9 +1388,11 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
     }
     __syncthreads();
     const int ph = ph_s, r = r_s;
+    // one game: every vertex is in game 0 and the bodies skip the gid loads
+    const int* gid1 = a.G == 1 ? nullptr : glb(a.gid);
     switch (ph) {
     case PH_INIT:
-        val_init_body(a.N, glb(a.vinfo), glb(a.off), glb(a.tgt), glb(a.gid), glb(a.gc), glb(a.strat),
+        val_init_body(a.N, glb(a.vinfo), glb(a.off), glb(a.tgt), gid1, glb(a.gc), glb(a.strat),
                       glb(a.solved), glb(a.lab), glb(a.par), glb(a.Sl), glb(a.Zl), glb(a.jc),
                       glb(a.blive), glb(a.bact), glb(a.blast), a.trace, glb(a.hi));
         break;
@@ -1409,7 +1411,7 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         break;
     }
     case PH_LABEL:
-        label_body(a.N, a.maxn, a.lmax, glb(a.Sl), glb(a.Zl), glb(a.blast), glb(a.gid), glb(a.gc),
+        label_body(a.N, a.maxn, a.lmax, glb(a.Sl), glb(a.Zl), glb(a.blast), gid1, glb(a.gc),
                    glb(a.strat), glb(a.solved), glb(a.lab), glb(a.dep), glb(a.key), glb(a.dig),
                    glb(a.tr), a.trace_cap, 1, a.trace, glb(a.jc), glb(a.blive), glb(a.hi), glb(a.chg),
                    glb(a.chg0), 0);
@@ -1426,7 +1428,7 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
     case PH_SW0: {                                // one inlined copy of the switch
         const int pl = ph == PH_SW1 ? 1 : 0;
         switch_body(pl, a.N, pl ? glb(a.own1) : glb(a.own0), pl ? a.own_n1 : a.own_n0, glb(a.vinfo),
-                    glb(a.off), glb(a.tgt), glb(a.gid), glb(a.gc), glb(a.strat), glb(a.key),
+                    glb(a.off), glb(a.tgt), gid1, glb(a.gc), glb(a.strat), glb(a.key),
                     glb(a.dep), glb(a.par), glb(a.Sl), glb(a.jc), 1, glb(a.flag), nullptr, glb(a.hi),
                     glb(a.chg), glb(a.chg0));
         break;
