@@ -219,7 +219,9 @@ int psi_get_trace(psi_solver* s, int game, uint64_t* hashes, int cap, int* count
  * psi_solve_profiled, psi_solve with PSI_SIT_PROGRAM 0); the step program
  * keeps only *last and *err. */
 int psi_get_jump_stats(psi_solver* s, int* out, int rounds, int* last, int* err);
-/* Loads strategy (-1 sink), solved flags and labels of solved vertices. */
+/* Loads strategy (-1 sink), solved flags and labels of solved vertices.  The
+ * game's run restarts from that state: its round and iteration counts return
+ * to zero and a finished game takes part in the next valuation again. */
 int psi_set_state(psi_solver* s, int game, const int* strat, const int* solved,
                   const int* labels);
 

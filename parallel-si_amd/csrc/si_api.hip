@@ -384,6 +384,7 @@ extern "C" int psi_set_state(psi_solver* s, int game, const int* strat, const in
     PSI_HIP(hipMemcpyAsync(c.strat + vb, st.data(), n * 4, hipMemcpyHostToDevice, c.s));
     PSI_HIP(hipMemcpyAsync(c.solved + vb, so.data(), n, hipMemcpyHostToDevice, c.s));
     PSI_HIP(hipMemcpyAsync(c.lab + vb, la.data(), n, hipMemcpyHostToDevice, c.s));
+    if (int rc = core_state_loaded(c, game)) return rc;
     PSI_HIP(hipStreamSynchronize(c.s));
     return PSI_OK;
 }

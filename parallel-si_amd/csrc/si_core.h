@@ -168,6 +168,8 @@ int core_set_params(Core& c, int reset, int max_rounds, int bf = 0);
 // mark_solved(1) and counts the round (the device-resident loop's form).
 int core_valuation(Core& c, bool loop_form);
 int core_mark_solved(Core& c, int player);
+// After strategies, solved flags and labels of one game were loaded from the host.
+int core_state_loaded(Core& c, int game);
 int core_reset_opp(Core& c);
 int core_switch(Core& c, int player, int* switched_host);   // step-wise, syncs
 // Enqueues `steps` SI steps (device-resident loop body).

@@ -186,10 +186,11 @@ __global__ void k_init_strat(long long N, const int* __restrict__ vinfo, const i
     lab[v] = 0;
 }
 
-__global__ void k_reset_ctl(int G, GameCtl* gc)
+// Restarts the control blocks of games [g0, g1).
+__global__ void k_reset_ctl(int g0, int g1, GameCtl* gc)
 {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= G) return;
+    int g = g0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= g1) return;
     GameCtl c = gc[g];
     c.done = 0;
     c.br = 0;
@@ -708,6 +709,7 @@ __device__ __forceinline__ void label_body(
                 for (int q = 0; q < kInitRows; q++) issued(g[q], sol[q], lold[q], s[q].x, s[q].y, (int)c0w[q]);
                 // then the games' control words
                 int done[kInitRows], vbase[kInitRows], clr0[kInitRows], bf[kInitRows], bfi[kInitRows];
+                int first[kInitRows];
 #pragma unroll
                 for (int q = 0; q < kInitRows; q++) {
                     done[q] = gc[g[q]].done;
@@ -715,9 +717,10 @@ __device__ __forceinline__ void label_body(
                     clr0[q] = gc[g[q]].clear0;
                     bf[q] = gc[g[q]].bf;
                     bfi[q] = gc[g[q]].bf_init;
+                    first[q] = gc[g[q]].br == 0;
                 }
 #pragma unroll
-                for (int q = 0; q < kInitRows; q++) issued(done[q], vbase[q], clr0[q], bf[q], bfi[q], 0);
+                for (int q = 0; q < kInitRows; q++) issued(done[q], vbase[q], clr0[q], bf[q], bfi[q], first[q]);
 #pragma unroll
                 for (int q = 0; q < kInitRows; q++) {
                     const long long v = b * kBS + (h + q) * kWave + lane;
@@ -739,15 +742,21 @@ __device__ __forceinline__ void label_body(
                     bool ch = false;
                     if (live) {
                         loc = (int)(v - vbase[q]);
+                        // Key and depth are stored only in a game's first
+                        // valuation and when the valuation changed: a finite
+                        // path with no changed link is the same path (same
+                        // length and maximum), an infinite label that stayed
+                        // has the same key, a solved vertex keeps its label.
+                        int kv, dv;
                         if (sol[q]) {
                             l = lold[q];
-                            key[v] = (l + 1) << 30;
-                            dep[v] = 0;
+                            kv = (l + 1) << 30;
+                            dv = 0;
                         } else {
                             if (kind == 0) {                   // reaches the sink: finite
                                 l = 0;
-                                dep[v] = term_count(s[q].x);
-                                key[v] = (1 << 30) | (s[q].y & kPriMask);
+                                dv = term_count(s[q].x);
+                                kv = (1 << 30) | (s[q].y & kPriMask);
                                 ch = (s[q].y & kChangedBit) != 0; // a link on the path changed
                                 if (trace) d = ZF[v];
                             } else {
@@ -765,12 +774,16 @@ __device__ __forceinline__ void label_body(
                                 } else {
                                     l = 1;                     // kind 3 (finished game): never on a live path
                                 }
-                                key[v] = (l + 1) << 30;
-                                dep[v] = 0;
+                                kv = (l + 1) << 30;
+                                dv = 0;
                             }
                             ch = ch || l != lold[q];
                             if (l != lold[q]) lab[v] = (int8_t)l;
                             if (loop_form && l == -1) solved[v] = 1;
+                        }
+                        if (first[q] || ch) {
+                            key[v] = kv;
+                            dep[v] = dv;
                         }
                         if (trace) dig[v] = d;
                     }
@@ -2080,7 +2093,7 @@ int core_set_params(Core& c, int reset, int max_rounds, int bf)
 int core_init_strat(Core& c)
 {
     k_init_strat<<<GRID(c.N)>>>(c.N, c.vinfo, c.off, c.tgt, c.strat, c.solved, c.lab);
-    k_reset_ctl<<<GRID(c.G)>>>(c.G, c.gctl);
+    k_reset_ctl<<<GRID(c.G)>>>(0, c.G, c.gctl);
     k_fill<int><<<GRID((long long)(sizeof(JumpCnt) / 4))>>>((int*)c.jc, (long long)(sizeof(JumpCnt) / 4), 0);
     if (c.trace) k_fill<unsigned long long><<<GRID((long long)c.G * c.trace_cap * 4)>>>(
         c.tr, (long long)c.G * c.trace_cap * 4, 0ull);
@@ -2101,6 +2114,17 @@ int core_valuation(Core& c, bool loop_form)
 int core_reset_opp(Core& c)
 {
     k_reset_opp<<<GRID(c.N)>>>(c.N, c.vinfo, c.off, c.tgt, c.strat);
+    PSI_LAUNCH_CHECK();
+    return PSI_OK;
+}
+
+// A state loaded from the host (strategies, solved flags, labels) starts the
+// game's run afresh: its control block restarts, so that a finished game takes
+// part in the next valuation again and that valuation, as the first of the
+// run, stores every key, depth and link (later ones store only changes).
+int core_state_loaded(Core& c, int game)
+{
+    k_reset_ctl<<<1, 64, 0, c.s>>>(game, game + 1, c.gctl);
     PSI_LAUNCH_CHECK();
     return PSI_OK;
 }

@@ -123,6 +123,41 @@ def test_switch_of_given_state(gpu, seed):
         assert np.array_equal(s.strategy(), exp)
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_state_loaded_after_solve(gpu, seed):
+    """A state loaded into a handle that has already solved (the game
+    finished): the game's run restarts from that state, and the next
+    valuation and switch equal the oracle's for it, whatever the earlier
+    valuations stored."""
+    rng = np.random.default_rng(300 + seed)
+    n = int(rng.integers(200, 3000))
+    g = gpu.gen_random_game(700 + seed, n, int(rng.integers(1, 30)), 1, 4)
+    s = gpu.Solver(g, trace=True)
+    s.solve()
+    assert s.stats()[1] > 0
+    for rep in range(2):
+        strat = np.array([g.targets[rng.integers(g.offsets[v], g.offsets[v + 1])]
+                          for v in range(n)], np.int32)
+        strat[(g.owner == 0) & (rng.random(n) < 0.3)] = -1
+        solved = (rng.random(n) < 0.1).astype(np.int32)
+        lab_in = rng.integers(-1, 2, n).astype(np.int32)
+        lab_in[(solved == 1) & (lab_in == 0)] = 1
+        inf, dig, depth = oracle.valuate(g, strat, solved, lab_in)
+        s.set_state(strat, solved, lab_in)
+        assert s.stats() == (0, 0, False)
+        s.compute_valuation()
+        assert np.array_equal(s.labels(), inf), rep
+        assert np.array_equal(s.depths(), depth), rep
+        assert np.array_equal(s.digests(), dig), rep
+        for player in (0, 1):
+            cnt, exp = oracle.switch(g, strat, solved, lab_in, player)
+            s.set_state(strat, solved, lab_in)
+            s.compute_valuation()
+            assert s.switch_strategy(player) == (1 if cnt else 0)
+            assert np.array_equal(s.strategy(), exp), (rep, player)
+    s.close()
+
+
 def test_stepwise_api_matches_solve(gpu):
     """Driving init_strat / compute_valuation / mark_solved / switch_strategy
     from the host (the reference's solve_si loop) gives the same result."""
