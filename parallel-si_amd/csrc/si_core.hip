@@ -1057,8 +1057,47 @@ __device__ __forceinline__ void switch_body(
         // are done by all lanes together (coop_scan), one at a time.
         int cur = live ? c0 : -1;
         int kcur = live ? key_of(key, cur) : 0;
-        int e = live ? eb : 0;
-        const int e1 = live ? ee : 0;
+        // First the best candidate by key alone.  The exact order refines the
+        // key order, and the scan ends at the first candidate (current
+        // successor first) that is a maximum of the exact order, so its result
+        // has the best key: when no two candidates share that key (no tie
+        // against the best so far), the best by key is the result; otherwise
+        // the scan runs over just the candidates with the best key.
+        int best = cur, kbest = kcur;
+        bool exact = false;
+        {
+            int e = live ? eb : 0;
+            const int e1 = live ? ee : 0;
+            while (e < e1) {
+                const int cnt = min(kSwBatch, e1 - e);
+                int us[kSwBatch], ks[kSwBatch];
+#pragma unroll
+                for (int k = 0; k < kSwBatch; k++) us[k] = k < cnt ? tgt[e + k] : -1;
+                if (only_changed) {
+#pragma unroll
+                    for (int k = 0; k < kSwBatch; k++)
+                        if (us[k] >= 0 && !((cb[us[k] >> 6] >> (us[k] & 63)) & 1)) us[k] = -2;
+                }
+#pragma unroll
+                for (int k = 0; k < kSwBatch; k++) ks[k] = us[k] >= 0 ? key[us[k]] : 0;
+#pragma unroll
+                for (int k = 0; k < kSwBatch; k++) {
+                    if (us[k] < 0) continue;
+                    bool tie;
+                    const int c = cmp_fast(us[k], ks[k], best, kbest, &tie);
+                    if (player == 0 ? c > 0 : c < 0) {
+                        best = us[k];
+                        kbest = ks[k];
+                        exact = false;
+                    } else if (tie) {
+                        exact = true;
+                    }
+                }
+                e += kSwBatch;
+            }
+        }
+        int e = live && exact ? eb : 0;
+        const int e1 = live && exact ? ee : 0;
         while (__any(e < e1)) {
             const int cnt = e < e1 ? min(kSwBatch, e1 - e) : 0;
             int us[kSwBatch], ks[kSwBatch];
@@ -1071,6 +1110,12 @@ __device__ __forceinline__ void switch_body(
             }
 #pragma unroll
             for (int k = 0; k < kSwBatch; k++) ks[k] = us[k] >= 0 ? key[us[k]] : 0;
+#pragma unroll
+            for (int k = 0; k < kSwBatch; k++) {  // -3: worse key than the best, skipped
+                bool tie;
+                const int c = us[k] >= 0 ? cmp_fast(us[k], ks[k], best, kbest, &tie) : 0;
+                if (player == 0 ? c < 0 : c > 0) us[k] = -3;
+            }
             // One compare at a time (a rolled loop, so the exact-compare code
             // exists once; k is uniform, so us[k] and ks[k] are register reads).
 #pragma unroll 1
@@ -1101,6 +1146,8 @@ __device__ __forceinline__ void switch_body(
             }
             e += kSwBatch;
         }
+        if (live && !exact) cur = best;
+        sw = live && cur != c0;
         if (sw) strat[v] = cur;
         // one store per wave and game: the flag only has to become nonzero
         unsigned long long m = __ballot(sw);
