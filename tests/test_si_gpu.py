@@ -158,6 +158,35 @@ def test_state_loaded_after_solve(gpu, seed):
     s.close()
 
 
+def test_state_loaded_into_one_game_of_a_batch(gpu):
+    """Loading a state into one game of a solved batch restarts that game
+    alone: its valuation equals the oracle's, the other games keep their
+    results and counts."""
+    rng = np.random.default_rng(77)
+    games = [gpu.gen_random_game(810 + k, n, 40, 1, 4) for k, n in enumerate((900, 2500, 1300))]
+    b = gpu.Solver(games, trace=True)
+    b.solve()
+    before = b.results()
+    g = games[1]
+    n = g.n
+    strat = np.array([g.targets[rng.integers(g.offsets[v], g.offsets[v + 1])] for v in range(n)],
+                     np.int32)
+    solved = (rng.random(n) < 0.1).astype(np.int32)
+    lab_in = np.where(solved == 1, 1, rng.integers(-1, 2, n)).astype(np.int32)
+    inf, dig, depth = oracle.valuate(g, strat, solved, lab_in)
+    b.set_state(strat, solved, lab_in, game=1)
+    assert b.stats(1) == (0, 0, False)
+    b.compute_valuation()
+    assert np.array_equal(b.labels(1), inf)
+    assert np.array_equal(b.depths(1), depth)
+    assert np.array_equal(b.digests(1), dig)
+    for k in (0, 2):
+        assert b.stats(k) == before[k][0], k
+        assert np.array_equal(b.strategy(k), before[k][1]), k
+        assert np.array_equal(b.labels(k), before[k][2]), k
+    b.close()
+
+
 def test_stepwise_api_matches_solve(gpu):
     """Driving init_strat / compute_valuation / mark_solved / switch_strategy
     from the host (the reference's solve_si loop) gives the same result."""
