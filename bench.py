@@ -439,7 +439,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              f"(independent games would shard by game across GPUs, see DESIGN.md)",
+              f"(independent games would shard by game across GPUs, one process per GPU)",
               file=sys.stderr)
         sys.exit(2)
     import psi
