@@ -136,6 +136,7 @@ struct Core {
     int pargs_two = -1;                // two_levels in that copy
     hipGraphExec_t pgraph = nullptr;   // captured step-program launches
     int pgraph_launches = 0;
+    unsigned pgraph_grid = 0;          // workgroups per launch in that graph
 };
 
 // Launch shapes of the SI step kernels (grid-stride launches).

@@ -2405,6 +2405,15 @@ static int upload_step_args(Core& c, StepArgs** out)
     return PSI_OK;
 }
 
+// Workgroups of a step-program launch: at most the tuned grid, and at most
+// one per two 256-vertex blocks, so that on small games every wave of a
+// workgroup has a block to run (a workgroup's blocks are dealt to its four
+// waves) and fewer workgroups are dispatched per launch.
+static unsigned program_grid(const Core& c)
+{
+    return std::max(1u, std::min((nblocks(c.N, kBS) + 1) / 2, g_si_tune.phase_grid));
+}
+
 // Launches one graph of `launches` (a multiple of 4) step-program launches,
 // captured on first use.
 static int launch_program(Core& c, int launches)
@@ -2413,14 +2422,14 @@ static int launch_program(Core& c, int launches)
         StepArgs* a = nullptr;
         if (int rc = upload_step_args(c, &a)) return rc;
     }
-    if (!c.pgraph || c.pgraph_launches != launches) {
+    if (!c.pgraph || c.pgraph_launches != launches || c.pgraph_grid != program_grid(c)) {
         if (c.pgraph) {
             (void)hipGraphExecDestroy(c.pgraph);
             c.pgraph = nullptr;
         }
         StepArgs* a = nullptr;
         if (int rc = upload_step_args(c, &a)) return rc;
-        const unsigned grid = std::max(1u, std::min(nblocks(c.N, kBS), g_si_tune.phase_grid));
+        const unsigned grid = program_grid(c);
         hipGraph_t gr;
         std::lock_guard<std::recursive_mutex> lk(setup_mutex());
         PSI_HIP(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
@@ -2433,6 +2442,7 @@ static int launch_program(Core& c, int launches)
         (void)hipGraphDestroy(gr);
         if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
         c.pgraph_launches = launches;
+        c.pgraph_grid = grid;
     }
     PSI_HIP(hipGraphLaunch(c.pgraph, c.s));
     return PSI_OK;
@@ -2451,7 +2461,7 @@ int core_solve_program_profiled(Core& c, double* times, long long* counts)
     if (c.pool || !c.prog) return fail(PSI_ERR_INVALID, "step program: not available on this handle");
     StepArgs* a = nullptr;
     if (int rc = upload_step_args(c, &a)) return rc;
-    const unsigned grid = std::max(1u, std::min(nblocks(c.N, kBS), g_si_tune.phase_grid));
+    const unsigned grid = program_grid(c);
     hipEvent_t e0, e1;
     PSI_HIP(hipEventCreate(&e0));
     PSI_HIP(hipEventCreate(&e1));

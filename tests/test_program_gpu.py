@@ -196,3 +196,22 @@ def test_program_two_levels_long_cycles(modes):
         assert a[0][0][:2] == (r.iter, r.br)
         assert np.array_equal(a[0][1], r.strat) and np.array_equal(a[0][2], r.inf)
         _same(a, _run(gpu, g, 0))
+
+
+def test_program_grid_retuned_on_one_handle(modes):
+    """The step-program grid retuned between solves of one handle (the
+    captured launches are rebuilt with it): the same results every time."""
+    gpu = modes
+    g = gpu.gen_random_game(51, 60000, 4000, 1, 5)
+    r = oracle.solve(g)
+    gpu.si_tune(gpu.SIT_PROGRAM, 1)
+    s = gpu.Solver(g)
+    try:
+        for grid in (2048, 64, 100, 2048):        # 118, 64, 100, 118 workgroups
+            gpu.si_tune(gpu.SIT_PHASE_GRID, grid)
+            s.solve()
+            assert s.stats()[:2] == (r.iter, r.br), grid
+            assert np.array_equal(s.strategy(), r.strat) and np.array_equal(s.labels(), r.inf), grid
+    finally:
+        gpu.si_tune(gpu.SIT_PHASE_GRID, 2048)
+        s.close()
