@@ -49,6 +49,30 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+LARGE_FIXTURES = os.path.join(ROOT, "tests", "golden", "large")
+
+
+def load_fixture(name):
+    """Oracle fixture data (expected results written on the CPU by
+    tests/golden/make_large.py); None if absent."""
+    path = os.path.join(LARGE_FIXTURES, name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh)
+
+
+def si_parity(psi, fx, s, n, seed):
+    """The timed solve's result against the oracle's fixture of this game."""
+    if fx is None or fx["n"] != n or fx["gen"]["seed"] != seed:
+        return None
+    it, br, capped = s.stats()
+    return bool((it, br, capped) == (fx["iter"], fx["br"], False) and
+                str(psi.hash_i32(s.strategy())) == fx["strat"] and
+                str(psi.hash_i32(s.labels())) == fx["labels"] and
+                str(psi.hash_i32(s.solved())) == fx["solved"])
+
+
 def si_bench(psi, n, steps, warmup, seed):
     t0 = time.time()
     g = psi.gen_random_game(seed, n, 1 << 20, 2, 8)
@@ -57,7 +81,7 @@ def si_bench(psi, n, steps, warmup, seed):
     for _ in range(warmup):
         s.solve()
     s.sync()
-    timer = psi.Timer()
+    timer = psi.Timer(s)            # events on the handle's stream, where its kernels run
     brs = []
     t_wall = time.perf_counter()
     timer.start()
@@ -70,6 +94,8 @@ def si_bench(psi, n, steps, warmup, seed):
     dev_ms = timer.ms()
     it, br, capped = s.stats()
     log(f"[bench] {steps} solves: iter={it} br={br} wall={wall:.3f}s device={dev_ms:.1f}ms")
+    parity = si_parity(psi, load_fixture("cfg2_4M.json"), s, g.n, seed)
+    log(f"[bench] timed solve vs oracle fixture: {parity}")
     # Attribution: the loop launched kernel by kernel (k_jump etc., the
     # roofline's source), then the step program launch by launch (the phases
     # the timed solves ran).
@@ -79,7 +105,7 @@ def si_bench(psi, n, steps, warmup, seed):
     pprof = psi.solve_program_profiled(s)
     log("[bench] step-program phases ms/launches: " +
         ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in pprof.items() if v[1]))
-    return g, s, it, br, wall, prof, pprof, prounds
+    return g, s, it, br, wall, dev_ms, parity, prof, pprof, prounds
 
 
 def listrank_bench(psi, reps=10, tune=True):
@@ -220,6 +246,17 @@ def batch_bench(psi, n_games=1024, seed=3, lo=1000, hi=1000000, check_alone=True
     autotune.apply_si(autotune.SI_DEFAULT)        # later legs run with the defaults
     same = all(x[0] == y[0] and np.array_equal(x[1], y[1]) and np.array_equal(x[2], y[2])
                for x, y in zip(res_b, res_a))
+    # every game of the batch against the oracle's fixture of this set
+    fx = load_fixture("cfg3_batch.json")
+    vs_oracle, differing = None, None
+    if fx is not None and fx["gen"]["games"] == n_games and fx["gen"]["size_seed"] == seed:
+        exp = {r["k"]: r for r in fx["games"]}
+        differing = [k for k, (st_, s_, l_) in enumerate(res_b)
+                     if st_ != (exp[k]["iter"], exp[k]["br"], False) or
+                     str(psi.hash_i32(s_)) != exp[k]["strat"] or
+                     str(psi.hash_i32(l_)) != exp[k]["labels"]]
+        vs_oracle = not differing
+        log(f"[bench] batch vs oracle fixture: {len(res_b) - len(differing)}/{len(res_b)} equal")
     n_tot = int(sizes.sum())
     br = [x[0][1] for x in res_b]
     return {"games": n_games, "sum_n": n_tot, "sum_m": int(sum(g.m for g in games)),
@@ -229,6 +266,8 @@ def batch_bench(psi, n_games=1024, seed=3, lo=1000, hi=1000000, check_alone=True
             "speedup": round(t_alone / t_batch, 2), "br_max": int(max(br)),
             "vertex_iterations": int(sum(int(sz) * b_ for sz, b_ in zip(sizes, br))),
             "batched_equals_alone": bool(same),
+            "batched_equals_oracle": vs_oracle,
+            "games_differing_from_oracle": differing,
             "autotuned": {"config": tuned, "trials": len(trials), "tuning_s": round(tuning_s, 2),
                           "sample": "every 16th game of the batch"}}
 
@@ -274,10 +313,17 @@ def serve_bench(psi, requests=96, rate=2.0, seed=4, budget_gib=64.0, steps=4, lo
     pool.drain(steps)
     for f in futs[:min(lookahead, requests)]:
         f.result()
+    # requests with an oracle fixture (the stream's first requests, large
+    # chain games excepted) are checked against it; a few others are solved
+    # again alone and compared
+    fx = load_fixture("cfg4_stream.json")
+    expect = {}
+    if fx is not None and fx["gen"]["seed"] == seed and fx["gen"]["requests"] == 4096:
+        expect = {r["k"]: r for r in fx["requests"] if r["k"] < requests}
     order = np.argsort(sizes)
-    sample = set([int(k) for k in order if not chain[k]][:6] +
-                 [int(k) for k in order if chain[k]][:3] +
-                 [int(k) for k in order[::-1] if not chain[k]][:1])
+    sample = set([int(k) for k in order if chain[k] and k not in expect][:2] +
+                 [int(k) for k in order[::-1] if not chain[k] and k not in expect][:1])
+    vs_oracle = {"checked": 0, "equal": 0}
     kept, n_of, br_of, finish, ticket_of = {}, {}, {}, {}, {}
     sub, late, max_running, max_queued, n_steps = 0, 0, 0, 0, 0
     t0 = time.perf_counter()
@@ -306,6 +352,12 @@ def serve_bench(psi, requests=96, rate=2.0, seed=4, budget_gib=64.0, steps=4, lo
             br_of[k] = stats[1]
             if k in kept:
                 kept[k] = (kept[k], stats, st, lb)
+            if k in expect:
+                e = expect[k]
+                vs_oracle["checked"] += 1
+                vs_oracle["equal"] += int(stats == (e["iter"], e["br"], False) and
+                                          str(psi.hash_i32(st)) == e["strat"] and
+                                          str(psi.hash_i32(lb)) == e["labels"])
         info = pool.info()
         if tn - t_log > 10.0:
             t_log = tn
@@ -329,7 +381,10 @@ def serve_bench(psi, requests=96, rate=2.0, seed=4, budget_gib=64.0, steps=4, lo
     vit = sum(n_of[k] * br_of[k] for k in range(requests))
     lat_c = lat[chain]
     lat_r = lat[~chain]
-    return {"requests": requests, "rate_per_s": rate, "seed": seed, "chain_requests": int(chain.sum()),
+    return {"requests": requests, "requests_run": requests, "requests_in_stream": 4096,
+            "reduced": requests < 4096,
+            "prefix_note": f"the first {requests} requests of the seeded 4,096-request stream",
+            "rate_per_s": rate, "seed": seed, "chain_requests": int(chain.sum()),
             "sum_n": int(sizes.sum()), "budget_gib": budget_gib, "steps_per_sync": steps,
             "makespan_s": round(makespan, 3),
             "throughput_req_per_s": round(requests / makespan, 3),
@@ -343,7 +398,10 @@ def serve_bench(psi, requests=96, rate=2.0, seed=4, budget_gib=64.0, steps=4, lo
             "latency_p99_chain_s": round(float(np.percentile(lat_c, 99)), 4) if len(lat_c) else None,
             "br_max": int(max(br_of.values())), "max_running": max_running,
             "max_queued": max_queued, "late_generation": late, "pool_steps": n_steps,
-            "served_equals_alone": bool(same), "checked_alone": len(kept)}
+            "served_equals_alone": bool(same), "checked_alone": len(kept),
+            "served_equals_oracle": (vs_oracle["equal"] == vs_oracle["checked"]
+                                     if vs_oracle["checked"] else None),
+            "checked_vs_oracle": vs_oracle["checked"]}
 
 
 def bf_game(psi, n, seed):
@@ -396,15 +454,16 @@ def bf_bench(psi, n=1 << 20, seed=6, reps=3, cpu_n=1 << 14, cpu=True):
     return out
 
 
-def measured_traffic(n, seed):
-    """HBM bytes per active jump round of this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None for
-    other workloads or if the summary is absent."""
+def measured_traffic(n, seed, kernel):
+    """HBM bytes per launch of `kernel` on this workload (k_phase: the
+    average over one timed solve's launches; k_jump: a running round), from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated);
+    None for other workloads or if the summary is absent."""
     import re
     path = os.path.join(ROOT, "profiles", "si_solve_4M_hbm_bytes.md")
     if n != 1 << 22 or seed != 2 or not os.path.exists(path):
         return None
-    m = re.search(r"k_jump active launch HBM traffic: (\d+) bytes", open(path).read())
+    m = re.search(kernel + r" HBM traffic per launch: (\d+) bytes", open(path).read())
     return int(m.group(1)) if m else None
 
 
@@ -466,26 +525,49 @@ def main():
         return
     if a.only == "si":
         a.no_listrank = a.no_cpu = a.no_batch = a.no_serve = a.no_bf = True
-    g, s, it, br, wall, prof, pprof, prounds = si_bench(psi, a.n, a.steps, a.warmup, a.seed)
+    g, s, it, br, wall, dev_ms, parity, prof, pprof, prounds = si_bench(psi, a.n, a.steps,
+                                                                        a.warmup, a.seed)
     value = g.n * br * a.steps / wall
-    # Roofline of the dominant kernel: a pointer-jumping round reads and
-    # writes every vertex's 8-byte record (successor or terminal code and
-    # count, largest priority) and gathers the 8-byte record of each
-    # still-jumping vertex's successor.  Timed on k_jump, launched round by
-    # round, by each kernel's own start and end timestamps (as rocprofv3's
-    # kernel trace reads them); the timed solves run the same round code as
-    # the step program's jump phase (program_avg_launch_us: its launches,
-    # timed by stream events around each, so with the dispatch gap).
+    # Roofline of the dominant kernel, k_phase: the timed solves are graphs
+    # of k_phase launches only (the step program).  Algorithmic bytes per
+    # valuation round (one valuation + one switch pass over all n vertices):
+    # tour build 4n + 4(2n+2), tour rank 8(2n+2), key materialisation 12n,
+    # switch 4n + 4(n+1) + 4m (owner, offsets, targets) + 4m + 4n (key
+    # gathers) + 4n (strategy write): B_round = 56n + 8m + 28.  Per solve:
+    # br * B_round over the k_phase launches of one solve, each launch timed
+    # live by HIP events attached to its own dispatch on the handle's stream
+    # (its kernel start and end, as rocprofv3's kernel trace reads them).
+    phase_keys = [k for k in pprof if k not in ("total", "gathers", "ties", "scans", "scan_steps")]
+    k_launches = sum(pprof[k][1] for k in phase_keys)
+    k_ms = sum(pprof[k][0] for k in phase_keys)
+    b_round = 56 * g.n + 8 * g.m + 28
+    alg_solve = float(br) * b_round
+    achieved = alg_solve / (k_ms * 1e-3) / 1e9 if k_ms else 0.0
+    timed_gbs = alg_solve / (wall / a.steps) / 1e9
+    # the pointer-jumping round alone (k_jump launched round by round, the
+    # kernel-by-kernel solve): 16 B per vertex visited + 8 B per gather
     jump_ms, jump_launches = prof["jump"]
     gathers = prof["gathers"][1]
     jump_bytes = (16.0 * g.n * jump_launches + 8.0 * gathers) / max(jump_launches, 1)
-    achieved = jump_bytes / (jump_ms / jump_launches * 1e-3) / 1e9 if jump_launches else 0.0
-    roof = {"bound": "hbm", "kernel": "k_jump", "achieved": round(achieved, 1),
+    jump_gbs = jump_bytes / (jump_ms / jump_launches * 1e-3) / 1e9 if jump_launches else 0.0
+    roof = {"bound": "hbm", "kernel": "k_phase", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(g.n, a.seed), "alg_bytes_per_launch": round(jump_bytes),
-            "avg_launch_us": round(jump_ms / max(jump_launches, 1) * 1e3, 2),
-            "launches": jump_launches,
-            "program_avg_launch_us": round(pprof["jump"][0] / max(pprof["jump"][1], 1) * 1e3, 2),
+            "traffic": measured_traffic(g.n, a.seed, "k_phase"),
+            "alg_bytes_per_launch": round(alg_solve / max(k_launches, 1)),
+            "b_round_bytes": b_round, "alg_bytes_per_solve": round(alg_solve),
+            "launches_per_solve": k_launches,
+            "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 2),
+            "kernel_ms_per_solve": round(k_ms, 3),
+            "timed_region": {"ms_per_solve": round(wall / a.steps * 1e3, 3),
+                             "device_ms_per_solve": round(dev_ms / a.steps, 3),
+                             "achieved": round(timed_gbs, 1),
+                             "frac": round(timed_gbs / HBM_PEAK_GBS, 4)},
+            "jump_round": {"kernel": "k_jump", "achieved": round(jump_gbs, 1),
+                           "frac": round(jump_gbs / HBM_PEAK_GBS, 4),
+                           "alg_bytes_per_launch": round(jump_bytes),
+                           "avg_launch_us": round(jump_ms / max(jump_launches, 1) * 1e3, 2),
+                           "launches": jump_launches,
+                           "traffic": measured_traffic(g.n, a.seed, "k_jump")},
             "rounds_per_valuation": prounds}
     line = {
         "metric": "SI vertex-iterations/s on 4M-vertex game",
@@ -503,6 +585,9 @@ def main():
         "config": {"workload": "plain SI solve, random game n=4194304, priority U[0,2^20), "
                                "out-degree U[2,8], seed 2",
                    "n": g.n, "m": g.m, "iter": it, "br": br, "parallelism": "single GPU"},
+        "parity": parity,
+        "parity_check": "timed solve's iter, br and hashes of the final strategy, labels and "
+                        "solved flags equal the oracle's (tests/golden/large/cfg2_4M.json)",
         "roofline": roof,
         "kernels_ms": {k: {"ms": round(v[0], 3), "launches": v[1]} for k, v in prof.items()
                        if k != "gathers"},

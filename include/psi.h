@@ -50,9 +50,9 @@ int psi_event_elapsed_ms(void* start, void* end, float* ms);
 
 /* ---- list ranking ------------------------------------------------------- */
 /* Replaces the list-ranking core of GPUList::compute_valuation
- * (gpu_list_ranking.cu:322-369: d_pick_splitters :139, d_traverse_sublists
- * :196, process_reduced :240, d_broadcast_val :287) and of CPUList
- * (cpu_list_ranking.cpp:154-250) as a standalone entry with unit weights.
+ * (gpu_list_ranking.cu:279-326: d_pick_splitters :96, d_traverse_sublists
+ * :153, process_reduced :197, d_broadcast_val :244) and of CPUList
+ * (cpu_list_ranking.cpp:108-239) as a standalone entry with unit weights.
  * succ[i] in [-1, n) (-1 ends a list); heads[] holds the first node of every
  * list.  rank[i] = distance of i from its head, -1 if no head reaches i.  */
 #define PSI_LR_AUTO 0      /* WALK if mean list length n/n_heads is short, else HJ */
@@ -105,6 +105,11 @@ int psi_game_create(int n, int m, const int* owner, const int* priority, const i
 int psi_game_destroy(psi_game* g);
 int psi_game_info(const psi_game* g, int* n, int* m, int* max_pri);
 int psi_game_csr(const psi_game* g, int* owner, int* priority, int* offsets, int* targets);
+/* GPUGame's public device arrays (gpu_game.h:11-16, uploaded as
+ * gpu_game.cpp:11-41 does): player[n], priority[n], vertices[n+1] (offsets),
+ * edges[m] (targets), int32 device pointers owned by the game handle (made on
+ * the first call, freed by psi_game_destroy).  Pointers may be null. */
+int psi_game_device_csr(psi_game* g, int** player, int** priority, int** vertices, int** edges);
 /* Parses PGSolver text (Game::parse_pgsolver, game.cpp:51-107: header line
  * skipped; "id priority owner s1,s2,...[ label];" per vertex, ids in order
  * from 0).  Errors return PSI_ERR_INVALID with the reference's message. */
@@ -132,14 +137,19 @@ int psi_solver_create_games(int n_games, const int* n_per, const int64_t* m_per,
 int psi_solver_destroy(psi_solver* s);
 int psi_solver_info(psi_solver* s, int* n_games, int64_t* n_total, int64_t* m_total);
 int psi_solver_synchronize(psi_solver* s);
+/* Records an event (psi_event_create) on the handle's own stream, where its
+ * kernels run (timing; no reference counterpart). */
+int psi_solver_event_record(psi_solver* s, void* ev);
 
 /* StrategyImprovement (si.h:6-16), one call each, all games of the handle:
- *   init_strat              gpu_parity.cu:91-96, cpu_parity.cpp:317-327
- *   compute_valuation       gpu_parity.cu:191-227 (compute_first_val is the same)
- *   mark_solved(player)     cpu_parity.cpp:306-315 (GPUParity's is a no-op)
- *   switch_strategy(player) gpu_parity.cu:343-357: *switched = 1 if any vertex
+ *   init_strat              gpu_parity.cu:51-56, cpu_parity.cpp:276-286
+ *   compute_valuation       gpu_parity.cu:151-187, cpu_parity.cpp:116-133
+ *                           (compute_first_val is the same)
+ *   mark_solved(player)     cpu_parity.cpp:265-274 (GPUParity's is a no-op,
+ *                           gpu_parity.cu:333-336)
+ *   switch_strategy(player) gpu_parity.cu:303-317: *switched = 1 if any vertex
  *                           switched (the reference returns this 0/1 flag)
- *   reset_opponent_strategy cpu_parity.cpp:341-348 */
+ *   reset_opponent_strategy cpu_parity.cpp:300-307 */
 int psi_init_strat(psi_solver* s);
 int psi_compute_valuation(psi_solver* s);
 int psi_mark_solved(psi_solver* s, int player);
@@ -198,7 +208,8 @@ int psi_get_profile_rounds(psi_solver* s, int* out, int cap, int* count);
 int psi_solve_program_profiled(psi_solver* s, int reset, int max_rounds, double* times_ms,
                                int64_t* counts);
 
-/* Results of one game, local vertex ids.  get_winning: gpu_parity.cu:380-394. */
+/* Results of one game, local vertex ids.  get_winning: gpu_parity.cu:340-354,
+ * cpu_parity.cpp:288-297. */
 int psi_get_winning(psi_solver* s, int game, int player, int* out, int* len);
 int psi_get_strategy(psi_solver* s, int game, int* out);   /* -1 = sink */
 int psi_get_labels(psi_solver* s, int game, int* out);     /* infinite[]: -1, 0, +1 */
@@ -269,7 +280,7 @@ int psi_pool_info(psi_pool* p, int* running, int* queued, int64_t* vertex_cap, i
  * Grid sizes (blocks of 256 threads) of the SI step kernels, which all run
  * grid-stride loops; they apply to step graphs captured afterwards.  No
  * reference counterpart (the reference hard-codes its launch shapes,
- * gpu_parity.cu:10-14, gpu_list_ranking.cu:19). */
+ * gpu_util.h:8, gpu_util.cpp:19-22, gpu_list_ranking.cu:19). */
 #define PSI_SIT_JUMP_GRID 1  /* pointer-jumping rounds (default 2048)          */
 #define PSI_SIT_STEP_GRID 2  /* val_init / label / switch / mark (16384)       */
 #define PSI_SIT_POOL_GRID 3  /* the same kernels in a serving pool (4096)      */

@@ -6,26 +6,26 @@
 //
 // What it restates (reference file:line):
 //   * PGSolver parser ............ game.cpp:51-107 (grammar and error strings)
-//   * CPUParity semantics ........ cpu_parity.cpp:50-348
-//       - init_strat ............. cpu_parity.cpp:317-327
-//       - compute_valuation ...... cpu_parity.cpp:95-174 (recursive DFS, restated
+//   * CPUParity semantics ........ cpu_parity.cpp:9-307
+//       - init_strat ............. cpu_parity.cpp:276-286
+//       - compute_valuation ...... cpu_parity.cpp:54-133 (recursive DFS, restated
 //                                  here as an explicit stack; identical results)
-//       - even_cycle ............. cpu_parity.cpp:76-93
-//       - compare_valuation ...... cpu_parity.cpp:195-277
-//       - switch_strategy ........ cpu_parity.cpp:280-304
-//       - mark_solved ............ cpu_parity.cpp:306-315
-//       - get_winning ............ cpu_parity.cpp:329-338
-//       - reset_opponent_strategy  cpu_parity.cpp:341-348
+//       - even_cycle ............. cpu_parity.cpp:35-52
+//       - compare_valuation ...... cpu_parity.cpp:154-236
+//       - switch_strategy ........ cpu_parity.cpp:239-263
+//       - mark_solved ............ cpu_parity.cpp:265-274
+//       - get_winning ............ cpu_parity.cpp:288-297
+//       - reset_opponent_strategy  cpu_parity.cpp:300-307
 //   * solve_si driver loop ....... main.cpp:20-55 (== gpumain.cpp:18-53)
 //   * CPU_BellmanFord + solve_bf . cpu_bf.cpp:8-166, main.cpp:57-77 (dense rows,
 //                                  in-place relaxation in vertex order)
-//   * list ranking ............... the walk of cpu_list_ranking.cpp:197-250 with
+//   * list ranking ............... the walk of cpu_list_ranking.cpp:151-204 with
 //                                  unit weights is the sequential distance from
 //                                  the head; restated as a brute-force walk.
 //
 // Two exact valuation representations are provided:
 //   dense  : one int count per priority per vertex, as the reference stores it
-//            (cpu_parity.cpp:50-73).  Needs n*(max_pri+1) ints.
+//            (cpu_parity.cpp:9-32).  Needs n*(max_pri+1) ints.
 //   sparse : a finite vertex's valuation is the multiset of priorities on its
 //            strategy path to the sink; it is never materialised.  Two finite
 //            valuations are compared by walking both paths up to their lowest
@@ -191,20 +191,20 @@ struct Parity {
         if (dense) vals.assign((size_t)n * P, 0);
     }
 
-    // cpu_parity.cpp:317-327
+    // cpu_parity.cpp:276-286
     void init_strat()
     {
         for (int v = 0; v < n; v++) strat[v] = g.owner[v] == 0 ? -1 : g.tgt[g.off[v]];
     }
 
-    // cpu_parity.cpp:341-348
+    // cpu_parity.cpp:300-307
     void reset_opponent_strategy()
     {
         for (int v = 0; v < n; v++)
             if (g.owner[v] == 1) strat[v] = g.tgt[g.off[v]];
     }
 
-    // cpu_parity.cpp:76-93: parity of the largest priority on the strategy cycle
+    // cpu_parity.cpp:35-52: parity of the largest priority on the strategy cycle
     // through v (the running maximum starts at 0).
     bool even_cycle(int v) const
     {
@@ -241,7 +241,7 @@ struct Parity {
     }
 
     // Finishing step of the recursive compute_valuation(v) after the recursive
-    // call on next = strat[v] has returned (cpu_parity.cpp:139-155).
+    // call on next = strat[v] has returned (cpu_parity.cpp:98-114).
     void finish(int v)
     {
         int next = strat[v];
@@ -253,7 +253,7 @@ struct Parity {
         done[v] = 2;
     }
 
-    // compute_valuation(v) of cpu_parity.cpp:95-155 with the recursion replaced
+    // compute_valuation(v) of cpu_parity.cpp:54-114 with the recursion replaced
     // by an explicit stack of vertices whose done flag is 1.
     void valuate_from(int v0)
     {
@@ -261,17 +261,17 @@ struct Parity {
         int v = v0;
         for (;;) {
             bool leaf = true;
-            if (inf[v] == 1) {                      // cpu_parity.cpp:102-106
+            if (inf[v] == 1) {                      // cpu_parity.cpp:61-65
                 done[v] = 2;
-            } else if (strat[v] == -1) {            // cpu_parity.cpp:108-117
+            } else if (strat[v] == -1) {            // cpu_parity.cpp:67-76
                 set_sink_val(v);
                 done[v] = 2;
             } else {
                 int next = strat[v];
-                if (done[next] == 1) {              // cpu_parity.cpp:122-130
+                if (done[next] == 1) {              // cpu_parity.cpp:81-89
                     inf[v] = even_cycle(v) ? 1 : -1;
                     done[v] = 2;
-                } else if (done[next] == 0) {       // cpu_parity.cpp:133-137
+                } else if (done[next] == 0) {       // cpu_parity.cpp:92-96
                     done[v] = 1;
                     stack.push_back(v);
                     v = next;
@@ -290,7 +290,7 @@ struct Parity {
         }
     }
 
-    // cpu_parity.cpp:157-174
+    // cpu_parity.cpp:116-133
     void compute_valuation()
     {
         for (int v = 0; v < n; v++) {
@@ -306,7 +306,7 @@ struct Parity {
         if (!dense) vstrat = strat;
     }
 
-    // cpu_parity.cpp:195-233 (dense rows; row == nullptr is the zero vector)
+    // cpu_parity.cpp:154-192 (dense rows; row == nullptr is the zero vector)
     int compare_dense(const int* a, const int* b) const
     {
         int maxdiff = -1;
@@ -348,7 +348,7 @@ struct Parity {
         }
     }
 
-    // cpu_parity.cpp:235-277
+    // cpu_parity.cpp:194-236
     int compare(int v, int u)
     {
         if (v != -1 && u != -1 && inf[v] && inf[u]) {
@@ -364,7 +364,7 @@ struct Parity {
         return compare_sparse(v, u);
     }
 
-    // cpu_parity.cpp:280-304
+    // cpu_parity.cpp:239-263
     int switch_strategy(int player)
     {
         int total = 0;
@@ -384,7 +384,7 @@ struct Parity {
         return total;
     }
 
-    // cpu_parity.cpp:306-315
+    // cpu_parity.cpp:265-274
     void mark_solved(int player)
     {
         for (int v = 0; v < n; v++) {

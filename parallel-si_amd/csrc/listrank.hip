@@ -3,8 +3,8 @@
 //
 // Reference roles: the splitter pick, sublist walk, reduced-list walk and
 // broadcast of gpu_list_ranking.cu:96-110, :153-193, :197-240 (host),
-// :244-277 and cpu_list_ranking.cpp:154-250; the tree-Wyllie pointer jumping of
-// gpu_parity.cu:119-146.  The design here differs:
+// :244-277 and cpu_list_ranking.cpp:108-239; the tree-Wyllie pointer jumping of
+// gpu_parity.cu:79-106.  The design here differs:
 //   * splitters are one per aligned block of D node ids, at a hashed offset
 //     inside the block, so "is x a splitter" and "which sublist does it start"
 //     are pure arithmetic on x (no marker writes into succ, which stays

@@ -12,6 +12,11 @@ using namespace psi;
 struct psi_game {
     int n = 0, m = 0, max_pri = 0;
     std::vector<int> owner, priority, offsets, targets;
+    // Device copy of the CSR, made by psi_game_device_csr on first request.
+    int* d_owner = nullptr;
+    int* d_priority = nullptr;
+    int* d_offsets = nullptr;
+    int* d_targets = nullptr;
 };
 
 struct psi_solver {
@@ -51,7 +56,45 @@ extern "C" int psi_game_create(int n, int m, const int* owner, const int* priori
 
 extern "C" int psi_game_destroy(psi_game* g)
 {
+    if (!g) return PSI_OK;
+    if (g->d_owner) {
+        std::lock_guard<std::recursive_mutex> lk(setup_mutex());
+        for (int* p : {g->d_owner, g->d_priority, g->d_offsets, g->d_targets}) (void)hipFree(p);
+    }
     delete g;
+    return PSI_OK;
+}
+
+// GPUGame's device arrays (gpu_game.cpp:11-41: player, priority, vertices =
+// offsets, edges = targets, int32, uploaded synchronously).  Owned by the game
+// handle and freed with it (the reference's destructor frees only player and
+// priority, gpu_game.cpp:43-47).
+extern "C" int psi_game_device_csr(psi_game* g, int** player, int** priority, int** vertices,
+                                   int** edges)
+{
+    if (!g) return fail(PSI_ERR_INVALID, "psi_game_device_csr: null game");
+    if (!g->d_owner) {
+        std::lock_guard<std::recursive_mutex> lk(setup_mutex());
+        int* d[4] = {nullptr, nullptr, nullptr, nullptr};
+        const std::vector<int>* h[4] = {&g->owner, &g->priority, &g->offsets, &g->targets};
+        for (int i = 0; i < 4; i++) {
+            size_t bytes = h[i]->size() * sizeof(int);
+            hipError_t e = hipMalloc((void**)&d[i], bytes);
+            if (e == hipSuccess) e = hipMemcpy(d[i], h[i]->data(), bytes, hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                for (int k = 0; k <= i; k++) (void)hipFree(d[k]);
+                return hip_fail(e, "psi_game_device_csr", __FILE__, __LINE__);
+            }
+        }
+        g->d_owner = d[0];
+        g->d_priority = d[1];
+        g->d_offsets = d[2];
+        g->d_targets = d[3];
+    }
+    if (player) *player = g->d_owner;
+    if (priority) *priority = g->d_priority;
+    if (vertices) *vertices = g->d_offsets;
+    if (edges) *edges = g->d_targets;
     return PSI_OK;
 }
 
@@ -351,7 +394,7 @@ extern "C" int psi_get_trace(psi_solver* s, int game, uint64_t* hashes, int cap,
     return PSI_OK;
 }
 
-// get_winning(out, player) (cpu_parity.cpp:329-338): player 0 gets the +inf
+// get_winning(out, player) (cpu_parity.cpp:288-297, gpu_parity.cu:340-354): player 0 gets the +inf
 // vertices, player 1 every other vertex, ascending local ids.
 extern "C" int psi_get_winning(psi_solver* s, int game, int player, int* out, int* len)
 {
@@ -392,6 +435,14 @@ extern "C" int psi_set_state(psi_solver* s, int game, const int* strat, const in
 extern "C" int psi_solver_synchronize(psi_solver* s)
 {
     PSI_HIP(hipStreamSynchronize(s->core.s));
+    return PSI_OK;
+}
+
+// An event (psi_event_create) recorded on the handle's own stream, the one
+// its kernels run on.
+extern "C" int psi_solver_event_record(psi_solver* s, void* ev)
+{
+    PSI_HIP(hipEventRecord((hipEvent_t)ev, s->core.s));
     return PSI_OK;
 }
 

@@ -1,14 +1,14 @@
 // Device-resident strategy improvement with exact compact valuations.
 //
-// Reference semantics: CPUParity (cpu_parity.cpp:95-348) driven by solve_si
-// (gpumain.cpp:18-53).  Reference GPU roles: GPUParity (gpu_parity.cu:56-394):
-// d_init_strat :78, the tree-Wyllie valuation d_init_val/d_val_step/
-// d_mark_infinite :101-227, compare_valuation :231-302, d_switch_strategy
-// :304-357, d_mark_solved :361-376, get_winning :380-394.
+// Reference semantics: CPUParity (cpu_parity.cpp:54-307) driven by solve_si
+// (gpumain.cpp:18-53).  Reference GPU roles: GPUParity (gpu_parity.cu:16-354):
+// d_init_strat :38-56, the tree-Wyllie valuation d_init_val/d_val_step/
+// d_mark_infinite :61-187, compare_valuation :191-262, d_switch_strategy
+// :264-317, d_mark_solved :321-336, get_winning :340-354.
 //
 // Valuation.  A finite vertex's valuation is the multiset of priorities on its
 // strategy path to the sink.  Instead of the reference's dense count vector
-// per vertex (n * (max_pri+1) ints, gpu_parity.cu:66-73) each vertex keeps
+// per vertex (n * (max_pri+1) ints, gpu_parity.cu:25-29) each vertex keeps
 //   par  - its strategy successor at valuation time (the sink tree),
 //   dep  - the length of its path to the sink,
 //   key  - (label + 1) << 30 | the largest priority on that path,
@@ -18,10 +18,10 @@
 // resolves nothing proves that every still-jumping vertex leads into a cycle;
 // from then on the rounds only have to make the jump window cover the longest
 // cycle, after which the window maximum at the cycle vertex gives the cycle's
-// largest priority and so the label (cpu_parity.cpp:76-93, :122-130).
-// Solved vertices are terminals with their frozen label (cpu_parity.cpp:157-168).
+// largest priority and so the label (cpu_parity.cpp:35-52, :81-89).
+// Solved vertices are terminals with their frozen label (cpu_parity.cpp:116-127).
 //
-// Compare (total preorder of cpu_parity.cpp:195-277).  +inf > finite > -inf.
+// Compare (total preorder of cpu_parity.cpp:154-236).  +inf > finite > -inf.
 // Two finite valuations whose path maxima differ are ordered by that maximum
 // alone: the larger one appears only on one side, so it is the maxdiff
 // priority.  Equal maxima (a few percent of compares on random games) take the
@@ -174,7 +174,7 @@ __device__ __forceinline__ unsigned long long summary_word(int run, int stop, in
 }
 
 // ---------------------------------------------------------------------------
-// init_strat (gpu_parity.cu:78-96; cpu_parity.cpp:317-327)
+// init_strat (gpu_parity.cu:38-56; cpu_parity.cpp:276-286)
 __global__ void k_init_strat(long long N, const int* __restrict__ vinfo, const int* __restrict__ off,
                              const int* __restrict__ tgt, int* __restrict__ strat,
                              int8_t* __restrict__ solved, int8_t* __restrict__ lab)
@@ -206,7 +206,7 @@ __global__ void k_reset_ctl(int g0, int g1, GameCtl* gc)
     gc[g] = c;
 }
 
-// reset_opponent_strategy (cpu_parity.cpp:341-348)
+// reset_opponent_strategy (cpu_parity.cpp:300-307)
 __global__ void k_reset_opp(long long N, const int* __restrict__ vinfo, const int* __restrict__ off,
                             const int* __restrict__ tgt, int* __restrict__ strat)
 {
@@ -649,7 +649,7 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int2
 }
 
 // Labels (d_mark_infinite role), compare keys, trace hashes; in the loop form
-// also mark_solved(1) (cpu_parity.cpp:306-315), which solve_si calls right
+// also mark_solved(1) (cpu_parity.cpp:265-274), which solve_si calls right
 // after every valuation.
 //
 // check_rounds: flag the valuation if a round at lmax would still have run
@@ -907,7 +907,7 @@ __device__ __forceinline__ int wave_sum(int x)
 
 // Compares the priority multisets of u's first la and w's first lb path
 // vertices from the top down (compare_valuation's finite case,
-// cpu_parity.cpp:235-277): the largest priority whose counts differ decides,
+// cpu_parity.cpp:194-236): the largest priority whose counts differ decides,
 // even priorities counting for player 0.  Run by the whole wave: each lane
 // walks 1/64 of each segment (its start found through the lifting tables), and
 // each step of the top-down compare is a wave reduction.  Every lane returns
@@ -960,7 +960,7 @@ __device__ __forceinline__ int coop_scan(int u, int la, int w, int lb, const int
     }
 }
 
-// Sign of compare_valuation(u, w) (cpu_parity.cpp:235-277) from the two keys;
+// Sign of compare_valuation(u, w) (cpu_parity.cpp:194-236) from the two keys;
 // -1 is the sink (finite, empty path).  *tie: both finite with equal path
 // maxima (the exact compare decides).
 __device__ __forceinline__ int cmp_fast(int u, int ku, int w, int kw, bool* tie)
@@ -985,7 +985,7 @@ __device__ __forceinline__ int key_of(const int* __restrict__ key, int u)
     return u < 0 ? (1 << 30) : key[u];
 }
 
-// switch_strategy(player) (cpu_parity.cpp:280-304, gpu_parity.cu:304-357):
+// switch_strategy(player) (cpu_parity.cpp:239-263, gpu_parity.cu:264-317):
 // keep the current successor unless one is strictly better; the first
 // strictly-better-than-current edge in CSR order wins each step of the scan.
 // Loop form with player 0 also applies mark_solved(0) first.
@@ -1188,7 +1188,7 @@ __global__ void __launch_bounds__(kBS) k_switch(
 }
 
 // mark_solved(0) of the games that take the outer step (fused form of
-// cpu_parity.cpp:306-315 as solve_si calls it, gpumain.cpp:44).
+// cpu_parity.cpp:265-274 as solve_si calls it, gpumain.cpp:44).
 // solve_bf games after a relaxation also record succ: the chosen successor of
 // every unsolved Odd vertex with a finite valuation (cpu_bf.cpp:120-147).
 __device__ __forceinline__ void mark0_body(long long N, const int* __restrict__ gid,
@@ -1237,7 +1237,7 @@ __global__ void k_bf_final(long long N, const int* __restrict__ gid, const GameC
     if (gc[g].bf && gc[g].iter >= 2 && !gc[g].err && owner_of(vinfo[v]) == 1) strat[v] = succ[v];
 }
 
-// mark_solved(player) as a separate step (cpu_parity.cpp:306-315).
+// mark_solved(player) as a separate step (cpu_parity.cpp:265-274).
 __global__ void k_mark_solved(int player, long long N, const int8_t* __restrict__ lab,
                               int8_t* __restrict__ solved)
 {
@@ -1856,7 +1856,7 @@ __global__ void k_pool_targets(long long ebase, long long m, long long vbase, in
     if (e < m) tgt[ebase + e] += (int)vbase;
 }
 
-// init_strat of one region (gpu_parity.cu:78-96).
+// init_strat of one region (gpu_parity.cu:38-56).
 __global__ void k_pool_init(long long vbase, int n, const int* __restrict__ vinfo,
                             const int* __restrict__ off, const int* __restrict__ tgt,
                             int* __restrict__ strat, int8_t* __restrict__ solved,
@@ -2469,9 +2469,11 @@ int core_solve_program_profiled(Core& c, double* times, long long* counts)
     int rc = PSI_OK;
     for (long long i = 0; i < (1LL << 26); i++) {
         const int par = (int)(i & 1);
-        if (hipEventRecord(e0, c.s) != hipSuccess) { rc = fail(PSI_ERR_HIP, "hipEventRecord"); break; }
-        k_phase<<<grid, kBS, 0, c.s>>>((int)(i & 3), a, c.prog);
-        if (hipEventRecord(e1, c.s) != hipSuccess) { rc = fail(PSI_ERR_HIP, "hipEventRecord"); break; }
+        // events attached to the dispatch itself: the kernel's own start and
+        // end, as a profiler's kernel trace reads them
+        hipExtLaunchKernelGGL(k_phase, dim3(grid), dim3(kBS), 0, c.s, e0, e1, 0, (int)(i & 3), a,
+                              c.prog);
+        if (hipGetLastError() != hipSuccess) { rc = fail(PSI_ERR_HIP, "step program launch"); break; }
         int st = 0;
         if (hipMemcpyAsync(&st, &c.prog->st[par], sizeof st, hipMemcpyDeviceToHost, c.s) != hipSuccess ||
             hipStreamSynchronize(c.s) != hipSuccess) {

@@ -1,12 +1,11 @@
-// MI355X solvers behind the reference's StrategyImprovement interface:
+// MI355X solver behind the reference's StrategyImprovement interface:
 //   GPUParity(Game&, GPUGame&)  - the reference's gpu mode  (gpu_parity.h:8-38)
-//   GPUList(Game&, GPUGame&)    - the reference's gpulist mode (gpu_list_ranking.h:11-41)
-// Both compute CPUParity's semantics (cpu_parity.cpp), the reference's
+// (GPUList, the gpulist mode, is declared in gpu_list_ranking.h.)
+// It computes CPUParity's semantics (cpu_parity.cpp), the reference's
 // canonical CPU path, bit-exact: mark_solved freezes solved vertices (the
-// reference's GPUParity::mark_solved is a no-op, gpu_parity.cu:373-376) and
+// reference's GPUParity::mark_solved is a no-op, gpu_parity.cu:333-336) and
 // every vertex not reaching the sink is labelled by its cycle's largest
-// priority (the reference's GPUList labels them all +inf and can fail to
-// terminate, cpu_list_ranking.cpp:254-285).
+// priority.
 // switch_strategy returns 1 if any vertex switched (the reference's GPU
 // returns this flag too; its CPU returns a count; solve_si only tests 0).
 // solve() runs the whole solve_si loop on the device instead of from the host.
@@ -46,12 +45,6 @@ public:
     std::vector<int> strategy();
     std::vector<int> labels();
     psi_solver* handle() { return s; }
-};
-
-class GPUList : public GPUParity
-{
-public:
-    GPUList(Game& g, GPUGame& g_dev) : GPUParity(g, g_dev) {}
 };
 
 #endif

@@ -16,6 +16,7 @@
 #include "game.h"
 #include "gpu_game.h"
 #include "gpu_parity.h"
+#include "gpu_list_ranking.h"
 
 #include <zlib.h>
 

@@ -51,6 +51,7 @@ _sig("psi_memcpy_d2d", C.c_int, _vp, _vp, C.c_size_t)
 _sig("psi_event_create", C.c_int, C.POINTER(_vp))
 _sig("psi_event_destroy", C.c_int, _vp)
 _sig("psi_event_record", C.c_int, _vp)
+_sig("psi_solver_event_record", C.c_int, _vp, _vp)
 _sig("psi_event_elapsed_ms", C.c_int, _vp, _vp, C.POINTER(C.c_float))
 _sig("psi_list_rank", C.c_int, _vp, C.c_int, _vp, C.c_int, _vp, C.c_int)
 _sig("psi_list_rank_host", C.c_int, _i32p, C.c_int, _i32p, C.c_int, _i32p, C.c_int)
@@ -141,6 +142,25 @@ def gen_chain_game(seed, n):
     return Game(owner, pri, off, tgt[: m.value].copy())
 
 
+def _splitmix64(x):
+    """splitmix64 finaliser over a uint64 numpy array (wrapping arithmetic)."""
+    with np.errstate(over="ignore"):
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+
+def hash_i32(a):
+    """Order-independent hash of an int32 array, the one the fixtures and the
+    trace hashes use: sum over i of splitmix64(i << 32 | (uint32) a[i]),
+    mod 2^64."""
+    a = np.ascontiguousarray(a, np.int32)
+    i = np.arange(a.shape[0], dtype=np.uint64)
+    x = (i << np.uint64(32)) | a.view(np.uint32).astype(np.uint64)
+    return int(_splitmix64(x).sum(dtype=np.uint64))
+
+
 def gen_sizes(seed, count, lo, hi):
     out = np.empty(count, np.int32)
     check(lib.psi_gen_sizes(seed, count, float(lo), float(hi), out), "psi_gen_sizes")
@@ -198,18 +218,26 @@ class DeviceArray:
 
 
 class Timer:
-    """HIP events on the library stream."""
+    """HIP events on the library stream, or on a solver handle's own stream
+    (the stream its kernels run on) when one is given."""
 
-    def __init__(self):
+    def __init__(self, solver=None):
         self.a, self.b = _vp(), _vp()
+        self.solver = solver
         check(lib.psi_event_create(C.byref(self.a)), "event")
         check(lib.psi_event_create(C.byref(self.b)), "event")
 
+    def _record(self, ev):
+        if self.solver is not None:
+            check(lib.psi_solver_event_record(self.solver.h, ev), "record")
+        else:
+            check(lib.psi_event_record(ev), "record")
+
     def start(self):
-        check(lib.psi_event_record(self.a), "record")
+        self._record(self.a)
 
     def stop(self):
-        check(lib.psi_event_record(self.b), "record")
+        self._record(self.b)
 
     def ms(self):
         f = C.c_float()
