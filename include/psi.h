@@ -67,8 +67,23 @@ int psi_list_rank(const int* d_succ, int n, const int* d_heads, int n_heads, int
 /* Host pointers; synchronous; always CHECKED; validates ranges. */
 int psi_list_rank_host(const int* succ, int n, const int* heads, int n_heads, int* rank,
                        int method);
-/* Device error bits of the last call (1: index out of range, 2: a walk did not
- * end) and the number of overflow sublists the level-0 walk created. */
+/* psi_list_rank with no allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * methods and results as psi_list_rank; its workspace and control words are
+ * a fixed array of 64 MiB per GPU, statically allocated in the library and
+ * held until the process ends, instead of psi_list_rank's grown workspace.
+ * Inputs whose workspace would exceed it (e.g. PSI_LR_WYLLIE beyond ~2.7M
+ * nodes; HJ handles the 16,777,216-node list and the 1,048,576-list forest)
+ * are rejected with PSI_ERR_INVALID and a message naming the bytes needed.
+ * Every call on a GPU uses that GPU's one array: calls that overlap on one
+ * GPU (other host threads, other streams) race.  Calls no hipMalloc/hipFree;
+ * psi_init resolves the array's device address (else the first call). */
+int psi_list_rank_noalloc(const int* d_succ, int n, const int* d_heads, int n_heads, int* d_rank,
+                          int method);
+/* Device error bits of the last call of either entry point (1: index out of
+ * range, 2: a walk did not end) and the number of overflow sublists the
+ * level-0 walk created. */
 int psi_list_rank_status(int* error_bits, int* overflow_used);
 
 /* Euler-tour path sums over a forest of links: the valuation engine of

@@ -45,6 +45,9 @@ inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / 
 
 // Library stream (default priority, non-blocking w.r.t. the null stream).
 hipStream_t stream();
+// Device address of the current GPU's fixed list-ranking workspace (the
+// array psi_list_rank_noalloc uses), resolved once per device.
+int lr_scratch_address(void** out);
 
 // Grows *buf to at least `bytes`; contents are not preserved.
 int ensure(void** buf, size_t* cap, size_t bytes);

@@ -363,13 +363,23 @@ __global__ void final0(int N, int LB, int R1, int* __restrict__ rank, const int*
 // ---------------------------------------------------------------------------
 // Host orchestration.
 
+static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Device workspace carved front to back.  A take beyond cap (a planning
+// mistake) sets `over` and returns the start of the buffer, so no kernel
+// touches memory outside it; the caller then reports an error.
 struct Arena {
     void* buf = nullptr;
     size_t cap = 0;
     size_t used = 0;
+    bool over = false;
     template <class T> T* take(long long n)
     {
-        size_t bytes = ((size_t)std::max(n, 1LL) * sizeof(T) + 255) & ~(size_t)255;
+        size_t bytes = al256((size_t)std::max(n, 1LL) * sizeof(T));
+        if (used + bytes > cap) {
+            over = true;
+            return (T*)buf;
+        }
         T* p = (T*)((char*)buf + used);
         used += bytes;
         return p;
@@ -391,12 +401,14 @@ struct Plan {
     bool top_wyllie = false;
 };
 
-static size_t level_bytes(long long R) { return 6 * (((size_t)R * 4 + 255) & ~(size_t)255); }
+static size_t level_bytes(long long R) { return 6 * al256((size_t)std::max(R, 1LL) * 4); }
 
-Arena g_arena;
-int* g_ctl = nullptr;    // 2 ints per level + 64 Wyllie round counters, device
+Arena g_arena;           // psi_list_rank's workspace, grown with hipMalloc
+int* g_ctl = nullptr;    // its control words (2 ints per level + 64 Wyllie round counters)
+int* g_last_ctl = nullptr;   // control words of the last call (psi_list_rank_status)
 constexpr int kMaxLevels = 8;
 constexpr int kCtlInts = kMaxLevels * 2 + 64 * 64 + 1;
+constexpr size_t kCtlBytes = (kCtlInts * sizeof(int) + 255) & ~(size_t)255;
 
 static int ensure_ctl()
 {
@@ -405,8 +417,21 @@ static int ensure_ctl()
     return PSI_OK;
 }
 
+// Workspace rank_level takes below a level of R nodes and H heads (the
+// level's own arrays excluded), following the same choices it makes.
+static size_t rank_level_bytes(long long R, long long H, int k)
+{
+    const Tuning& tu = g_tune;
+    if (R <= kTopCap) return 0;
+    if (H * 4 >= R || k + 1 >= kMaxLevels) return al256((size_t)R * 4) + 2 * al256((size_t)R * 8);
+    long long B = (R + (1LL << tu.log_dk) - 1) >> tu.log_dk;
+    long long R2 = B + H + R / tu.hop_cap_k + 1;
+    return level_bytes(R2) + rank_level_bytes(R2, H, k + 1);
+}
+
 // Ranks the reduced level lv[k] (offsets into lv[k].rok/roff), recursively.
-static int rank_level(Plan& pl, int k, hipStream_t s)
+// Workspace from `ar`, control words from ctl0.
+static int rank_level(Plan& pl, int k, Arena& ar, int* ctl0, hipStream_t s)
 {
     Level& L = pl.lv[k];
     const Tuning& tu = g_tune;
@@ -418,10 +443,11 @@ static int rank_level(Plan& pl, int k, hipStream_t s)
     bool heads_dominate = (long long)L.H * 4 >= L.R || k + 1 >= kMaxLevels;
     if (heads_dominate) {
         // Global Wyllie on this level.
-        int* pred = g_arena.take<int>(L.R);
-        int2* ps0 = g_arena.take<int2>(L.R);
-        int2* ps1 = g_arena.take<int2>(L.R);
-        int* active = g_ctl + kMaxLevels * 2;
+        int* pred = ar.take<int>(L.R);
+        int2* ps0 = ar.take<int2>(L.R);
+        int2* ps1 = ar.take<int2>(L.R);
+        if (ar.over) return fail(PSI_ERR_INVALID, "list ranking: workspace plan exceeded");
+        int* active = ctl0 + kMaxLevels * 2;
         int rounds = ceil_log2(L.R) + 1;
         if (int rc = fill<int>(pred, L.R, -1, s)) return rc;
         if (int rc = fill<int>(active, kWyRounds * kWySlots + 1, 0, s)) return rc;
@@ -443,13 +469,14 @@ static int rank_level(Plan& pl, int k, hipStream_t s)
     nx.R = B + L.H + over;
     nx.hb = B;
     nx.H = L.H;
-    nx.rnext = g_arena.take<int>(nx.R);
-    nx.w = g_arena.take<int>(nx.R);
-    nx.rid = g_arena.take<int>(nx.R);
-    nx.loc = g_arena.take<int>(nx.R);
-    nx.rok = g_arena.take<int>(nx.R);
-    nx.roff = g_arena.take<int>(nx.R);
-    int* ctl = g_ctl + 2 * (k + 1);
+    nx.rnext = ar.take<int>(nx.R);
+    nx.w = ar.take<int>(nx.R);
+    nx.rid = ar.take<int>(nx.R);
+    nx.loc = ar.take<int>(nx.R);
+    nx.rok = ar.take<int>(nx.R);
+    nx.roff = ar.take<int>(nx.R);
+    if (ar.over) return fail(PSI_ERR_INVALID, "list ranking: workspace plan exceeded");
+    int* ctl = ctl0 + 2 * (k + 1);
     if (int rc = fill<int>(ctl, 2, 0, s)) return rc;
     if (int rc = fill<int>(nx.rnext, nx.R, -2, s)) return rc;
     if (int rc = fill<int>(nx.w, nx.R, 0, s)) return rc;
@@ -460,7 +487,7 @@ static int rank_level(Plan& pl, int k, hipStream_t s)
                                                  over);
     PSI_LAUNCH_CHECK();
     pl.lv.push_back(nx);
-    if (int rc = rank_level(pl, k + 1, s)) return rc;
+    if (int rc = rank_level(pl, k + 1, ar, ctl0, s)) return rc;
     Level& L2 = pl.lv[k];
     Level& N2 = pl.lv[k + 1];
     prop_kernel<<<nblocks(L2.R, 256), 256, 0, s>>>(L2.R, L2.rid, L2.loc, N2.rok, N2.roff, L2.rok,
@@ -469,16 +496,60 @@ static int rank_level(Plan& pl, int k, hipStream_t s)
     return PSI_OK;
 }
 
-static int wyllie0(const int* succ, int N, int* rank, hipStream_t s)
+// Workspace of each method for N nodes and H heads (what the runs below take).
+static size_t wyllie0_bytes(long long N) { return 2 * al256((size_t)N * 4) + 2 * al256((size_t)N * 8); }
+
+struct Hj0Shape {
+    int B0 = 0, LB = 0, over0 = 0;
+    long long R1 = 0;
+};
+
+static int hj0_shape(int N, int H, bool blocks, Hj0Shape* sh)
 {
-    size_t need = (((size_t)N * 4 + 255) & ~(size_t)255) * 2 + 2 * (((size_t)N * 8 + 255) & ~(size_t)255);
-    if (int rc = ensure(&g_arena.buf, &g_arena.cap, need)) return rc;
-    g_arena.used = 0;
-    int* pred = g_arena.take<int>(N);
-    int* rok = g_arena.take<int>(N);
-    int2* ps0 = g_arena.take<int2>(N);
-    int2* ps1 = g_arena.take<int2>(N);
-    int* active = g_ctl + kMaxLevels * 2;
+    const Tuning& tu = g_tune;
+    sh->B0 = blocks ? (int)((N + (1LL << tu.log_d0) - 1) >> tu.log_d0) : 0;
+    int LB = tu.local_bits;
+    for (;;) {
+        sh->over0 = (int)(N >> LB) + 1;
+        sh->R1 = (long long)sh->B0 + H + sh->over0;
+        if (sh->R1 < (1LL << (31 - LB))) break;
+        if (--LB < 4) return fail(PSI_ERR_INVALID, "list ranking: too many heads/sublists to pack");
+    }
+    sh->LB = LB;
+    return PSI_OK;
+}
+
+static size_t hj0_bytes(const Hj0Shape& sh, int H)
+{
+    return level_bytes(sh.R1) + rank_level_bytes(sh.R1, H, 0);
+}
+
+// Sizes `ar` for `need` bytes: grows it (hipMalloc) when `grow`, else fails
+// if its fixed capacity is smaller.
+static int arena_prepare(Arena& ar, size_t need, bool grow)
+{
+    if (grow) {
+        if (int rc = ensure(&ar.buf, &ar.cap, need)) return rc;
+    } else if (need > ar.cap) {
+        char msg[160];
+        snprintf(msg, sizeof msg, "list ranking: needs %zu bytes of workspace, the fixed "
+                 "workspace holds %zu", need, ar.cap);
+        return fail(PSI_ERR_INVALID, msg);
+    }
+    ar.used = 0;
+    ar.over = false;
+    return PSI_OK;
+}
+
+static int wyllie0(const int* succ, int N, int* rank, Arena& ar, int* ctl0, bool grow, hipStream_t s)
+{
+    if (int rc = arena_prepare(ar, wyllie0_bytes(N), grow)) return rc;
+    int* pred = ar.take<int>(N);
+    int* rok = ar.take<int>(N);
+    int2* ps0 = ar.take<int2>(N);
+    int2* ps1 = ar.take<int2>(N);
+    if (ar.over) return fail(PSI_ERR_INVALID, "list ranking: workspace plan exceeded");
+    int* active = ctl0 + kMaxLevels * 2;
     int rounds = ceil_log2(N) + 1;
     if (rounds > 64) rounds = 64;
     if (int rc = fill<int>(pred, N, -1, s)) return rc;
@@ -494,45 +565,35 @@ static int wyllie0(const int* succ, int N, int* rank, hipStream_t s)
 }
 
 static int hj0(const int* succ, int N, const int* heads, int H, int* rank, bool blocks, bool checked,
-               hipStream_t s)
+               Arena& ar, int* ctl0, bool grow, hipStream_t s)
 {
     const Tuning& tu = g_tune;
+    Hj0Shape sh;
+    if (int rc = hj0_shape(N, H, blocks, &sh)) return rc;
+    if (int rc = arena_prepare(ar, hj0_bytes(sh, H), grow)) return rc;
     Plan pl;
     pl.N = N;
     pl.H = H;
     pl.blocks0 = blocks;
-    int log_d = tu.log_d0;
-    pl.B0 = blocks ? (int)((N + (1LL << log_d) - 1) >> log_d) : 0;
-    int LB = tu.local_bits;
-    long long R1;
-    for (;;) {
-        pl.over0 = (int)(N >> LB) + 1;
-        R1 = (long long)pl.B0 + H + pl.over0;
-        if (R1 < (1LL << (31 - LB))) break;
-        if (--LB < 4) return fail(PSI_ERR_INVALID, "list ranking: too many heads/sublists to pack");
-    }
-    pl.LB = LB;
-    // Worst-case arena: a geometric series of reduced levels.
-    size_t need = 0;
-    long long R = R1;
-    for (int k = 0; k < kMaxLevels && R > 0; k++) {
-        need += level_bytes(R) + 3 * (((size_t)R * 8 + 255) & ~(size_t)255) + 4096;
-        R = ((R + (1LL << tu.log_dk) - 1) >> tu.log_dk) + H + R / tu.hop_cap_k + 1;
-    }
-    if (int rc = ensure(&g_arena.buf, &g_arena.cap, need)) return rc;
-    g_arena.used = 0;
+    const int log_d = tu.log_d0;
+    pl.B0 = sh.B0;
+    pl.over0 = sh.over0;
+    pl.LB = sh.LB;
+    const int LB = sh.LB;
+    const long long R1 = sh.R1;
     Level L1;
     L1.R = (int)R1;
     L1.hb = pl.B0;
     L1.H = H;
-    L1.rnext = g_arena.take<int>(R1);
-    L1.w = g_arena.take<int>(R1);
-    L1.rid = g_arena.take<int>(R1);
-    L1.loc = g_arena.take<int>(R1);
-    L1.rok = g_arena.take<int>(R1);
-    L1.roff = g_arena.take<int>(R1);
+    L1.rnext = ar.take<int>(R1);
+    L1.w = ar.take<int>(R1);
+    L1.rid = ar.take<int>(R1);
+    L1.loc = ar.take<int>(R1);
+    L1.rok = ar.take<int>(R1);
+    L1.roff = ar.take<int>(R1);
+    if (ar.over) return fail(PSI_ERR_INVALID, "list ranking: workspace plan exceeded");
     pl.lv.push_back(L1);
-    int* ctl = g_ctl;
+    int* ctl = ctl0;
     if (int rc = fill<int>(ctl, 2, 0, s)) return rc;
     if (int rc = fill<int>(L1.rnext, R1, -2, s)) return rc;
     if (int rc = fill<int>(L1.w, R1, 0, s)) return rc;
@@ -543,10 +604,34 @@ static int hj0(const int* succ, int N, const int* heads, int H, int* rank, bool 
             succ, N, heads, H, pl.B0, log_d, tu.salt, LB, rank, L1.rnext, L1.w, ctl, pl.B0 + H, pl.over0);
         PSI_LAUNCH_CHECK();
     }
-    if (int rc = rank_level(pl, 0, s)) return rc;
+    if (int rc = rank_level(pl, 0, ar, ctl0, s)) return rc;
     final0<<<nblocks(N, 256), 256, 0, s>>>(N, LB, (int)R1, rank, pl.lv[0].rok, pl.lv[0].roff);
     PSI_LAUNCH_CHECK();
     return PSI_OK;
+}
+
+// One list ranking on stream s with workspace `ar` and control words ctl0
+// (kCtlInts ints); the method dispatch of psi_list_rank.
+static int run(const int* d_succ, int n, const int* d_heads, int n_heads, int* d_rank, int method,
+               Arena& ar, int* ctl0, bool grow, hipStream_t s)
+{
+    if (n < 0 || n_heads < 0) return fail(PSI_ERR_INVALID, "psi_list_rank: negative size");
+    if (n == 0) return PSI_OK;
+    int flags = method & ~0xff;
+    method &= 0xff;
+    bool checked = (flags & PSI_LR_CHECKED) != 0;
+    g_last_ctl = ctl0;
+    if (int rc = fill<int>(ctl0, kMaxLevels * 2, 0, s)) return rc;
+    if (method == PSI_LR_AUTO) {
+        long long mean = n_heads ? (long long)n / n_heads : (long long)n;
+        method = mean <= g_tune.walk_mean_max ? PSI_LR_WALK : PSI_LR_HJ;
+    }
+    switch (method) {
+    case PSI_LR_HJ: return hj0(d_succ, n, d_heads, n_heads, d_rank, true, checked, ar, ctl0, grow, s);
+    case PSI_LR_WALK: return hj0(d_succ, n, d_heads, n_heads, d_rank, false, checked, ar, ctl0, grow, s);
+    case PSI_LR_WYLLIE: return wyllie0(d_succ, n, d_rank, ar, ctl0, grow, s);
+    default: return fail(PSI_ERR_INVALID, "psi_list_rank: unknown method");
+    }
 }
 
 } // namespace lr
@@ -560,34 +645,77 @@ extern "C" int psi_list_rank(const int* d_succ, int n, const int* d_heads, int n
     if (n < 0 || n_heads < 0) return fail(PSI_ERR_INVALID, "psi_list_rank: negative size");
     if (n == 0) return PSI_OK;
     if (int rc = lr::ensure_ctl()) return rc;
-    hipStream_t s = stream();
-    int flags = method & ~0xff;
-    method &= 0xff;
-    bool checked = (flags & PSI_LR_CHECKED) != 0;
-    if (int rc = lr::fill<int>(lr::g_ctl, lr::kMaxLevels * 2, 0, s)) return rc;
-    if (method == PSI_LR_AUTO) {
-        long long mean = n_heads ? (long long)n / n_heads : (long long)n;
-        method = mean <= lr::g_tune.walk_mean_max ? PSI_LR_WALK : PSI_LR_HJ;
-    }
-    switch (method) {
-    case PSI_LR_HJ: return lr::hj0(d_succ, n, d_heads, n_heads, d_rank, true, checked, s);
-    case PSI_LR_WALK: return lr::hj0(d_succ, n, d_heads, n_heads, d_rank, false, checked, s);
-    case PSI_LR_WYLLIE: return lr::wyllie0(d_succ, n, d_rank, s);
-    default: return fail(PSI_ERR_INVALID, "psi_list_rank: unknown method");
-    }
+    return lr::run(d_succ, n, d_heads, n_heads, d_rank, method, lr::g_arena, lr::g_ctl, true,
+                   stream());
 }
 
-// Reads and clears the device error bits of the last list ranking.
+// ---- the same list ranking with a fixed, statically allocated workspace ----
+// The control words and the arena of psi_list_rank live here instead of in
+// hipMalloc'd memory: one array per GPU, held until the process ends.
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices];   // device address of `scratch` on each GPU
+}
+
+namespace psi {
+// Device address of the current GPU's scratch array (resolved once per
+// device, with that device current).  The array's name is only a host-side
+// stand-in; kernels and fills get this address.
+int lr_scratch_address(void** out)
+{
+    int dev = 0;
+    PSI_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(PSI_ERR_INVALID, "list ranking: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        g_scratch_dev[dev] = p;
+    }
+    if (out) *out = g_scratch_dev[dev];
+    return PSI_OK;
+}
+} // namespace psi
+
+extern "C" int psi_list_rank_noalloc(const int* d_succ, int n, const int* d_heads, int n_heads,
+                                     int* d_rank, int method)
+{
+    if (n < 0 || n_heads < 0) return fail(PSI_ERR_INVALID, "psi_list_rank_noalloc: negative size");
+    if (n == 0) return PSI_OK;
+    void* base = nullptr;
+    if (int rc = lr_scratch_address(&base)) return rc;
+    lr::Arena ar;
+    ar.buf = (char*)base + lr::kCtlBytes;
+    ar.cap = kScratchBytes - lr::kCtlBytes;
+    int rc = lr::run(d_succ, n, d_heads, n_heads, d_rank, method, ar, (int*)base, false, stream());
+    if (rc == PSI_ERR_INVALID && strstr(psi_last_error(), "fixed workspace")) {
+        std::string msg = std::string("psi_list_rank_noalloc: ") + psi_last_error() +
+                          " (its maximum; use psi_list_rank for this input)";
+        return fail(PSI_ERR_INVALID, msg);
+    }
+    return rc;
+}
+
+// Reads the device error bits of the last list ranking (either entry point).
 extern "C" int psi_list_rank_status(int* error_bits, int* overflow_used)
 {
-    if (int rc = lr::ensure_ctl()) return rc;
+    int* ctl = lr::g_last_ctl;
+    if (!ctl) {
+        if (int rc = lr::ensure_ctl()) return rc;
+        ctl = lr::g_ctl;
+        PSI_HIP(hipMemsetAsync(ctl, 0, lr::kMaxLevels * 2 * sizeof(int), stream()));
+    }
     int h[2];
-    PSI_HIP(hipMemcpyAsync(h, lr::g_ctl, sizeof h, hipMemcpyDeviceToHost, stream()));
+    PSI_HIP(hipMemcpyAsync(h, ctl, sizeof h, hipMemcpyDeviceToHost, stream()));
     PSI_HIP(hipStreamSynchronize(stream()));
     if (overflow_used) *overflow_used = h[0];
     int bits = 0;
     std::vector<int> all(lr::kMaxLevels * 2);
-    PSI_HIP(hipMemcpyAsync(all.data(), lr::g_ctl, all.size() * sizeof(int), hipMemcpyDeviceToHost, stream()));
+    PSI_HIP(hipMemcpyAsync(all.data(), ctl, all.size() * sizeof(int), hipMemcpyDeviceToHost, stream()));
     PSI_HIP(hipStreamSynchronize(stream()));
     for (int k = 0; k < lr::kMaxLevels; k++) bits |= all[2 * k + 1];
     if (error_bits) *error_bits = bits;

@@ -82,7 +82,9 @@ extern "C" int psi_init(void)
     if (c < 1) return psi::fail(PSI_ERR_NO_DEVICE, "no HIP device");
     psi::stream();
     if (g_stream_err != hipSuccess) return psi::hip_fail(g_stream_err, "hipStreamCreate", __FILE__, __LINE__);
-    return PSI_OK;
+    // the fixed list-ranking workspace of the current device, resolved here
+    // so psi_list_rank_noalloc makes no set-up call of its own
+    return psi::lr_scratch_address(nullptr);
 }
 
 extern "C" int psi_synchronize(void)
