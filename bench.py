@@ -468,13 +468,33 @@ def measured_traffic(n, seed, kernel):
 
 
 def cpu_baseline(g, max_rounds):
+    """CPUParity (the reference's cpu mode, single-threaded) restated with
+    exact sparse valuations, on the first max_rounds valuation rounds of the
+    same game, or the whole solve when max_rounds is 0."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     r = oracle.solve(g, rep=oracle.SPARSE, max_rounds=max_rounds)
+    what = (f"first {r.br} valuation rounds of the same game" if r.capped else
+            f"the whole solve of the same game (iter {r.iter}, br {r.br})")
     return {"value": round(g.n * r.br / r.seconds, 1), "unit": "vertex-iterations/s", "cores": 1,
-            "kind": "port",
-            "sample": f"first {r.br} valuation rounds of the same game (oracle sparse CPUParity "
-                      f"restatement, single thread, {r.seconds:.1f}s, {cpu_model()})"}
+            "kind": "port", "seconds": round(r.seconds, 2), "rounds": r.br,
+            "sample": f"{what} (oracle sparse CPUParity restatement, single thread, "
+                      f"{r.seconds:.1f}s, {cpu_model()})"}
+
+
+def cpu_baseline_threaded(g, max_rounds, threads):
+    """CPUList (the reference's cpulist mode: Euler tour + Helman-JaJa
+    valuation, every phase on `threads` std::threads) restated with exact
+    sparse valuations, on the first max_rounds rounds of the same game.  Its
+    labels follow cpulist's rule (a vertex not reaching the sink is +inf), so
+    its trajectory differs from CPUParity's after the first round."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    r = oracle.solve_list(g, threads=threads, max_rounds=max_rounds)
+    return {"value": round(g.n * r.br / r.seconds, 1), "unit": "vertex-iterations/s",
+            "cores": threads, "kind": "port", "seconds": round(r.seconds, 2), "rounds": r.br,
+            "sample": f"first {r.br} valuation rounds of the same game (oracle CPUList restatement, "
+                      f"{threads} threads, {r.seconds:.1f}s, {cpu_model()})"}
 
 
 def main():
@@ -484,7 +504,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1 << 22)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--cpu-rounds", type=int, default=6)
+    ap.add_argument("--cpu-rounds", type=int, default=6,
+                    help="valuation rounds of the CPU baselines' bounded sample")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="time the whole CPUParity solve for cpu_baseline (minutes)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPUList baseline (default: min(16, cores))")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-listrank", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -493,7 +518,8 @@ def main():
     ap.add_argument("--no-bf", action="store_true")
     ap.add_argument("--requests", type=int, default=96, help="serving: requests")
     ap.add_argument("--rate", type=float, default=2.0, help="serving: arrivals per second")
-    ap.add_argument("--only", choices=["si", "listrank", "batch", "serve", "bf", "copy"], default=None,
+    ap.add_argument("--only", choices=["si", "listrank", "batch", "serve", "bf", "copy", "cpu"],
+                    default=None,
                     help="run one workload only (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -525,6 +551,8 @@ def main():
         return
     if a.only == "si":
         a.no_listrank = a.no_cpu = a.no_batch = a.no_serve = a.no_bf = True
+    if a.only == "cpu":            # the SI leg and the CPU baselines beside it
+        a.no_listrank = a.no_batch = a.no_serve = a.no_bf = True
     g, s, it, br, wall, dev_ms, parity, prof, pprof, prounds = si_bench(psi, a.n, a.steps,
                                                                         a.warmup, a.seed)
     value = g.n * br * a.steps / wall
@@ -608,7 +636,11 @@ def main():
         line["serve"] = serve_bench(psi, a.requests, a.rate)
     if not a.no_cpu:
         log("[bench] cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline(g, a.cpu_rounds)
+        line["cpu_baseline"] = cpu_baseline(g, 0 if a.cpu_full else a.cpu_rounds)
+        log(f"[bench] cpu baseline: {line['cpu_baseline']['sample']}")
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        line["cpu_baseline_threaded"] = cpu_baseline_threaded(g, a.cpu_rounds, threads)
+        log(f"[bench] threaded cpu baseline: {line['cpu_baseline_threaded']['sample']}")
     print(json.dumps(line), flush=True)
 
 

@@ -190,6 +190,15 @@ int psi_solve(psi_solver* s, int reset, int max_rounds, int steps_per_sync);
  * relaxations converge; Odd's succ equals it where the minimum successor is
  * unique; br counts device valuation rounds, not the reference's sweeps. */
 int psi_solve_bf(psi_solver* s, int max_rounds, int steps_per_sync);
+/* solve_si with CPUBV<ArrayVal> (the CPU command line's cpubv mode,
+ * main.cpp:163-167; CPUBV::compute_valuation, cpu_bv.cpp:49-106): the
+ * canonical solve, except that a game stops at the first valuation that finds
+ * a strategy cycle whose largest priority is odd, where the reference prints
+ * "Game contains an odd cycle that uses only odd nodes" and exits
+ * (cpu_bv.cpp:73-84).  Such games report psi_get_bf_status = 1 and the call
+ * returns PSI_ERR_INVALID with that message; iter/br then count what was
+ * completed before the stop.  Other games end with CPUParity's result. */
+int psi_solve_bv(psi_solver* s, int max_rounds, int steps_per_sync);
 int psi_get_bf_status(psi_solver* s, int game, int* status);   /* 1 = odd cycle */
 int psi_get_stats(psi_solver* s, int game, int* iter, int* br, int* capped);
 /* Profiling (no reference counterpart; the reference times solve_si as a

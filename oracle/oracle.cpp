@@ -48,12 +48,14 @@
 //                   labels, iter and br counts) on the committed fixtures
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -425,6 +427,243 @@ void record(Parity& s, Trace* t, std::vector<uint64_t>& dig)
 }
 
 // ---------------------------------------------------------------------------
+// CPUList<ArrayVal> (cpu_list_ranking.cpp:15-276), the reference's threaded
+// CPU path: each valuation is an Euler tour of the strategy forest ranked by
+// Helman-JaJa, every phase forked onto T std::threads and joined as launch()
+// does (:40-71).  Semantics (cpulist): a vertex whose strategy reaches the
+// sink is finite, every other vertex +inf (broadcast_val, :208-239); solved
+// flags are kept (CPUBV::mark_solved, cpu_bv.cpp:222-231) but the valuation
+// never reads them.  Values are carried as (path length, digest) pairs, both
+// additive like the reference's count vectors (+ at an up node, - at a down
+// node), so the sublist walk, reduced-list prefix and broadcast are the
+// reference's; compares use the exact sparse form on the valuation-time
+// strategy tree.  The O(P^2) copy in update_val (:138-143) is not repeated.
+struct ListSolver {
+    Parity s;
+    int T, n, l, S;
+    std::vector<int> succ, vert_list, scratch, split_succ, split_inf;
+    std::vector<int> dep_scr, split_dep, split_dep_sum;
+    std::vector<uint64_t> dig_scr, split_dig, split_dig_sum;
+
+    ListSolver(const Game& g, int threads) : s(g, false), T(std::max(1, threads))
+    {
+        n = g.n;
+        l = 2 * n + 2;
+        S = (l - 1) / 128 + 1;                      // cpu_list_ranking.cpp:21
+        succ.assign(l, 0);
+        vert_list.assign(n + 1, -1);
+        scratch.assign(S, 0);
+        split_succ.assign(S, 0);
+        split_inf.assign(S, 1);
+        dep_scr.assign(n, 0);
+        dig_scr.assign(n, 0);
+        split_dep.assign(S, 0);
+        split_dep_sum.assign(S, 0);
+        split_dig.assign(S, 0);
+        split_dig_sum.assign(S, 0);
+    }
+
+    // cpu_list_ranking.cpp:40-71: T threads over [0, count) in equal ranges,
+    // the last taking the remainder; serial when count < T.
+    template <class F> void launch(F f, int count)
+    {
+        if (count < T) {
+            if (count > 0) f(0, count - 1);
+            return;
+        }
+        std::vector<std::thread> th;
+        th.reserve(T);
+        for (int i = 0; i < T; i++) {
+            int start = i * (count / T);
+            int end = i == T - 1 ? count - 1 : (i + 1) * (count / T) - 1;
+            th.emplace_back(f, start, end);
+        }
+        for (auto& t : th) t.join();
+    }
+
+    // :74-85: up(i) = 2i+1 -> down(i) = 2i; the sink is vertex n, its down ends the list
+    void init_list(int a, int b)
+    {
+        for (int i = a; i <= b; i++) {
+            succ[2 * i + 1] = 2 * i;
+            if (i == n) succ[2 * n] = -1;
+            else vert_list[i] = -1;
+        }
+    }
+
+    // :87-106: push up(i) onto the child list of its strategy successor
+    void build_list(int a, int b)
+    {
+        for (int i = a; i <= b; i++) {
+            int next = s.strat[i];
+            int next_up = next == -1 ? 2 * n + 1 : 2 * next + 1;
+            int my_up = 2 * i + 1;
+            succ[2 * i] = __atomic_exchange_n(&succ[next_up], my_up, __ATOMIC_RELAXED);
+        }
+    }
+
+    // :108-121: evenly spaced splitters, the last one at the head up(sink);
+    // integer positions (the reference's float product picks the same kind of
+    // spread; which nodes are splitters does not change any valuation)
+    void pick_splitters(int a, int b)
+    {
+        for (int i = a; i <= b; i++) {
+            int pos = i == S - 1 ? l - 1 : (int)((long long)(l - 1) * i / S);
+            scratch[i] = succ[pos];
+            succ[pos] = -100 - i;
+        }
+    }
+
+    // :123-170: walk each sublist, carrying the running value
+    void traverse_sublists(int a, int b)
+    {
+        for (int idx = a; idx <= b; idx++) {
+            int cur = scratch[idx];
+            int cd = 0;
+            uint64_t cg = 0;
+            while (cur >= 0) {
+                int v = cur / 2;
+                if (v != n) {                        // the sink has no priority (:127-128)
+                    if (cur & 1) {                   // up: add, and record at v
+                        cd += 1;
+                        cg += pri_weight(s.g.pri[v]);
+                        dep_scr[v] = cd;
+                        dig_scr[v] = cg;
+                        vert_list[v] = idx;
+                    } else {                         // down: remove
+                        cd -= 1;
+                        cg -= pri_weight(s.g.pri[v]);
+                    }
+                }
+                cur = succ[cur];
+            }
+            split_dep[idx] = cd;
+            split_dig[idx] = cg;
+            split_succ[idx] = cur;
+        }
+    }
+
+    // :172-204: serial walk of the reduced list from the head's sublist
+    void process_reduced()
+    {
+        std::fill(split_inf.begin(), split_inf.end(), 1);
+        std::fill(split_dep_sum.begin(), split_dep_sum.end(), 0);
+        std::fill(split_dig_sum.begin(), split_dig_sum.end(), 0);
+        int cur = S - 1, d = 0;
+        uint64_t g = 0;
+        while (cur != -1) {
+            split_dep_sum[cur] = d;
+            split_dig_sum[cur] = g;
+            d += split_dep[cur];
+            g += split_dig[cur];
+            split_inf[cur] = 0;
+            int next = split_succ[cur];
+            if (next == -1) break;
+            cur = -(next + 100);
+        }
+    }
+
+    // :208-239
+    void broadcast_val(int a, int b)
+    {
+        for (int v = a; v <= b; v++) {
+            if (s.inf[v] == -1) continue;
+            int k = vert_list[v];
+            if (k == -1) { s.inf[v] = 1; continue; }
+            if (split_inf[k] == 0) {
+                s.depth[v] = split_dep_sum[k] + dep_scr[v];
+                s.digest[v] = split_dig_sum[k] + dig_scr[v];
+                s.inf[v] = 0;
+            } else {
+                s.inf[v] = 1;
+            }
+        }
+    }
+
+    // :242-255
+    void compute_valuation()
+    {
+        launch([this](int a, int b) { init_list(a, b); }, n + 1);
+        launch([this](int a, int b) { build_list(a, b); }, n);
+        launch([this](int a, int b) { pick_splitters(a, b); }, S);
+        launch([this](int a, int b) { traverse_sublists(a, b); }, S);
+        process_reduced();
+        launch([this](int a, int b) { broadcast_val(a, b); }, n);
+        s.vstrat = s.strat;
+    }
+
+    // Exact compare of two finite valuations (private path parts), with
+    // per-thread scratch.
+    int compare_sparse(int u, int w, std::vector<int>& A, std::vector<int>& B) const
+    {
+        if (u == w) return 0;
+        A.clear();
+        B.clear();
+        int a = u, b = w;
+        int da = a < 0 ? 0 : s.depth[a], db = b < 0 ? 0 : s.depth[b];
+        while (da > db) { A.push_back(s.g.pri[a]); a = s.vstrat[a]; da--; }
+        while (db > da) { B.push_back(s.g.pri[b]); b = s.vstrat[b]; db--; }
+        while (a != b) {
+            A.push_back(s.g.pri[a]); a = s.vstrat[a];
+            B.push_back(s.g.pri[b]); b = s.vstrat[b];
+        }
+        std::sort(A.begin(), A.end(), std::greater<int>());
+        std::sort(B.begin(), B.end(), std::greater<int>());
+        for (size_t i = 0;; i++) {
+            int x = i < A.size() ? A[i] : -1, y = i < B.size() ? B[i] : -1;
+            if (x == -1 && y == -1) return 0;
+            if (x != y) {
+                if (x > y) return x % 2 == 0 ? 1 : -1;
+                return y % 2 == 0 ? -1 : 1;
+            }
+        }
+    }
+
+    // CPUBV::compare_valuation (cpu_bv.cpp:144-186)
+    int compare(int v, int u, std::vector<int>& A, std::vector<int>& B) const
+    {
+        const std::vector<int>& inf = s.inf;
+        if (v != -1 && u != -1 && inf[v] && inf[u]) {
+            if (inf[v] == 1 && inf[u] == -1) return 1;
+            if (inf[v] == -1 && inf[u] == 1) return -1;
+            return 0;
+        }
+        if ((v != -1 && inf[v]) && (u == -1 || !inf[u])) return inf[v] == 1 ? 1 : -1;
+        if ((u != -1 && inf[u]) && (v == -1 || !inf[v])) return inf[u] == 1 ? -1 : 1;
+        return compare_sparse(v, u, A, B);
+    }
+
+    // CPUList::switch_strategy (:262-276) over CPUBV::switch_vertex
+    // (cpu_bv.cpp:188-207); per-thread counts are added after the join.
+    int switch_strategy(int player)
+    {
+        std::vector<long long> part(T, 0);
+        std::atomic<int> slot{0};
+        launch([&](int a, int b) {
+            std::vector<int> A, B;
+            long long total = 0;
+            for (int v = a; v <= b; v++) {
+                if (s.g.owner[v] != player) continue;
+                int cur = s.strat[v];
+                for (int e = s.g.off[v]; e < s.g.off[v + 1]; e++) {
+                    int u = s.g.tgt[e];
+                    int c = compare(u, cur, A, B);
+                    if ((player == 0 && c == 1) || (player == 1 && c == -1)) {
+                        s.strat[v] = u;
+                        cur = u;
+                        total++;
+                    }
+                }
+            }
+            part[slot.fetch_add(1) % T] += total;
+        }, n);
+        long long t = 0;
+        for (long long x : part) t += x;
+        return (int)std::min<long long>(t, 1 << 30);
+    }
+};
+
+// ---------------------------------------------------------------------------
 // CPU_BellmanFord<ArrayVal> (cpu_bf.cpp:8-166) driven by solve_bf (main.cpp:57-77),
 // on top of CPUBV<ArrayVal> (cpu_bv.cpp), whose init_strat / compute_valuation /
 // compare / switch / mark_solved are CPUParity's (the Parity struct, dense
@@ -669,6 +908,95 @@ int orc_solve(int n, const int* off, const int* tgt, const int* owner, const int
     if (iter_out) *iter_out = iter;
     if (br_out) *br_out = br;
     if (n_rounds) *n_rounds = tp ? tr.count : br;
+    return capped;
+}
+
+// solve_si (main.cpp:20-55, the cpulist mode of main.cpp:178-191) with the
+// CPUList restatement on `threads` threads.  Returns 1 if max_rounds was
+// reached (cpulist need not terminate: on games where Odd closes an odd
+// cycle its labels oscillate).  Outputs and traces as orc_solve; seconds =
+// the solve region.
+int orc_solve_list(int n, const int* off, const int* tgt, const int* owner, const int* pri,
+                   int threads, int max_rounds, int* strat_out, int* inf_out, int* solved_out,
+                   int* iter_out, int* br_out, uint64_t* hashes, int trace_cap, int* n_rounds,
+                   double* seconds)
+{
+    Game g;
+    if (load_csr(n, off, tgt, owner, pri, g)) return -1;
+    ListSolver L(g, threads);
+    Parity& s = L.s;
+    Trace tr{hashes, nullptr, nullptr, nullptr, trace_cap, 0};
+    Trace* tp = hashes ? &tr : nullptr;
+    std::vector<uint64_t> dig(n);
+    int br = 0, iter = 0, capped = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    s.init_strat();                                  // CPUBV::init_strat (cpu_bv.cpp:233-243)
+    for (;;) {
+        for (;;) {
+            if (max_rounds > 0 && br >= max_rounds) { capped = 1; break; }
+            L.compute_valuation();                   // compute_first_val is the same (:257-260)
+            record(s, tp, dig);
+            br++;
+            s.mark_solved(1);
+            if (L.switch_strategy(1) == 0) break;
+        }
+        if (capped) break;
+        s.mark_solved(0);
+        iter++;
+        if (L.switch_strategy(0) == 0) break;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (strat_out) std::copy(s.strat.begin(), s.strat.end(), strat_out);
+    if (inf_out) std::copy(s.inf.begin(), s.inf.end(), inf_out);
+    if (solved_out) std::copy(s.solved.begin(), s.solved.end(), solved_out);
+    if (iter_out) *iter_out = iter;
+    if (br_out) *br_out = br;
+    if (n_rounds) *n_rounds = tp ? tr.count : br;
+    return capped;
+}
+
+// solve_si (main.cpp:20-55, the cpubv mode of main.cpp:163-167) with
+// CPUBV<ArrayVal>: CPUParity's init_strat, valuation, compare, switch and
+// mark_solved (cpu_bv.cpp:30-254 restate them for a templated value type),
+// except that compute_valuation stops the program when a strategy cycle's
+// largest priority is odd (cpu_bv.cpp:73-84: "Game contains an odd cycle that
+// uses only odd nodes", exit(1)).  status: 0 completed, 1 stopped there; then
+// iter and br count the outer iterations and valuations completed before the
+// stopping one (the reference prints neither).  Completed games are CPUParity
+// solves in which no -1 label ever appears.
+int orc_solve_bv(int n, const int* off, const int* tgt, const int* owner, const int* pri,
+                 int rep, int max_rounds, int* strat_out, int* inf_out, int* solved_out,
+                 int* iter_out, int* br_out, int* status_out)
+{
+    Game g;
+    if (load_csr(n, off, tgt, owner, pri, g)) return -1;
+    if (rep == 0 && (double)n * (g.max_pri + 1) > 4.0e9) { g_err = "dense valuation too large"; return -1; }
+    Parity s(g, rep == 0);
+    int br = 0, iter = 0, status = 0, capped = 0;
+    s.init_strat();
+    for (;;) {
+        for (;;) {
+            if (max_rounds > 0 && br >= max_rounds) { capped = 1; break; }
+            s.compute_valuation();
+            for (int v = 0; v < n && !status; v++)
+                if (!s.solved[v] && s.inf[v] == -1) status = 1;     // cpu_bv.cpp:79-83
+            if (status) break;
+            br++;
+            s.mark_solved(1);
+            if (s.switch_strategy(1) == 0) break;
+        }
+        if (capped || status) break;
+        s.mark_solved(0);
+        iter++;
+        if (s.switch_strategy(0) == 0) break;
+    }
+    if (strat_out) std::copy(s.strat.begin(), s.strat.end(), strat_out);
+    if (inf_out) std::copy(s.inf.begin(), s.inf.end(), inf_out);
+    if (solved_out) std::copy(s.solved.begin(), s.solved.end(), solved_out);
+    if (iter_out) *iter_out = iter;
+    if (br_out) *br_out = br;
+    if (status_out) *status_out = status;
     return capped;
 }
 

@@ -200,6 +200,69 @@ def zielonka(game):
     return win
 
 
+lib.orc_solve_list.restype = C.c_int
+lib.orc_solve_list.argtypes = [C.c_int, _i32p, _i32p, _i32p, _i32p, C.c_int, C.c_int, _i32p,
+                               _i32p, _i32p, C.POINTER(C.c_int), C.POINTER(C.c_int), _vp, C.c_int,
+                               C.POINTER(C.c_int), C.POINTER(C.c_double)]
+
+
+def solve_list(game, threads=8, max_rounds=0, trace=False):
+    """solve_si with the CPUList restatement (the cpulist mode: Euler tour +
+    Helman-JaJa valuation on `threads` threads; vertices not reaching the
+    sink are +inf).  Returns SolveResult with strat, inf, solved, iter, br,
+    seconds, capped and (if trace) hashes[br, 4]."""
+    g = game
+    n = g.n
+    strat = np.empty(n, np.int32)
+    inf = np.empty(n, np.int32)
+    solved = np.empty(n, np.int32)
+    it, br, nr = C.c_int(), C.c_int(), C.c_int()
+    secs = C.c_double()
+    cap = max_rounds if max_rounds else 4096
+    hashes = np.zeros(4 * cap, np.uint64) if trace else None
+    rc = lib.orc_solve_list(n, g.offsets, g.targets, g.owner, g.priority, threads, max_rounds, strat,
+                            inf, solved, C.byref(it), C.byref(br),
+                            hashes.ctypes.data_as(_vp) if trace else None, cap, C.byref(nr),
+                            C.byref(secs))
+    if rc < 0:
+        raise _err("solve_list")
+    r = SolveResult()
+    r.strat, r.inf, r.solved = strat, inf, solved
+    r.iter, r.br, r.seconds, r.capped = it.value, br.value, secs.value, rc == 1
+    r.hashes = hashes.reshape(-1, 4)[:min(nr.value, cap)] if trace else None
+    return r
+
+
+lib.orc_solve_bv.restype = C.c_int
+lib.orc_solve_bv.argtypes = [C.c_int, _i32p, _i32p, _i32p, _i32p, C.c_int, C.c_int, _i32p, _i32p,
+                             _i32p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+BV_OK, BV_ODD_CYCLE = 0, 1
+
+
+def solve_bv(game, rep=SPARSE, max_rounds=0):
+    """solve_si with CPUBV<ArrayVal> (the cpubv mode): CPUParity's solve that
+    stops at the first valuation with an odd strategy cycle.  Returns an
+    object with strat, inf, solved, iter, br, status (BV_OK / BV_ODD_CYCLE),
+    capped."""
+    g = game
+    n = g.n
+    strat = np.empty(n, np.int32)
+    inf = np.empty(n, np.int32)
+    solved = np.empty(n, np.int32)
+    it, br, st = C.c_int(), C.c_int(), C.c_int()
+    rc = lib.orc_solve_bv(n, g.offsets, g.targets, g.owner, g.priority, rep, max_rounds, strat, inf,
+                          solved, C.byref(it), C.byref(br), C.byref(st))
+    if rc < 0:
+        raise _err("solve_bv")
+
+    class R:
+        pass
+    r = R()
+    r.strat, r.inf, r.solved = strat, inf, solved
+    r.iter, r.br, r.status, r.capped = it.value, br.value, st.value, rc == 1
+    return r
+
+
 lib.orc_solve_bf.restype = C.c_int
 lib.orc_solve_bf.argtypes = [C.c_int, _i32p, _i32p, _i32p, _i32p, C.c_int, _i32p, _i32p, _i32p,
                              _u64p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]

@@ -242,6 +242,27 @@ extern "C" int psi_solve_bf(psi_solver* s, int max_rounds, int steps_per_sync)
     return PSI_OK;
 }
 
+// solve_si with CPUBV semantics (the cpubv mode, main.cpp:163-167): the
+// canonical solve, in which a game stops at the first valuation that finds
+// a strategy cycle whose largest priority is odd (cpu_bv.cpp:73-84 prints
+// the message and exits).
+extern "C" int psi_solve_bv(psi_solver* s, int max_rounds, int steps_per_sync)
+{
+    Core& c = s->core;
+    if (int rc = core_init_strat(c)) return rc;
+    if (int rc = core_set_params(c, 0, max_rounds, 2)) return rc;
+    if (steps_per_sync <= 0) steps_per_sync = 4;
+    if (int rc = core_solve(c, steps_per_sync)) return rc;
+    std::vector<GameCtl> gc(c.G);
+    PSI_HIP(hipMemcpyAsync(gc.data(), c.gctl, c.G * sizeof(GameCtl), hipMemcpyDeviceToHost, c.s));
+    PSI_HIP(hipStreamSynchronize(c.s));
+    for (int g = 0; g < c.G; g++)
+        if (gc[g].err)
+            return fail(PSI_ERR_INVALID, "game " + std::to_string(g) +
+                                             ": Game contains an odd cycle that uses only odd nodes");
+    return PSI_OK;
+}
+
 extern "C" int psi_get_bf_status(psi_solver* s, int game, int* status)
 {
     if (int rc = check_game(s, game)) return rc;

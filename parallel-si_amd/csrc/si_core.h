@@ -48,9 +48,11 @@ struct GameCtl {
     int vbase;      // first global vertex id of the game
     int n;          // vertices of the game
     int clear0;     // the last step was an outer step: restart the switch(0) change record
-    int bf;         // solve_bf (main.cpp:57-77): later best responses by relaxation
+    int bf;         // 1: solve_bf (main.cpp:57-77), later best responses by relaxation;
+                    // 2: CPUBV's solve_si (main.cpp:163-167), every valuation stops on an
+                    // odd strategy cycle
     int bf_init;    // solve_bf: the first best response (an SI inner loop) is done
-    int err;        // solve_bf: odd strategy cycle in the first best response (cpu_bv.cpp:73-84)
+    int err;        // odd strategy cycle: the game stopped there (cpu_bv.cpp:73-84)
     int activate;   // serving pool, step program: admitted (done until the next step boundary)
 };
 

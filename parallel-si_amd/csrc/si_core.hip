@@ -764,11 +764,13 @@ __device__ __forceinline__ void label_body(
                                 else if (kind == 2) l = -1;
                                 else if (kind < 0) {           // largest priority on the cycle
                                     l = (cyc_pri & 1) ? -1 : 1;
-                                    // solve_bf: the relaxation never closes a cycle (such
-                                    // vertices stay infinite, cpu_bf.cpp:156-160); the
-                                    // first best response stops on an odd one.
+                                    // solve_bf (bf 1): the relaxation never closes a cycle
+                                    // (such vertices stay infinite, cpu_bf.cpp:156-160);
+                                    // the first best response stops on an odd one.
+                                    // CPUBV solve (bf 2): every valuation stops on one
+                                    // (cpu_bv.cpp:73-84).
                                     if (bf[q]) {
-                                        if (bfi[q]) l = 1;
+                                        if (bfi[q] && bf[q] == 1) l = 1;
                                         else if (l == -1) gc[g[q]].err = 1;
                                     }
                                 } else {
@@ -1203,7 +1205,7 @@ __device__ __forceinline__ void mark0_body(long long N, const int* __restrict__ 
         int g = gid[v];
         if (gc[g].done || !gc[g].outer) continue;
         const int l = lab[v];
-        if (succ && gc[g].bf && gc[g].bf_init && l == 0 && !solved[v] && owner_of(vinfo[v]) == 1)
+        if (succ && gc[g].bf == 1 && gc[g].bf_init && l == 0 && !solved[v] && owner_of(vinfo[v]) == 1)
             succ[v] = strat[v];
         if (l == 1) solved[v] = 1;
     }
@@ -2122,7 +2124,7 @@ int core_set_params(Core& c, int reset, int max_rounds, int bf)
         x.max_rounds = max_rounds;
         x.bf = bf;
     }
-    if (bf && !c.succ) {
+    if (bf == 1 && !c.succ) {
         if (int rc = dalloc(&c.succ, c.N)) return rc;
         // captured launches hold the (null) succ they were captured with
         std::lock_guard<std::recursive_mutex> lk(setup_mutex());
@@ -2131,7 +2133,7 @@ int core_set_params(Core& c, int reset, int max_rounds, int bf)
         c.graph = nullptr;
         c.pgraph = nullptr;
     }
-    if (bf) k_succ_init<<<GRID(c.N)>>>(c.N, c.gid, c.gctl, c.succ);
+    if (bf == 1) k_succ_init<<<GRID(c.N)>>>(c.N, c.gid, c.gctl, c.succ);
     PSI_HIP(hipMemcpyAsync(c.gctl, gc.data(), c.G * sizeof(GameCtl), hipMemcpyHostToDevice, c.s));
     PSI_HIP(hipStreamSynchronize(c.s));
     return PSI_OK;

@@ -53,6 +53,13 @@ void GPUParity::solve_bf(int* iter_count, int* br_count)
     check(psi_get_stats(s, 0, iter_count, br_count, &capped), "solve_bf");
 }
 
+void GPUParity::solve_bv(int* iter_count, int* br_count)
+{
+    check(psi_solve_bv(s, 0, 4), "solve_bv");
+    int capped = 0;
+    check(psi_get_stats(s, 0, iter_count, br_count, &capped), "solve_bv");
+}
+
 std::vector<int> GPUParity::strategy()
 {
     std::vector<int> out(g_dev.num_vert);

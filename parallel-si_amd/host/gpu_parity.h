@@ -42,6 +42,10 @@ public:
     // best responses for Odd.  Throws on an odd strategy cycle in the first
     // best response (the reference prints the message and exits).
     void solve_bf(int* iter_count, int* br_count);
+    // Device-resident solve_si with CPUBV semantics (main.cpp:163-167, the
+    // cpubv mode): throws on the first odd strategy cycle, where the
+    // reference prints the message and exits (cpu_bv.cpp:73-84).
+    void solve_bv(int* iter_count, int* br_count);
     std::vector<int> strategy();
     std::vector<int> labels();
     psi_solver* handle() { return s; }

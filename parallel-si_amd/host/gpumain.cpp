@@ -1,6 +1,6 @@
 // gpuparsi: the reference's GPU command line (gpumain.cpp:102-146) on MI355X.
 //
-//   gpuparsi [options] <gpu|gpulist|gpubf> <game.gm | game.gm.gz>
+//   gpuparsi [options] <gpu|gpulist|gpubf|gpubv> <game.gm | game.gm.gz>
 //
 // Prints "<seconds> <iter_count> <br_count>" for the solve_si region, as the
 // reference's test() does (gpumain.cpp:56-71).  By default the whole solve_si
@@ -8,7 +8,11 @@
 // StrategyImprovement calls exactly as gpumain.cpp:18-53 does.  gpubf runs
 // solve_bf (main.cpp:57-77, the CPU command line's cpubf) on the device; an
 // odd strategy cycle in its first best response prints the reference's
-// message and exits with status 1, as cpu_bv.cpp:73-84 does.
+// message and exits with status 1, as cpu_bv.cpp:73-84 does; its br counts
+// device valuation rounds, not the reference's relaxation sweeps (said on
+// stderr, and as "br_kind" in the JSON line).  gpubv runs solve_si with CPUBV
+// semantics (main.cpp:163-167, the CPU command line's cpubv) on the device:
+// the same stop on the first odd strategy cycle of any valuation.
 // Options: --host-loop, --reset (the cpu-reset schedule), --winning (print W0
 // and W1), --strategy (print the final strategy), --json (one JSON line).
 // Exit status: 0 ok, 1 usage, 255 parse error ("parsing error: ..." on stderr,
@@ -118,13 +122,13 @@ int main(int argc, char** argv)
         std::cout << "usage: " << argv[0] << " [--host-loop] [--reset] [--winning] [--strategy] [--json]"
                   << " [algorithm] [game]" << std::endl
                   << std::endl
-                  << "    algorithms: gpu, gpulist, gpubf" << std::endl
+                  << "    algorithms: gpu, gpulist, gpubf, gpubv" << std::endl
                   << std::endl;
         return 1;
     }
     const std::string& algorithm = pos[0];
     const std::string& filename = pos[1];
-    if (algorithm != "gpu" && algorithm != "gpulist" && algorithm != "gpubf") {
+    if (algorithm != "gpu" && algorithm != "gpulist" && algorithm != "gpubf" && algorithm != "gpubv") {
         std::cout << "unknown algorithm: " << algorithm << std::endl;
         return 1;
     }
@@ -149,9 +153,15 @@ int main(int argc, char** argv)
         GPUParity* si = algorithm == "gpulist" ? new GPUList(g, gpu_g) : new GPUParity(g, gpu_g);
         int iter = 0, br = 0;
         auto t0 = std::chrono::steady_clock::now();
-        if (algorithm == "gpubf") {
+        if (algorithm == "gpubf" || algorithm == "gpubv") {
             try {
-                si->solve_bf(&iter, &br);
+                if (algorithm == "gpubf") {
+                    si->solve_bf(&iter, &br);
+                    std::cerr << "gpubf: br counts device valuation rounds, not the reference's "
+                                 "relaxation sweeps" << std::endl;
+                } else {
+                    si->solve_bv(&iter, &br);
+                }
             } catch (const std::exception& e) {
                 if (std::strstr(e.what(), "odd cycle")) {
                     std::cout << "Game contains an odd cycle that uses only odd nodes" << std::endl;
@@ -176,6 +186,7 @@ int main(int argc, char** argv)
         if (strategy) st = si->strategy();
         if (json) {
             printf("{\"seconds\":%.6f,\"iter\":%d,\"br\":%d", sec, iter, br);
+            if (algorithm == "gpubf") printf(",\"br_kind\":\"device valuation rounds\"");
             if (winning) {
                 printf(",");
                 print_list("w0", w0, true);

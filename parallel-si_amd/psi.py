@@ -316,6 +316,7 @@ _sig("psi_switch_strategy", C.c_int, _vp, C.c_int, C.POINTER(C.c_int))
 _sig("psi_reset_opponent_strategy", C.c_int, _vp)
 _sig("psi_solve", C.c_int, _vp, C.c_int, C.c_int, C.c_int)
 _sig("psi_solve_bf", C.c_int, _vp, C.c_int, C.c_int)
+_sig("psi_solve_bv", C.c_int, _vp, C.c_int, C.c_int)
 _sig("psi_get_bf_status", C.c_int, _vp, C.c_int, C.POINTER(C.c_int))
 _sig("psi_get_stats", C.c_int, _vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
      C.POINTER(C.c_int))
@@ -364,6 +365,15 @@ class Solver:
     def solve_bf(self, max_rounds=0, steps_per_sync=4):
         """solve_bf with the Bellman-Ford best response (cpu_bf role)."""
         check(lib.psi_solve_bf(self.h, max_rounds, steps_per_sync), "psi_solve_bf")
+
+    def solve_bv(self, max_rounds=0, steps_per_sync=4):
+        """solve_si with CPUBV semantics (the cpubv role): returns False if a
+        game stopped on an odd strategy cycle (bf_status(game) == 1)."""
+        rc = lib.psi_solve_bv(self.h, max_rounds, steps_per_sync)
+        if rc == PSI_ERR_INVALID and b"odd cycle" in lib.psi_last_error():
+            return False
+        check(rc, "psi_solve_bv")
+        return True
 
     def bf_status(self, game=0):
         s = C.c_int()
@@ -573,13 +583,17 @@ class Pool:
     def step(self, steps=4):
         """Returns the tickets that finished in this call."""
         k = C.c_int()
-        check(lib.psi_pool_step(self.h, steps, self._done, self.max_games, C.byref(k)),
-              "psi_pool_step")
-        done = [int(x) for x in self._done[: k.value]]
-        for t in list(self._held):
-            if self.poll(t) >= RUNNING:
-                del self._held[t]
-        return done
+        try:
+            check(lib.psi_pool_step(self.h, steps, self._done, self.max_games, C.byref(k)),
+                  "psi_pool_step")
+        finally:
+            # a request that was admitted (state >= RUNNING) or whose admission
+            # failed (its ticket is gone, state -1) no longer needs its arrays
+            for t in list(self._held):
+                st = self.poll(t)
+                if st >= RUNNING or st < 0:
+                    del self._held[t]
+        return [int(x) for x in self._done[: k.value]]
 
     def poll(self, ticket):
         s = C.c_int()

@@ -124,11 +124,44 @@ def make_bf():
         print(name, g.n, res["final"]["status"] if res["final"] else res["message"])
 
 
+def make_bv():
+    """cpubv fixtures (CPUBV<ArrayVal> under solve_si): games the reference
+    solves, and games where a valuation meets an odd strategy cycle and the
+    reference exits (cpu_bv.cpp:73-84)."""
+    rng = np.random.default_rng(91)
+    done_ok, done_abort, k = 0, 0, 0
+    while done_ok < 8 or done_abort < 4:
+        k += 1
+        n = int(rng.integers(4, 60))
+        g = psi.gen_random_game(9100 + k, n, int(rng.integers(2, 3 * n)), 1, int(rng.integers(1, 5)))
+        if k % 3 == 0:
+            g.priority[:] &= ~1          # even priorities only: no cycle is odd
+        with tempfile.NamedTemporaryFile("w", suffix=".gm", delete=False) as fh:
+            fh.write(g.to_gm())
+            path = fh.name
+        res = run_ref("cpubv", path, 100000)
+        os.unlink(path)
+        ok = res["final"] is not None and res["final"]["status"] == "ok"
+        if ok and done_ok < 8:
+            name, done_ok = f"bv_{done_ok:02d}", done_ok + 1
+        elif not ok and res["returncode"] != 0 and done_abort < 4:
+            name, done_abort = f"bv_abort_{done_abort:02d}", done_abort + 1
+        else:
+            continue
+        fx = {"name": name, "game": game_dict(g), "modes": {"cpubv": summarize(res, False)}}
+        with open(os.path.join(HERE, name + ".json"), "w") as fh:
+            json.dump(fx, fh, separators=(",", ":"))
+        print(name, g.n, res["final"]["status"] if res["final"] else res["message"])
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
     if "--bf-only" in sys.argv:
         make_bf()
+        return
+    if "--bv-only" in sys.argv:
+        make_bv()
         return
     all_modes = ["cpu", "cpu-reset", "cpubv", "cpulist", "cpubf"]
     # 1. the reference's own input fixture
@@ -155,6 +188,8 @@ def main():
         make(f"chain_{seed:02d}", g, ["cpu", "cpulist"], full=n <= 40, max_rounds=2000)
     # 6. solve_bf (cpubf) fixtures
     make_bf()
+    # 7. CPUBV (cpubv) fixtures
+    make_bv()
 
 
 if __name__ == "__main__":

@@ -241,3 +241,97 @@ def test_solve_bf_relaxation_is_a_best_response():
         assert cnt == 0, k
         checked += 1
     assert checked >= 10
+
+
+LIST_FIXTURES = [fx["name"] for fx in FIXTURES if "cpulist" in fx["modes"]]
+
+
+@pytest.mark.parametrize("name", LIST_FIXTURES)
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_solve_list_matches_reference(name, threads):
+    """The threaded CPUList restatement (Euler tour + Helman-JaJa valuation
+    on T threads) vs the compiled reference's cpulist run on every fixture:
+    per-round hashes of strategy, solved flags, labels and valuation digests,
+    and the final counts, strategy and labels; games the reference's cpulist
+    does not finish (test2.gm, whose odd vertices oscillate) match round for
+    round up to its cap."""
+    fx = _fx(name)
+    g = fixture_game(fx)
+    exp = fx["modes"]["cpulist"]
+    fin = exp["final"]
+    cap = len(exp["round_hashes"]) if fin["status"] == "capped" else 0
+    r = oracle.solve_list(g, threads=threads, max_rounds=cap, trace=True)
+    assert [[str(int(x)) for x in row] for row in r.hashes] == exp["round_hashes"]
+    assert r.capped == (fin["status"] == "capped")
+    assert (r.iter, r.br) == (fin["iter"], fin["br"])
+    assert r.strat.tolist() == fin["strat"] and r.inf.tolist() == fin["inf"]
+
+
+def test_solve_list_thread_count_changes_nothing():
+    """The tour's child order depends on the threads' atomic exchanges and
+    the sublists on the thread ranges; the valuations, hence the whole run,
+    do not (mid-size games, 8 threads vs 1, round-capped)."""
+    rng = np.random.default_rng(23)
+    for k in range(6):
+        n = int(rng.integers(2000, 20000))
+        g = psi.gen_random_game(9700 + k, n, int(rng.integers(2, 2 * n)), 1, 4)
+        a = oracle.solve_list(g, threads=1, max_rounds=60, trace=True)
+        b = oracle.solve_list(g, threads=8, max_rounds=60, trace=True)
+        assert (a.iter, a.br, a.capped) == (b.iter, b.br, b.capped)
+        assert np.array_equal(a.strat, b.strat) and np.array_equal(a.inf, b.inf)
+        assert np.array_equal(a.hashes, b.hashes)
+
+
+BV_FIXTURES = [fx["name"] for fx in FIXTURES if "cpubv" in fx["modes"]]
+
+
+@pytest.mark.parametrize("name", BV_FIXTURES)
+@pytest.mark.parametrize("rep", [oracle.DENSE, oracle.SPARSE])
+def test_solve_bv_matches_reference(name, rep):
+    """CPUBV restatement vs the compiled reference's cpubv run (test2.gm, the
+    reference's own fixture, and the bv_* games): games it completes end with
+    its iteration and round counts, strategy, labels and per-round hashes;
+    games it stops on an odd strategy cycle stop after the same number of
+    completed valuations, each of which equals the reference's."""
+    fx = _fx(name)
+    g = fixture_game(fx)
+    exp = fx["modes"]["cpubv"]
+    r = oracle.solve_bv(g, rep=rep)
+    rounds = exp["round_hashes"]
+    if exp["returncode"] != 0:
+        assert exp["message"] == "Game contains an odd cycle that uses only odd nodes"
+        assert r.status == oracle.BV_ODD_CYCLE
+        assert r.br == len(rounds)
+        if rounds:
+            t = oracle.solve(g, rep=rep, max_rounds=len(rounds), trace=True)
+            got = [[str(int(x)) for x in row] for row in t.hashes]
+            assert got == rounds
+        return
+    fin = exp["final"]
+    assert r.status == oracle.BV_OK and not r.capped
+    assert (r.iter, r.br) == (fin["iter"], fin["br"])
+    assert r.strat.tolist() == fin["strat"] and r.inf.tolist() == fin["inf"]
+    t = oracle.solve(g, rep=rep, trace=True)
+    assert [[str(int(x)) for x in row] for row in t.hashes] == rounds
+
+
+def test_solve_bv_completes_exactly_when_no_odd_cycle_label():
+    """CPUBV stops exactly at the first round in which CPUParity labels a
+    vertex -1 (only an odd strategy cycle produces -1 when nothing is solved
+    -1 yet); otherwise both solves are the same."""
+    rng = np.random.default_rng(17)
+    stopped = completed = 0
+    for k in range(80):
+        n = int(rng.integers(3, 40))
+        g = psi.gen_random_game(9500 + k, n, int(rng.integers(2, 2 * n + 2)), 1, 3)
+        r = oracle.solve(g, full_trace_rounds=4096)
+        first_neg = next((i for i, inf in enumerate(r.inf_rounds) if (inf == -1).any()), None)
+        b = oracle.solve_bv(g)
+        if first_neg is None:
+            assert b.status == oracle.BV_OK
+            assert (b.iter, b.br) == (r.iter, r.br) and np.array_equal(b.strat, r.strat)
+            completed += 1
+        else:
+            assert b.status == oracle.BV_ODD_CYCLE and b.br == first_neg
+            stopped += 1
+    assert stopped >= 10 and completed >= 10
