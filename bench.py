@@ -142,7 +142,34 @@ def listrank_bench(psi, reps=10, tune=True):
         log(f"[bench] {name}: {ms * 1e3:.1f} us, {gbs:.0f} GB/s algorithmic")
         for d in (ds, dh, dr):
             d.free()
+    out["by_length"] = listrank_sweep(psi)
     return out
+
+
+def listrank_sweep(psi, reps=20):
+    """One list of n nodes by each method (device time per ranking, us): the
+    measurements behind AUTO's choice by list length."""
+    res = {}
+    for n in (1024, 4096, 16384, 65536, 131072, 262144, 1 << 20):
+        succ, heads = psi.gen_list(7, n)
+        ds, dh = psi.DeviceArray.from_host(succ), psi.DeviceArray.from_host(heads)
+        dr = psi.DeviceArray(4 * n)
+        row = {}
+        for name, method in (("HJ", psi.LR_HJ), ("WYLLIE", psi.LR_WYLLIE), ("AUTO", psi.LR_AUTO)):
+            for _ in range(3):
+                psi.list_rank_device(ds, n, dh, 1, dr, method)
+            psi.synchronize()
+            t = psi.Timer()
+            t.start()
+            for _ in range(reps):
+                psi.list_rank_device(ds, n, dh, 1, dr, method)
+            t.stop()
+            row[name] = round(t.ms() / reps * 1e3, 2)
+        res[str(n)] = row
+        for d in (ds, dh, dr):
+            d.free()
+    log("[bench] list ranking by length (us): " + json.dumps(res))
+    return res
 
 
 def tour_bench(psi, g, reps=5):

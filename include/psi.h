@@ -55,7 +55,9 @@ int psi_event_elapsed_ms(void* start, void* end, float* ms);
  * (cpu_list_ranking.cpp:108-239) as a standalone entry with unit weights.
  * succ[i] in [-1, n) (-1 ends a list); heads[] holds the first node of every
  * list.  rank[i] = distance of i from its head, -1 if no head reaches i.  */
-#define PSI_LR_AUTO 0      /* WALK if mean list length n/n_heads is short, else HJ */
+#define PSI_LR_AUTO 0      /* by length: Wyllie (heads as given) up to PSI_LRT_WYLLIE_MAX
+                              nodes, else WALK if the mean list length n/n_heads is
+                              short, else HJ                                          */
 #define PSI_LR_HJ 1        /* Helman-JaJa: hashed block splitters + head slots     */
 #define PSI_LR_WYLLIE 2    /* pointer jumping; heads = nodes without predecessor   */
 #define PSI_LR_WALK 3      /* head slots only (one walk per list, hop-capped)      */
@@ -105,6 +107,7 @@ int psi_tour_path_sums(const int* d_link, int n, const uint64_t* d_w, uint64_t* 
 #define PSI_LRT_LOG_DK 3        /* reduced-level splitter density 2^-v           */
 #define PSI_LRT_HOP_CAP_K 4     /* reduced-level hop cap                         */
 #define PSI_LRT_WALK_MEAN_MAX 5 /* AUTO threshold on mean list length            */
+#define PSI_LRT_WYLLIE_MAX 6    /* AUTO: Wyllie for at most this many nodes      */
 int psi_list_rank_tune(int param, int value);
 
 /* ---- games --------------------------------------------------------------- */
@@ -199,6 +202,16 @@ int psi_solve_bf(psi_solver* s, int max_rounds, int steps_per_sync);
  * returns PSI_ERR_INVALID with that message; iter/br then count what was
  * completed before the stop.  Other games end with CPUParity's result. */
 int psi_solve_bv(psi_solver* s, int max_rounds, int steps_per_sync);
+/* solve_si with CPUList<ArrayVal>'s semantics (the cpulist mode,
+ * main.cpp:178-191; the valuation of cpu_list_ranking.cpp:242-255 and
+ * gpu_list_ranking.cu:279-326): a vertex whose strategy reaches the sink is
+ * finite, every other vertex +inf (broadcast_val, cpu_list_ranking.cpp:
+ * 208-239); solved flags are recorded (CPUBV::mark_solved) but the valuation
+ * never reads them.  Bit-exact with the reference's cpulist (counts,
+ * strategies, labels, per-round hashes).  Such a run need not end (on test2.gm
+ * Odd's vertices switch back and forth): max_rounds > 0 caps it, and a capped
+ * game returns PSI_ERR_CAP (its state is the reference's after that round). */
+int psi_solve_list(psi_solver* s, int max_rounds, int steps_per_sync);
 int psi_get_bf_status(psi_solver* s, int game, int* status);   /* 1 = odd cycle */
 int psi_get_stats(psi_solver* s, int game, int* iter, int* br, int* capped);
 /* Profiling (no reference counterpart; the reference times solve_si as a
@@ -231,6 +244,10 @@ int psi_get_profile_rounds(psi_solver* s, int* out, int cap, int* count);
  * switch(0); TOTAL counts steps).  Not on a serving pool's solver. */
 int psi_solve_program_profiled(psi_solver* s, int reset, int max_rounds, double* times_ms,
                                int64_t* counts);
+/* Its launch timeline: the phase word of each launch (low 7 bits: phase) and
+ * the launch's kernel duration in microseconds (profiling; no reference
+ * counterpart). */
+int psi_get_program_launches(psi_solver* s, int* phase, float* us, int cap, int* count);
 
 /* Results of one game, local vertex ids.  get_winning: gpu_parity.cu:340-354,
  * cpu_parity.cpp:288-297. */

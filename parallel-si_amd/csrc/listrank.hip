@@ -37,6 +37,10 @@ struct Tuning {
     int log_dk = 5;      // reduced-level splitter density
     int hop_cap_k = 1024;
     int walk_mean_max = 64; // AUTO: mean list length at or below this -> head walks only
+    // AUTO: at most this many nodes -> Wyllie (heads as given).  Measured on
+    // one list (MI355X, us per ranking, HJ / Wyllie): 4096 nodes 107 / 19,
+    // 16384: 94 / 51, 65536: 184 / 89, 262144: 148 / 181, 2^20: 279 / 500.
+    int wyllie_max = 131072;
     uint32_t salt = 0x2545F491u;
 };
 Tuning g_tune;
@@ -65,6 +69,54 @@ template <class T> int fill(T* p, long long n, T v, hipStream_t s)
     PSI_LAUNCH_CHECK();
     return PSI_OK;
 }
+
+// Several int fills in one launch (each launch costs a few microseconds,
+// more than filling a level of the reduced list).
+constexpr int kFillSegs = 6;
+struct FillSet {
+    int* p[kFillSegs];
+    long long start[kFillSegs + 1];
+    int v[kFillSegs];
+    int k;
+};
+
+__global__ void __launch_bounds__(256) fill_set_kernel(FillSet fs)
+{
+    const long long total = fs.start[fs.k];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        int j = 0;
+        while (i >= fs.start[j + 1]) j++;
+        fs.p[j][i - fs.start[j]] = fs.v[j];
+    }
+}
+
+struct Fills {
+    FillSet fs{};
+    hipStream_t s;
+    explicit Fills(hipStream_t st) : s(st) { fs.start[0] = 0; }
+    int add(int* p, long long n, int v)
+    {
+        if (n <= 0) return PSI_OK;
+        if (fs.k == kFillSegs)
+            if (int rc = launch()) return rc;
+        fs.p[fs.k] = p;
+        fs.v[fs.k] = v;
+        fs.start[fs.k + 1] = fs.start[fs.k] + n;
+        fs.k++;
+        return PSI_OK;
+    }
+    int launch()
+    {
+        if (fs.k == 0) return PSI_OK;
+        const long long total = fs.start[fs.k];
+        const unsigned grid = (unsigned)std::min<long long>(nblocks(total, 256), 4096);
+        fill_set_kernel<<<grid, 256, 0, s>>>(fs);
+        PSI_LAUNCH_CHECK();
+        fs.k = 0;
+        return PSI_OK;
+    }
+};
 
 // ctl[0]: overflow sublists allocated; ctl[1]: error bits (1 bad index, 2 overflow exhausted)
 
@@ -254,19 +306,127 @@ __global__ void __launch_bounds__(kTopThreads) top_lds(int R, const int* __restr
     }
 }
 
+// Level-0 Wyllie for a list of at most kTopCap nodes inside one workgroup's
+// LDS: predecessors and prefixes never leave LDS, ceil(log2 N) + 1 rounds,
+// one launch (the gpu mode's pointer jumping, gpu_parity.cu:79-106, on a
+// list).  use_heads: the heads[] nodes start the lists and any other node
+// without a predecessor is unreachable (-1), as for the walks; otherwise
+// every node without a predecessor is a head.  ctl[1] |= 1 for an index out
+// of range, 2 for a node with two predecessors (not a forest of lists).
+__global__ void __launch_bounds__(kTopThreads) wy_lds0(const int* __restrict__ succ, int N,
+                                                       const int* __restrict__ heads, int H,
+                                                       int use_heads, int* __restrict__ rank,
+                                                       int* __restrict__ ctl, int ctl_zero)
+{
+    __shared__ int P[kTopCap];
+    __shared__ int S[kTopCap];
+    constexpr int per = kTopCap / kTopThreads;
+    const int tid = threadIdx.x;
+    int bits = 0;
+    if (tid < ctl_zero) ctl[tid] = 0;             // this call's control words (one workgroup)
+    for (int i = tid; i < N; i += kTopThreads) P[i] = -3;
+    __syncthreads();
+    for (int i = tid; i < N; i += kTopThreads) {
+        int nx = succ[i];
+        if (nx >= 0 && nx < N) {
+            if (atomicExch(&P[nx], i) != -3) bits |= 2;
+        } else if (nx != -1) {
+            bits |= 1;
+        }
+    }
+    __syncthreads();
+    if (use_heads) {
+        for (int k = tid; k < H; k += kTopThreads) {
+            int h = heads[k];
+            if (h >= 0 && h < N) {
+                if (P[h] == -3) P[h] = -2;        // a given head
+            } else {
+                bits |= 1;
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < N; i += kTopThreads) {
+        int p = P[i];
+        if (p >= 0) {
+            S[i] = 1;
+        } else {
+            P[i] = (p == -2 || !use_heads) ? -1 : -3;
+            S[i] = 0;
+        }
+    }
+    __syncthreads();
+    const int rounds = ceil_log2_dev(N) + 1;
+    for (int r = 0; r < rounds; r++) {
+        int np[per], ns[per];
+#pragma unroll
+        for (int k = 0; k < per; k++) {
+            int i = tid + k * kTopThreads;
+            np[k] = -1;
+            ns[k] = 0;
+            if (i < N) {
+                int p = P[i], s = S[i];
+                if (p >= 0) {
+                    s += S[p];
+                    p = P[p];
+                }
+                np[k] = p;
+                ns[k] = s;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < per; k++) {
+            int i = tid + k * kTopThreads;
+            if (i < N) {
+                P[i] = np[k];
+                S[i] = ns[k];
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < N; i += kTopThreads) rank[i] = P[i] == -1 ? S[i] : -1;
+    __threadfence_block();
+    __syncthreads();                              // the zeroing above precedes the bits
+    if (bits) atomicOr(&ctl[1], bits);
+}
+
 // Wyllie pointer jumping along predecessors.  PS[i] = (pred-state, prefix).
 // hb < 0: every node without a predecessor is a head (level-0 Wyllie).
-__global__ void wy_pred(const int* __restrict__ nxt, int R, int* __restrict__ pred)
+// ctl (level 0 only, else null): bit 2 if a node has two predecessors.
+__global__ void wy_pred(const int* __restrict__ nxt, int R, int* __restrict__ pred,
+                        int* __restrict__ ctl)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= R) return;
     int nx = nxt[i];
-    if (nx >= 0 && nx < R) pred[nx] = i;
+    if (nx >= 0 && nx < R) {
+        if (ctl) {
+            if (atomicExch(&pred[nx], i) != -1) atomicOr(&ctl[1], 2);
+        } else {
+            pred[nx] = i;
+        }
+    } else if (ctl && nx != -1) {
+        atomicOr(&ctl[1], 1);
+    }
 }
 
+// Level-0 heads given by a list: hmark[h] = 1.
+__global__ void wy_mark_heads(const int* __restrict__ heads, int H, int N, int* __restrict__ hmark,
+                              int* __restrict__ ctl)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= H) return;
+    int h = heads[k];
+    if (h >= 0 && h < N) hmark[h] = 1;
+    else atomicOr(&ctl[1], 1);
+}
+
+// hmark (level 0 with given heads, else null): a node without a predecessor
+// is a head only if marked.
 template <bool UNIT>
 __global__ void wy_init(const int* __restrict__ pred, const int* __restrict__ w, int R, int hb,
-                        int H, int2* __restrict__ ps)
+                        int H, const int* __restrict__ hmark, int2* __restrict__ ps)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= R) return;
@@ -275,7 +435,7 @@ __global__ void wy_init(const int* __restrict__ pred, const int* __restrict__ w,
     if (p >= 0) {
         v = make_int2(p, UNIT ? 1 : w[p]);
     } else {
-        bool head = hb < 0 ? true : (i >= hb && i < hb + H);
+        bool head = hmark ? hmark[i] != 0 : (hb < 0 ? true : (i >= hb && i < hb + H));
         v = make_int2(head ? -1 : -3, 0);
     }
     ps[i] = v;
@@ -358,6 +518,29 @@ __global__ void final0(int N, int LB, int R1, int* __restrict__ rank, const int*
     int id = p >> LB;
     int o = id < R1 ? roff1[id] : -1;             // one gather: -1 marks unreachable
     rank[x] = o >= 0 ? o + (p & ((1 << LB) - 1)) : -1;
+}
+
+__device__ __forceinline__ int final_one(int p, int LB, int R1, const int* __restrict__ roff1)
+{
+    if (p < 0) return p;
+    int id = p >> LB;
+    int o = id < R1 ? roff1[id] : -1;
+    return o >= 0 ? o + (p & ((1 << LB) - 1)) : -1;
+}
+
+// final0 four nodes per lane (16-byte loads and stores of rank; the roff1
+// gathers stay in L2): rank[4i .. 4i+3] for i < N4.
+__global__ void __launch_bounds__(256) final0_v4(int N4, int LB, int R1, int4* __restrict__ rank4,
+                                                 const int* __restrict__ roff1)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N4) return;
+    int4 p = rank4[i];
+    p.x = final_one(p.x, LB, R1, roff1);
+    p.y = final_one(p.y, LB, R1, roff1);
+    p.z = final_one(p.z, LB, R1, roff1);
+    p.w = final_one(p.w, LB, R1, roff1);
+    rank4[i] = p;
 }
 
 // ---------------------------------------------------------------------------
@@ -449,10 +632,12 @@ static int rank_level(Plan& pl, int k, Arena& ar, int* ctl0, hipStream_t s)
         if (ar.over) return fail(PSI_ERR_INVALID, "list ranking: workspace plan exceeded");
         int* active = ctl0 + kMaxLevels * 2;
         int rounds = ceil_log2(L.R) + 1;
-        if (int rc = fill<int>(pred, L.R, -1, s)) return rc;
-        if (int rc = fill<int>(active, kWyRounds * kWySlots + 1, 0, s)) return rc;
-        wy_pred<<<nblocks(L.R, 256), 256, 0, s>>>(L.rnext, L.R, pred);
-        wy_init<false><<<nblocks(L.R, 256), 256, 0, s>>>(pred, L.w, L.R, L.hb, L.H, ps0);
+        Fills f(s);
+        f.add(pred, L.R, -1);
+        f.add(active, kWyRounds * kWySlots + 1, 0);
+        if (int rc = f.launch()) return rc;
+        wy_pred<<<nblocks(L.R, 256), 256, 0, s>>>(L.rnext, L.R, pred, nullptr);
+        wy_init<false><<<nblocks(L.R, 256), 256, 0, s>>>(pred, L.w, L.R, L.hb, L.H, nullptr, ps0);
         for (int r = 0; r < rounds; r++)
             wy_round<<<nblocks(L.R, 256), 256, 0, s>>>((r & 1) ? ps1 : ps0, (r & 1) ? ps0 : ps1,
                                                       L.R, active, r);
@@ -477,10 +662,14 @@ static int rank_level(Plan& pl, int k, Arena& ar, int* ctl0, hipStream_t s)
     nx.roff = ar.take<int>(nx.R);
     if (ar.over) return fail(PSI_ERR_INVALID, "list ranking: workspace plan exceeded");
     int* ctl = ctl0 + 2 * (k + 1);
-    if (int rc = fill<int>(ctl, 2, 0, s)) return rc;
-    if (int rc = fill<int>(nx.rnext, nx.R, -2, s)) return rc;
-    if (int rc = fill<int>(nx.w, nx.R, 0, s)) return rc;
-    if (int rc = fill<int>(L.rid, L.R, -1, s)) return rc;
+    {
+        Fills f(s);
+        f.add(ctl, 2, 0);
+        f.add(nx.rnext, nx.R, -2);
+        f.add(nx.w, nx.R, 0);
+        f.add(L.rid, L.R, -1);
+        if (int rc = f.launch()) return rc;
+    }
     if (B + L.H > 0)
     walk_w<<<nblocks(B + L.H, 256), 256, 0, s>>>(L.rnext, L.w, L.R, L.hb, L.H, B, log_d, tu.salt,
                                                  hc, L.rid, L.loc, nx.rnext, nx.w, ctl, B + L.H,
@@ -497,7 +686,11 @@ static int rank_level(Plan& pl, int k, Arena& ar, int* ctl0, hipStream_t s)
 }
 
 // Workspace of each method for N nodes and H heads (what the runs below take).
-static size_t wyllie0_bytes(long long N) { return 2 * al256((size_t)N * 4) + 2 * al256((size_t)N * 8); }
+static size_t wyllie0_bytes(long long N, bool use_heads)
+{
+    if (N <= kTopCap) return 0;
+    return (use_heads ? 3 : 2) * al256((size_t)N * 4) + 2 * al256((size_t)N * 8);
+}
 
 struct Hj0Shape {
     int B0 = 0, LB = 0, over0 = 0;
@@ -541,21 +734,40 @@ static int arena_prepare(Arena& ar, size_t need, bool grow)
     return PSI_OK;
 }
 
-static int wyllie0(const int* succ, int N, int* rank, Arena& ar, int* ctl0, bool grow, hipStream_t s)
+// Level-0 Wyllie.  use_heads: heads[] start the lists (AUTO's choice for
+// short lists, same results as the walks); else every node without a
+// predecessor is a head (PSI_LR_WYLLIE).  Lists of at most kTopCap nodes are
+// ranked in one workgroup's LDS (one launch, no workspace).
+static int wyllie0(const int* succ, int N, const int* heads, int H, bool use_heads, int* rank,
+                   Arena& ar, int* ctl0, bool grow, hipStream_t s)
 {
-    if (int rc = arena_prepare(ar, wyllie0_bytes(N), grow)) return rc;
+    if (N <= kTopCap) {
+        wy_lds0<<<1, kTopThreads, 0, s>>>(succ, N, heads, H, use_heads ? 1 : 0, rank, ctl0,
+                                          kMaxLevels * 2);
+        PSI_LAUNCH_CHECK();
+        return PSI_OK;
+    }
+    if (int rc = arena_prepare(ar, wyllie0_bytes(N, use_heads), grow)) return rc;
     int* pred = ar.take<int>(N);
     int* rok = ar.take<int>(N);
     int2* ps0 = ar.take<int2>(N);
     int2* ps1 = ar.take<int2>(N);
+    int* hmark = use_heads ? ar.take<int>(N) : nullptr;
     if (ar.over) return fail(PSI_ERR_INVALID, "list ranking: workspace plan exceeded");
     int* active = ctl0 + kMaxLevels * 2;
     int rounds = ceil_log2(N) + 1;
     if (rounds > 64) rounds = 64;
-    if (int rc = fill<int>(pred, N, -1, s)) return rc;
-    if (int rc = fill<int>(active, kWyRounds * kWySlots + 1, 0, s)) return rc;
-    wy_pred<<<nblocks(N, 256), 256, 0, s>>>(succ, N, pred);
-    wy_init<true><<<nblocks(N, 256), 256, 0, s>>>(pred, nullptr, N, -1, 0, ps0);
+    {
+        Fills f(s);
+        f.add(ctl0, kMaxLevels * 2, 0);
+        f.add(pred, N, -1);
+        f.add(active, kWyRounds * kWySlots + 1, 0);
+        if (use_heads) f.add(hmark, N, 0);
+        if (int rc = f.launch()) return rc;
+    }
+    if (use_heads && H > 0) wy_mark_heads<<<nblocks(H, 256), 256, 0, s>>>(heads, H, N, hmark, ctl0);
+    wy_pred<<<nblocks(N, 256), 256, 0, s>>>(succ, N, pred, ctl0);
+    wy_init<true><<<nblocks(N, 256), 256, 0, s>>>(pred, nullptr, N, -1, 0, hmark, ps0);
     for (int r = 0; r < rounds; r++)
         wy_round<<<nblocks(N, 256), 256, 0, s>>>((r & 1) ? ps1 : ps0, (r & 1) ? ps0 : ps1, N,
                                                 active, r);
@@ -594,18 +806,29 @@ static int hj0(const int* succ, int N, const int* heads, int H, int* rank, bool 
     if (ar.over) return fail(PSI_ERR_INVALID, "list ranking: workspace plan exceeded");
     pl.lv.push_back(L1);
     int* ctl = ctl0;
-    if (int rc = fill<int>(ctl, 2, 0, s)) return rc;
-    if (int rc = fill<int>(L1.rnext, R1, -2, s)) return rc;
-    if (int rc = fill<int>(L1.w, R1, 0, s)) return rc;
-    if (checked)
-        if (int rc = fill<int>(rank, N, -1, s)) return rc;
+    {
+        Fills f(s);
+        f.add(ctl0, kMaxLevels * 2, 0);
+        f.add(L1.rnext, R1, -2);
+        f.add(L1.w, R1, 0);
+        if (checked) f.add(rank, N, -1);
+        if (int rc = f.launch()) return rc;
+    }
     if (pl.B0 + H > 0) {
         walk0_unit<<<nblocks((long long)pl.B0 + H, 256), 256, 0, s>>>(
             succ, N, heads, H, pl.B0, log_d, tu.salt, LB, rank, L1.rnext, L1.w, ctl, pl.B0 + H, pl.over0);
         PSI_LAUNCH_CHECK();
     }
     if (int rc = rank_level(pl, 0, ar, ctl0, s)) return rc;
-    final0<<<nblocks(N, 256), 256, 0, s>>>(N, LB, (int)R1, rank, pl.lv[0].rok, pl.lv[0].roff);
+    if (((uintptr_t)rank & 15) == 0 && N >= 1024) {
+        const int N4 = N / 4;
+        final0_v4<<<nblocks(N4, 256), 256, 0, s>>>(N4, LB, (int)R1, (int4*)rank, pl.lv[0].roff);
+        if (N4 * 4 < N)
+            final0<<<1, 256, 0, s>>>(N - N4 * 4, LB, (int)R1, rank + N4 * 4, pl.lv[0].rok,
+                                     pl.lv[0].roff);
+    } else {
+        final0<<<nblocks(N, 256), 256, 0, s>>>(N, LB, (int)R1, rank, pl.lv[0].rok, pl.lv[0].roff);
+    }
     PSI_LAUNCH_CHECK();
     return PSI_OK;
 }
@@ -620,16 +843,20 @@ static int run(const int* d_succ, int n, const int* d_heads, int n_heads, int* d
     int flags = method & ~0xff;
     method &= 0xff;
     bool checked = (flags & PSI_LR_CHECKED) != 0;
-    g_last_ctl = ctl0;
-    if (int rc = fill<int>(ctl0, kMaxLevels * 2, 0, s)) return rc;
+    g_last_ctl = ctl0;        // each method zeroes ctl0[0, 2 * kMaxLevels) in its first launch
     if (method == PSI_LR_AUTO) {
+        // by list length: Wyllie for short lists (few nodes: its log n rounds
+        // beat the walks' launches and their longest sublist), head walks for
+        // many short lists, Helman-JaJa otherwise
+        if (n <= g_tune.wyllie_max)
+            return wyllie0(d_succ, n, d_heads, n_heads, true, d_rank, ar, ctl0, grow, s);
         long long mean = n_heads ? (long long)n / n_heads : (long long)n;
         method = mean <= g_tune.walk_mean_max ? PSI_LR_WALK : PSI_LR_HJ;
     }
     switch (method) {
     case PSI_LR_HJ: return hj0(d_succ, n, d_heads, n_heads, d_rank, true, checked, ar, ctl0, grow, s);
     case PSI_LR_WALK: return hj0(d_succ, n, d_heads, n_heads, d_rank, false, checked, ar, ctl0, grow, s);
-    case PSI_LR_WYLLIE: return wyllie0(d_succ, n, d_rank, ar, ctl0, grow, s);
+    case PSI_LR_WYLLIE: return wyllie0(d_succ, n, d_heads, n_heads, false, d_rank, ar, ctl0, grow, s);
     default: return fail(PSI_ERR_INVALID, "psi_list_rank: unknown method");
     }
 }
@@ -757,6 +984,7 @@ extern "C" int psi_list_rank_tune(int param, int value)
     case PSI_LRT_LOG_DK: if (value < 1 || value > 16) break; t.log_dk = value; return PSI_OK;
     case PSI_LRT_HOP_CAP_K: if (value < 16) break; t.hop_cap_k = value; return PSI_OK;
     case PSI_LRT_WALK_MEAN_MAX: if (value < 1) break; t.walk_mean_max = value; return PSI_OK;
+    case PSI_LRT_WYLLIE_MAX: if (value < 0) break; t.wyllie_max = value; return PSI_OK;
     default: break;
     }
     return fail(PSI_ERR_INVALID, "psi_list_rank_tune: bad parameter or value");

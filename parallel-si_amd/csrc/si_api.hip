@@ -263,6 +263,26 @@ extern "C" int psi_solve_bv(psi_solver* s, int max_rounds, int steps_per_sync)
     return PSI_OK;
 }
 
+// solve_si with CPUList's semantics (the cpulist mode, main.cpp:178-191;
+// CPUList::compute_valuation, cpu_list_ranking.cpp:242-255): a vertex whose
+// strategy reaches the sink is finite, every other vertex +inf, and solved
+// flags are recorded but never read.  Such runs need not end (Odd can close
+// an odd cycle and reopen it): max_rounds caps them (PSI_ERR_CAP).
+extern "C" int psi_solve_list(psi_solver* s, int max_rounds, int steps_per_sync)
+{
+    Core& c = s->core;
+    if (int rc = core_init_strat(c)) return rc;
+    if (int rc = core_set_params(c, 0, max_rounds, 3)) return rc;
+    if (steps_per_sync <= 0) steps_per_sync = 4;
+    if (int rc = core_solve(c, steps_per_sync)) return rc;
+    std::vector<GameCtl> gc(c.G);
+    PSI_HIP(hipMemcpyAsync(gc.data(), c.gctl, c.G * sizeof(GameCtl), hipMemcpyDeviceToHost, c.s));
+    PSI_HIP(hipStreamSynchronize(c.s));
+    for (int g = 0; g < c.G; g++)
+        if (gc[g].capped) return fail(PSI_ERR_CAP, "game " + std::to_string(g) + ": round cap reached");
+    return PSI_OK;
+}
+
 extern "C" int psi_get_bf_status(psi_solver* s, int game, int* status)
 {
     if (int rc = check_game(s, game)) return rc;
@@ -313,6 +333,19 @@ extern "C" int psi_solve_program_profiled(psi_solver* s, int reset, int max_roun
     int rc = core_solve_program_profiled(c, times_ms, cn.data());
     for (int k = 0; k < PSI_K_COUNT; k++) counts[k] = cn[k];
     return rc;
+}
+
+// Launch timeline of the last psi_solve_program_profiled: the phase word
+// each launch ran (low 7 bits: the phase) and its kernel duration in us.
+extern "C" int psi_get_program_launches(psi_solver* s, int* phase, float* us, int cap, int* count)
+{
+    const Core& c = s->core;
+    *count = (int)c.prof_phase.size();
+    for (int k = 0; k < cap && k < *count; k++) {
+        if (phase) phase[k] = c.prof_phase[k];
+        if (us) us[k] = c.prof_us[k];
+    }
+    return PSI_OK;
 }
 
 static int check_game(psi_solver* s, int game)

@@ -50,7 +50,8 @@ struct GameCtl {
     int clear0;     // the last step was an outer step: restart the switch(0) change record
     int bf;         // 1: solve_bf (main.cpp:57-77), later best responses by relaxation;
                     // 2: CPUBV's solve_si (main.cpp:163-167), every valuation stops on an
-                    // odd strategy cycle
+                    // odd strategy cycle; 3: CPUList's semantics (cpulist): vertices not
+                    // reaching the sink +inf, solved flags not read by the valuation
     int bf_init;    // solve_bf: the first best response (an SI inner loop) is done
     int err;        // odd strategy cycle: the game stopped there (cpu_bv.cpp:73-84)
     int activate;   // serving pool, step program: admitted (done until the next step boundary)
@@ -131,6 +132,8 @@ struct Core {
     std::vector<long long> h_m;
     unsigned long long* prof_ties = nullptr;  // profiled solves: compares taking the exact path
     std::vector<int> prof_rounds;             // last profiled solve: jump rounds run per valuation
+    std::vector<int> prof_phase;              // last program-profiled solve: phase of each launch
+    std::vector<float> prof_us;               // ... and its duration (kernel start to end)
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     StepProg* prog = nullptr;          // step program state (not in a serving pool)

@@ -272,15 +272,16 @@ __device__ __forceinline__ void val_init_body(
 #pragma unroll
             for (int q = 0; q < kInitRows; q++) issued(g[q], vi[q], s[q], old[q], sv[q], lv[q]);
             // then the games' control words
-            int done[kInitRows], rp[kInitRows], br[kInitRows];
+            int done[kInitRows], rp[kInitRows], br[kInitRows], lsem[kInitRows];
 #pragma unroll
             for (int q = 0; q < kInitRows; q++) {
                 done[q] = gc[g[q]].done;
                 rp[q] = gc[g[q]].reset_pending;
                 br[q] = gc[g[q]].br;
+                lsem[q] = gc[g[q]].bf;
             }
 #pragma unroll
-            for (int q = 0; q < kInitRows; q++) issued(done[q], rp[q], br[q], 0, 0, 0);
+            for (int q = 0; q < kInitRows; q++) issued(done[q], rp[q], br[q], lsem[q], 0, 0);
 #pragma unroll
             for (int q = 0; q < kInitRows; q++) {
                 const long long v = base + (h + q) * kWave;
@@ -295,7 +296,10 @@ __device__ __forceinline__ void val_init_body(
                     // w = 1 if this vertex's link differs from the previous
                     // valuation's (the OR along a path says the path changed).
                     const int first = br[q] == 0;
-                    if (sv[q]) {
+                    // solved vertices are terminals with their frozen label,
+                    // except under CPUList's semantics (bf 3), whose valuation
+                    // never reads the solved flags (cpu_list_ranking.cpp:242-255)
+                    if (sv[q] && lsem[q] != 3) {
                         const int code = term_code(lv[q] == 1 ? 1 : 2, 0);
                         S0[v] = make_int2(code, (first | (old[q] != code)) ? kChangedBit : 0);
                         if (old[q] != code) par[v] = code;   // (unchanged links: no store)
@@ -731,7 +735,7 @@ __device__ __forceinline__ void label_body(
                     // window maximum at its block's final level (loads for
                     // the whole row, other lanes reading their own vertex).
                     int cyc_pri = 0;
-                    const bool cyc = live && !sol[q] && kind < 0;
+                    const bool cyc = live && !sol[q] && kind < 0 && bf[q] != 3;
                     if (__any(cyc)) {
                         const int x = cyc ? s[q].x : (int)min(v, N - 1);
                         const int Fy = min(F, (int)blast[x / kBS]);
@@ -748,7 +752,7 @@ __device__ __forceinline__ void label_body(
                         // length and maximum), an infinite label that stayed
                         // has the same key, a solved vertex keeps its label.
                         int kv, dv;
-                        if (sol[q]) {
+                        if (sol[q] && bf[q] != 3) {
                             l = lold[q];
                             kv = (l + 1) << 30;
                             dv = 0;
@@ -768,8 +772,12 @@ __device__ __forceinline__ void label_body(
                                     // (such vertices stay infinite, cpu_bf.cpp:156-160);
                                     // the first best response stops on an odd one.
                                     // CPUBV solve (bf 2): every valuation stops on one
-                                    // (cpu_bv.cpp:73-84).
-                                    if (bf[q]) {
+                                    // (cpu_bv.cpp:73-84).  CPUList (bf 3): every vertex
+                                    // that does not reach the sink is +inf
+                                    // (broadcast_val, cpu_list_ranking.cpp:208-239).
+                                    if (bf[q] == 3) {
+                                        l = 1;
+                                    } else if (bf[q]) {
                                         if (bfi[q] && bf[q] == 1) l = 1;
                                         else if (l == -1) gc[g[q]].err = 1;
                                     }
@@ -2469,6 +2477,8 @@ int core_solve_program_profiled(Core& c, double* times, long long* counts)
     PSI_HIP(hipEventCreate(&e1));
     k_prog_reset<<<1, 256, 0, c.s>>>(c.G, c.gctl, c.prog);
     int rc = PSI_OK;
+    c.prof_phase.clear();
+    c.prof_us.clear();
     for (long long i = 0; i < (1LL << 26); i++) {
         const int par = (int)(i & 1);
         // events attached to the dispatch itself: the kernel's own start and
@@ -2486,6 +2496,8 @@ int core_solve_program_profiled(Core& c, double* times, long long* counts)
         if (ph == PH_DONE) break;
         float ms = 0;
         (void)hipEventElapsedTime(&ms, e0, e1);
+        c.prof_phase.push_back(st);
+        c.prof_us.push_back(ms * 1000.0f);
         int cls = PSI_K_CTL;
         switch (ph) {
         case PH_INIT: cls = PSI_K_VAL_INIT; counts[PSI_K_TOTAL]++; break;

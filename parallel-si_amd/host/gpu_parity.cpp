@@ -60,6 +60,15 @@ void GPUParity::solve_bv(int* iter_count, int* br_count)
     check(psi_get_stats(s, 0, iter_count, br_count, &capped), "solve_bv");
 }
 
+bool GPUParity::solve_list(int max_rounds, int* iter_count, int* br_count)
+{
+    int rc = psi_solve_list(s, max_rounds, 4);
+    if (rc != PSI_ERR_CAP) check(rc, "solve_list");
+    int capped = 0;
+    check(psi_get_stats(s, 0, iter_count, br_count, &capped), "solve_list");
+    return rc == PSI_OK;
+}
+
 std::vector<int> GPUParity::strategy()
 {
     std::vector<int> out(g_dev.num_vert);

@@ -46,6 +46,9 @@ public:
     // cpubv mode): throws on the first odd strategy cycle, where the
     // reference prints the message and exits (cpu_bv.cpp:73-84).
     void solve_bv(int* iter_count, int* br_count);
+    // Device-resident solve_si with CPUList's semantics (the cpulist mode,
+    // main.cpp:178-191): false if it stopped at max_rounds valuations.
+    bool solve_list(int max_rounds, int* iter_count, int* br_count);
     std::vector<int> strategy();
     std::vector<int> labels();
     psi_solver* handle() { return s; }

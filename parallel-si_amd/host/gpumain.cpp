@@ -14,7 +14,10 @@
 // semantics (main.cpp:163-167, the CPU command line's cpubv) on the device:
 // the same stop on the first odd strategy cycle of any valuation.
 // Options: --host-loop, --reset (the cpu-reset schedule), --winning (print W0
-// and W1), --strategy (print the final strategy), --json (one JSON line).
+// and W1), --strategy (print the final strategy), --json (one JSON line),
+// --cpulist[=N] (the CPU command line's cpulist semantics, CPUList: vertices
+// not reaching the sink are +inf; such runs need not end, so they stop after
+// N valuations, default 100000, with a note on stderr).
 // Exit status: 0 ok, 1 usage, 255 parse error ("parsing error: ..." on stderr,
 // as gpumain.cpp:124-127), 2 device error.
 #include "game.h"
@@ -24,8 +27,10 @@
 
 #include <zlib.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <iostream>
 #include <sstream>
@@ -108,6 +113,7 @@ static void print_list(const char* name, const std::vector<int>& v, bool json)
 int main(int argc, char** argv)
 {
     bool host_loop = false, reset = false, winning = false, strategy = false, json = false;
+    int cpulist_cap = 0;             // > 0: CPUList's semantics with this round cap
     std::vector<std::string> pos;
     for (int i = 1; i < argc; i++) {
         std::string a = argv[i];
@@ -116,10 +122,13 @@ int main(int argc, char** argv)
         else if (a == "--winning") winning = true;
         else if (a == "--strategy") strategy = true;
         else if (a == "--json") json = true;
+        else if (a == "--cpulist") cpulist_cap = 100000;
+        else if (a.rfind("--cpulist=", 0) == 0) cpulist_cap = std::max(1, atoi(a.c_str() + 10));
         else pos.push_back(a);
     }
     if (pos.size() != 2) {
         std::cout << "usage: " << argv[0] << " [--host-loop] [--reset] [--winning] [--strategy] [--json]"
+                  << " [--cpulist[=N]]"
                   << " [algorithm] [game]" << std::endl
                   << std::endl
                   << "    algorithms: gpu, gpulist, gpubf, gpubv" << std::endl
@@ -170,6 +179,10 @@ int main(int argc, char** argv)
                 }
                 throw;
             }
+        } else if (cpulist_cap > 0) {
+            if (!si->solve_list(cpulist_cap, &iter, &br))
+                std::cerr << "cpulist semantics: stopped at the round cap (" << cpulist_cap
+                          << " valuations)" << std::endl;
         } else if (host_loop) {
             solve_si(*si, &br, &iter, reset);
         } else {

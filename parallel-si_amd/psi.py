@@ -22,6 +22,8 @@ PSI_ERR_BUDGET = -5
 LR_AUTO, LR_HJ, LR_WYLLIE, LR_WALK = 0, 1, 2, 3
 LR_CHECKED = 0x100
 LRT_LOG_D0, LRT_LOCAL_BITS, LRT_LOG_DK, LRT_HOP_CAP_K, LRT_WALK_MEAN_MAX = 1, 2, 3, 4, 5
+LRT_WYLLIE_MAX = 6
+LRT_WYLLIE_MAX_DEFAULT = 131072
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is not built (run make -C parallel-si_amd)")
@@ -317,6 +319,7 @@ _sig("psi_reset_opponent_strategy", C.c_int, _vp)
 _sig("psi_solve", C.c_int, _vp, C.c_int, C.c_int, C.c_int)
 _sig("psi_solve_bf", C.c_int, _vp, C.c_int, C.c_int)
 _sig("psi_solve_bv", C.c_int, _vp, C.c_int, C.c_int)
+_sig("psi_solve_list", C.c_int, _vp, C.c_int, C.c_int)
 _sig("psi_get_bf_status", C.c_int, _vp, C.c_int, C.POINTER(C.c_int))
 _sig("psi_get_stats", C.c_int, _vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
      C.POINTER(C.c_int))
@@ -373,6 +376,15 @@ class Solver:
         if rc == PSI_ERR_INVALID and b"odd cycle" in lib.psi_last_error():
             return False
         check(rc, "psi_solve_bv")
+        return True
+
+    def solve_list(self, max_rounds=0, steps_per_sync=4):
+        """solve_si with CPUList semantics (the cpulist role): returns False
+        if a game reached max_rounds (such runs need not end)."""
+        rc = lib.psi_solve_list(self.h, max_rounds, steps_per_sync)
+        if rc == PSI_ERR_CAP:
+            return False
+        check(rc, "psi_solve_list")
         return True
 
     def bf_status(self, game=0):
@@ -493,6 +505,25 @@ def solve_program_profiled(solver, reset=False, max_rounds=0):
     check(lib.psi_solve_program_profiled(solver.h, int(reset), max_rounds, t, c),
           "psi_solve_program_profiled")
     return {k: (float(t[i]), int(c[i])) for i, k in enumerate(K_NAMES)}
+
+_sig("psi_get_program_launches", C.c_int, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_int))
+
+
+def program_launches(solver, cap=1 << 16):
+    """Launch timeline of the last solve_program_profiled: list of
+    (phase, round, us); phases as PH_NAMES."""
+    ph = np.zeros(cap, np.int32)
+    us = np.zeros(cap, np.float32)
+    k = C.c_int()
+    check(lib.psi_get_program_launches(solver.h, ph.ctypes.data, us.ctypes.data, cap, C.byref(k)),
+          "psi_get_program_launches")
+    k = min(k.value, cap)
+    return [(PH_NAMES[int(p) & 0x7f] if (int(p) & 0x7f) < len(PH_NAMES) else str(int(p) & 0x7f),
+             int(p) >> 8, float(u)) for p, u in zip(ph[:k], us[:k])]
+
+
+PH_NAMES = ["start", "init", "jump", "label", "switch1", "ctl1", "mark0", "switch0", "ctl0", "done",
+            "act"]
 
 _sig("psi_get_jump_stats", C.c_int, _vp, _i32p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int))
 

@@ -66,7 +66,7 @@ def test_forests(gpu, method, n_lists, n_nodes):
     _check(gpu, succ, heads, method)
 
 
-@pytest.mark.parametrize("method", ["HJ", "WALK", "WYLLIE"])
+@pytest.mark.parametrize("method", ["AUTO", "HJ", "WALK", "WYLLIE"])
 def test_unreachable_nodes_read_minus_one(gpu, method):
     # list 0->1->2, a 3-cycle 3->4->5->3 and a list 6->7 whose head is not passed
     succ = np.array([1, 2, -1, 4, 5, 3, 7, -1], np.int32)
@@ -76,6 +76,49 @@ def test_unreachable_nodes_read_minus_one(gpu, method):
     if method == "WYLLIE":       # Wyllie treats every node without predecessor as a head
         exp = [0, 1, 2, -1, -1, -1, 0, 1]
     assert got.tolist() == exp
+
+
+@pytest.mark.parametrize("wy_max", [0, 16384, 131072, 1 << 22])
+@pytest.mark.parametrize("n_lists,n_nodes", [(1, 16384), (1, 16385), (300, 16000), (3000, 200000),
+                                             (1, 1 << 20)])
+def test_auto_wyllie_with_given_heads(gpu, wy_max, n_lists, n_nodes):
+    """AUTO ranks lists of at most PSI_LRT_WYLLIE_MAX nodes by Wyllie with the
+    heads as given (in one workgroup's LDS up to 16,384 nodes, else by global
+    rounds): the same ranks as the walks, also where some lists' heads are not
+    passed (their nodes read -1) and with cycles no head reaches."""
+    import psi
+    succ, heads, _ = psi.gen_forest(n_lists * 7 + n_nodes, n_nodes, n_lists, 4096)
+    succ = succ.copy()
+    if n_lists > 1:
+        heads = heads[::3].copy()                  # two lists in three lose their head
+        dropped = np.setdiff1d(np.arange(n_lists), np.arange(0, n_lists, 3))
+        h = int(psi.gen_forest(n_lists * 7 + n_nodes, n_nodes, n_lists, 4096)[1][dropped[0]])
+        t = h                                      # close that list into a cycle
+        while succ[t] != -1:
+            t = succ[t]
+        succ[t] = h
+    print(f"AUTO/Wyllie lists={n_lists} nodes={n_nodes} wy_max={wy_max}", flush=True)
+    gpu.list_rank_tune(gpu.LRT_WYLLIE_MAX, wy_max)
+    try:
+        _check(gpu, succ, heads, "AUTO")
+        _check(gpu, succ, heads, "HJ")
+    finally:
+        gpu.list_rank_tune(gpu.LRT_WYLLIE_MAX, gpu.LRT_WYLLIE_MAX_DEFAULT)
+
+
+@pytest.mark.parametrize("n", [10, 20000])
+def test_auto_wyllie_reports_two_predecessors(gpu, n):
+    """A node with two predecessors (not a forest of lists) is reported by
+    the Wyllie paths as an error, not ranked silently."""
+    gpu.list_rank_tune(gpu.LRT_WYLLIE_MAX, 1 << 20)
+    try:
+        succ = np.arange(1, n + 1, dtype=np.int32)
+        succ[-1] = -1
+        succ[0] = 5                                # nodes 0 and 4 both lead to 5
+        with pytest.raises(gpu.PsiError):
+            gpu.list_rank_host(succ, np.array([0], np.int32), gpu.LR_AUTO)
+    finally:
+        gpu.list_rank_tune(gpu.LRT_WYLLIE_MAX, gpu.LRT_WYLLIE_MAX_DEFAULT)
 
 
 def test_long_list_head_walk_overflows_hop_cap(gpu):
