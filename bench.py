@@ -150,7 +150,7 @@ def listrank_sweep(psi, reps=20):
     """One list of n nodes by each method (device time per ranking, us): the
     measurements behind AUTO's choice by list length."""
     res = {}
-    for n in (1024, 4096, 16384, 65536, 131072, 262144, 1 << 20):
+    for n in (1024, 4096, 8192, 16384, 65536, 262144, 1 << 20):
         succ, heads = psi.gen_list(7, n)
         ds, dh = psi.DeviceArray.from_host(succ), psi.DeviceArray.from_host(heads)
         dr = psi.DeviceArray(4 * n)

@@ -76,7 +76,8 @@ int psi_list_rank_host(const int* succ, int n, const int* heads, int n_heads, in
  * a fixed array of 64 MiB per GPU, statically allocated in the library and
  * held until the process ends, instead of psi_list_rank's grown workspace.
  * Inputs whose workspace would exceed it (e.g. PSI_LR_WYLLIE beyond ~2.7M
- * nodes; HJ handles the 16,777,216-node list and the 1,048,576-list forest)
+ * nodes; the 16,777,216-node list by HJ and the 1,048,576-list forest by
+ * AUTO fit with the default densities)
  * are rejected with PSI_ERR_INVALID and a message naming the bytes needed.
  * Every call on a GPU uses that GPU's one array: calls that overlap on one
  * GPU (other host threads, other streams) race.  Calls no hipMalloc/hipFree;

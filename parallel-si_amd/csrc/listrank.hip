@@ -31,16 +31,22 @@ namespace lr {
 
 constexpr int kTopThreads = 1024;
 
+// Defaults measured on the 16,777,216-node list (MI355X, us per ranking):
+// level-0 / reduced-level splitter densities 2^-4 / 2^-4: 808, 2^-5 / 2^-4:
+// 812, 2^-6 / 2^-4: 819, 2^-5 / 2^-5: 854, 2^-3 / 2^-4: 882, 2^-2 / 2^-4: 1013;
+// 2^-4 / 2^-3: 778 (and one list of 2^20 nodes 137, against 281 at
+// 2^-6 / 2^-4: the reduced levels' walks are bound by their longest sublist).
 struct Tuning {
-    int log_d0 = 5;      // level-0 splitter density: one per 2^log_d0 ids
+    int log_d0 = 4;      // level-0 splitter density: one per 2^log_d0 ids
     int local_bits = 12; // level-0 hop cap 2^local_bits (also the packed local field)
-    int log_dk = 5;      // reduced-level splitter density
+    int log_dk = 3;      // reduced-level splitter density
     int hop_cap_k = 1024;
     int walk_mean_max = 64; // AUTO: mean list length at or below this -> head walks only
     // AUTO: at most this many nodes -> Wyllie (heads as given).  Measured on
-    // one list (MI355X, us per ranking, HJ / Wyllie): 4096 nodes 107 / 19,
-    // 16384: 94 / 51, 65536: 184 / 89, 262144: 148 / 181, 2^20: 279 / 500.
-    int wyllie_max = 131072;
+    // one list (MI355X, us per ranking, HJ with the densities above / Wyllie):
+    // 1024 nodes 26 / 11, 4096: 36 / 19, 16384: 48 / 51, 65536: 57 / 80,
+    // 2^20: 137 / 420.
+    int wyllie_max = 8192;
     uint32_t salt = 0x2545F491u;
 };
 Tuning g_tune;
@@ -446,33 +452,32 @@ __global__ void wy_init(const int* __restrict__ pred, const int* __restrict__ w,
 constexpr int kWySlots = 64;
 constexpr int kWyRounds = 64;
 
+// Grid-stride over at most kWyGrid workgroups, so that the rounds launched
+// after convergence (which exit at once) cost little to dispatch.
+constexpr int kWyGrid = 2048;
+
 __global__ void __launch_bounds__(256) wy_round(const int2* __restrict__ in, int2* __restrict__ out,
                                                 int R, int* __restrict__ active, int r)
 {
-    __shared__ int run_s;
     __shared__ unsigned int cnt;
+    // The round decision: a wave reduction over the previous round's slot
+    // words (written by the previous launch, the same for every wave).
+    if (r > 0 && !__any(active[(r - 1) * kWySlots + (threadIdx.x & (kWySlots - 1))] != 0))
+        return;                                   // converged: nothing left to jump
     if (threadIdx.x == 0) {
-        int run = 1;
-        if (r > 0) {
-            int a = 0;
-            for (int k = 0; k < kWySlots; k++) a |= active[(r - 1) * kWySlots + k];
-            run = a != 0;                         // converged: nothing left to jump
-        }
-        run_s = run;
         cnt = 0;
-        if (run && blockIdx.x == 0) active[kWyRounds * kWySlots] = r;
+        if (blockIdx.x == 0) active[kWyRounds * kWySlots] = r;
     }
     __syncthreads();
-    if (!run_s) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool act = false;
-    if (i < R) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < R;
+         i += (long long)gridDim.x * blockDim.x) {
         int2 v = in[i];
         if (v.x >= 0) {
             int2 u = in[v.x];
             v.y += u.y;
             v.x = u.x;
-            act = v.x >= 0;
+            act |= v.x >= 0;
         }
         out[i] = v;
     }
@@ -631,16 +636,21 @@ static int rank_level(Plan& pl, int k, Arena& ar, int* ctl0, hipStream_t s)
         int2* ps1 = ar.take<int2>(L.R);
         if (ar.over) return fail(PSI_ERR_INVALID, "list ranking: workspace plan exceeded");
         int* active = ctl0 + kMaxLevels * 2;
-        int rounds = ceil_log2(L.R) + 1;
+        // A list of this level has at most R nodes; after head walks alone
+        // (level 1 of PSI_LR_WALK) it is one head slot and the overflow slots
+        // its walk was cut into, so at most 1 + over0 nodes.
+        long long maxlen = L.R;
+        if (k == 0 && pl.B0 == 0) maxlen = std::min<long long>(maxlen, 1LL + pl.over0);
+        int rounds = ceil_log2(maxlen) + 1;
         Fills f(s);
         f.add(pred, L.R, -1);
         f.add(active, kWyRounds * kWySlots + 1, 0);
         if (int rc = f.launch()) return rc;
         wy_pred<<<nblocks(L.R, 256), 256, 0, s>>>(L.rnext, L.R, pred, nullptr);
         wy_init<false><<<nblocks(L.R, 256), 256, 0, s>>>(pred, L.w, L.R, L.hb, L.H, nullptr, ps0);
+        const unsigned wg = (unsigned)std::min<long long>(nblocks(L.R, 256), kWyGrid);
         for (int r = 0; r < rounds; r++)
-            wy_round<<<nblocks(L.R, 256), 256, 0, s>>>((r & 1) ? ps1 : ps0, (r & 1) ? ps0 : ps1,
-                                                      L.R, active, r);
+            wy_round<<<wg, 256, 0, s>>>((r & 1) ? ps1 : ps0, (r & 1) ? ps0 : ps1, L.R, active, r);
         wy_finish<<<nblocks(L.R, 256), 256, 0, s>>>(ps0, ps1, L.R, active, L.rok, L.roff);
         PSI_LAUNCH_CHECK();
         return PSI_OK;
@@ -768,9 +778,9 @@ static int wyllie0(const int* succ, int N, const int* heads, int H, bool use_hea
     if (use_heads && H > 0) wy_mark_heads<<<nblocks(H, 256), 256, 0, s>>>(heads, H, N, hmark, ctl0);
     wy_pred<<<nblocks(N, 256), 256, 0, s>>>(succ, N, pred, ctl0);
     wy_init<true><<<nblocks(N, 256), 256, 0, s>>>(pred, nullptr, N, -1, 0, hmark, ps0);
+    const unsigned wg = (unsigned)std::min<long long>(nblocks(N, 256), kWyGrid);
     for (int r = 0; r < rounds; r++)
-        wy_round<<<nblocks(N, 256), 256, 0, s>>>((r & 1) ? ps1 : ps0, (r & 1) ? ps0 : ps1, N,
-                                                active, r);
+        wy_round<<<wg, 256, 0, s>>>((r & 1) ? ps1 : ps0, (r & 1) ? ps0 : ps1, N, active, r);
     wy_finish<<<nblocks(N, 256), 256, 0, s>>>(ps0, ps1, N, active, rok, rank);
     PSI_LAUNCH_CHECK();
     return PSI_OK;

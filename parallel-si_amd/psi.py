@@ -23,7 +23,7 @@ LR_AUTO, LR_HJ, LR_WYLLIE, LR_WALK = 0, 1, 2, 3
 LR_CHECKED = 0x100
 LRT_LOG_D0, LRT_LOCAL_BITS, LRT_LOG_DK, LRT_HOP_CAP_K, LRT_WALK_MEAN_MAX = 1, 2, 3, 4, 5
 LRT_WYLLIE_MAX = 6
-LRT_WYLLIE_MAX_DEFAULT = 131072
+LRT_WYLLIE_MAX_DEFAULT = 8192
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is not built (run make -C parallel-si_amd)")
