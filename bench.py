@@ -105,10 +105,15 @@ def si_bench(psi, n, steps, warmup, seed):
     pprof = psi.solve_program_profiled(s)
     log("[bench] step-program phases ms/launches: " +
         ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in pprof.items() if v[1]))
+    # The phase of every launch in order: the timed graph solves run the same
+    # launch sequence (then idle launches to the end of their last graph), so
+    # this attributes per-dispatch profiler records to phases.
+    log("[bench] step-program launch phases: " +
+        " ".join(p for p, _, _ in psi.program_launches(s)))
     return g, s, it, br, wall, dev_ms, parity, prof, pprof, prounds
 
 
-def listrank_bench(psi, reps=10, tune=True):
+def listrank_bench(psi, reps=10, tune=True, sweep=True):
     out = {}
     n = 1 << 24
     if tune:
@@ -142,7 +147,8 @@ def listrank_bench(psi, reps=10, tune=True):
         log(f"[bench] {name}: {ms * 1e3:.1f} us, {gbs:.0f} GB/s algorithmic")
         for d in (ds, dh, dr):
             d.free()
-    out["by_length"] = listrank_sweep(psi)
+    if sweep:
+        out["by_length"] = listrank_sweep(psi)
     return out
 
 
@@ -543,6 +549,10 @@ def main():
     ap.add_argument("--batch-games", type=int, default=1024)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-bf", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="list ranking: default splitter densities, no tuning trials (for profiling)")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="list ranking: no sweep over list lengths (for profiling)")
     ap.add_argument("--requests", type=int, default=96, help="serving: requests")
     ap.add_argument("--rate", type=float, default=2.0, help="serving: arrivals per second")
     ap.add_argument("--only", choices=["si", "listrank", "batch", "serve", "bf", "copy", "cpu"],
@@ -557,7 +567,8 @@ def main():
     import psi
     psi.init()
     if a.only == "listrank":
-        print(json.dumps({"listrank": listrank_bench(psi)}), flush=True)
+        print(json.dumps({"listrank": listrank_bench(psi, tune=not a.no_autotune,
+                                                     sweep=not a.no_sweep)}), flush=True)
         return
     if a.only == "copy":
         # calibration for hardware byte counters: one device-to-device copy of 1 GiB
@@ -651,7 +662,7 @@ def main():
     }
     s.close()
     if not a.no_listrank:
-        line["listrank"] = listrank_bench(psi)
+        line["listrank"] = listrank_bench(psi, tune=not a.no_autotune, sweep=not a.no_sweep)
         line["listrank"]["tour_path_sums_4M"] = tour_bench(psi, g)
     if not a.no_batch:
         line["batch"] = batch_bench(psi, a.batch_games)

@@ -233,7 +233,8 @@ int psi_get_stats(psi_solver* s, int game, int* iter, int* br, int* capped);
 #define PSI_K_TIES 9      /* counts only: compares that took the exact (equal path max) path */
 #define PSI_K_SCANS 10    /* counts only: exact compares that walked the private segments */
 #define PSI_K_SCAN_STEPS 11 /* counts only: vertices visited by those walks */
-#define PSI_K_COUNT 12
+#define PSI_K_PUSH 12     /* step program: change marks for the next switch passes */
+#define PSI_K_COUNT 13
 int psi_solve_profiled(psi_solver* s, int reset, int max_rounds, double* times_ms,
                        int64_t* counts);  /* both arrays have PSI_K_COUNT entries */
 /* Jump rounds that ran in each valuation of the last psi_solve_profiled, in
@@ -335,6 +336,12 @@ int psi_pool_info(psi_pool* p, int* running, int* queued, int64_t* vertex_cap, i
  * launches advance two levels each (three dependent gathers per vertex),
  * 0 one level; results are identical. */
 #define PSI_SIT_TWO_LEVELS 6
+/* PSI_SIT_PUSH: k > 0 (default 16): in the step program on a one-game
+ * handle, after a valuation in which at most n / k vertices changed, the
+ * owner-list entries of their in-neighbours are marked and the next switch
+ * passes visit only marked entries (else every entry); 0 never marks.
+ * Results are identical. */
+#define PSI_SIT_PUSH 7
 int psi_si_tune(int param, int value);
 
 /* ---- seeded generators (host only, no device calls) --------------------- */

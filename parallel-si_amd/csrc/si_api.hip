@@ -215,6 +215,11 @@ extern "C" int psi_si_tune(int param, int value)
         g_si_tune.two_levels = value;
         return PSI_OK;
     }
+    if (param == PSI_SIT_PUSH) {
+        if (value < 0) return fail(PSI_ERR_INVALID, "psi_si_tune: push is 0 or positive");
+        g_si_tune.push = value;
+        return PSI_OK;
+    }
     if (value < 64 || value > (1 << 20)) return fail(PSI_ERR_INVALID, "psi_si_tune: value out of range");
     switch (param) {
     case PSI_SIT_PHASE_GRID: g_si_tune.phase_grid = (unsigned)value; return PSI_OK;
@@ -481,6 +486,9 @@ extern "C" int psi_set_state(psi_solver* s, int game, const int* strat, const in
     PSI_HIP(hipMemcpyAsync(c.strat + vb, st.data(), n * 4, hipMemcpyHostToDevice, c.s));
     PSI_HIP(hipMemcpyAsync(c.solved + vb, so.data(), n, hipMemcpyHostToDevice, c.s));
     PSI_HIP(hipMemcpyAsync(c.lab + vb, la.data(), n, hipMemcpyHostToDevice, c.s));
+    // a loaded strategy need not be a successor, which the switch's change
+    // marks assume of the current choice
+    c.push_ok = false;
     if (int rc = core_state_loaded(c, game)) return rc;
     PSI_HIP(hipStreamSynchronize(c.s));
     return PSI_OK;

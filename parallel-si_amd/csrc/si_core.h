@@ -62,6 +62,7 @@ struct GameCtl {
 // slot blockIdx % kSlots) so that no single address takes more than a few
 // hundred atomics per round.
 constexpr int kSlots = 64;
+constexpr int kCntStride = 32;   // 128-byte lines
 constexpr int kMaxRounds = 40;
 struct JumpCnt {
     unsigned long long slot[kMaxRounds][kSlots]; // (resolved << 32) | still-jumping
@@ -84,6 +85,10 @@ struct StepProg {
     int dev_seq[2]; // by launch parity: admissions activated as of that launch
     unsigned long long psum[2];
     unsigned long long pslot[4][kSlots];
+    unsigned ccnt[kSlots * kCntStride]; // vertices whose valuation changed at the last
+                                        // valuation, over kSlots slots a cache line apart
+    int full1;      // the next switch(1) scans every entry (the last valuation set no marks)
+    int full0;      // the next switch(0) does (some valuation since the last one set none)
 };
 
 struct Core {
@@ -115,6 +120,14 @@ struct Core {
     int levels = 0;        // buffers allocated
     int* own[2] = {nullptr, nullptr};   // vertices owned by each player
     long long own_n[2] = {0, 0};
+    // One game (not a pool): in-edge lists, each in-edge as its source's
+    // owner-list entry (index << 1 | owner), and per owner-list entry a mark
+    // set when a successor's valuation changed (the step program's switch
+    // visits only marked entries when few valuations changed).
+    int* rin_off = nullptr;            // N+1
+    int* rin_src = nullptr;            // M
+    uint8_t* dmark[2] = {nullptr, nullptr};
+    bool push_ok = false;              // marks usable (cleared when a state is loaded)
     unsigned long long* Zl = nullptr;  // trace: digests per jump level (as Sl)
     unsigned long long* dig = nullptr;
     GameCtl* gctl = nullptr;
@@ -139,6 +152,7 @@ struct Core {
     StepProg* prog = nullptr;          // step program state (not in a serving pool)
     void* pargs = nullptr;             // step program arguments (device copy)
     int pargs_two = -1;                // two_levels in that copy
+    long long pargs_push = -1;         // and its mark threshold
     hipGraphExec_t pgraph = nullptr;   // captured step-program launches
     int pgraph_launches = 0;
     unsigned pgraph_grid = 0;          // workgroups per launch in that graph
@@ -152,6 +166,8 @@ struct SiTuning {
     unsigned phase_grid = 2048;  // blocks of a step-program launch
     int program = 1;             // solve loop as a step program (else fixed launches per step)
     int two_levels = 1;          // step program: jump launches advance two levels
+    int push = 16;               // step program, one game: marks when at most N / push
+                                 // valuations changed (0: never)
 };
 extern SiTuning g_si_tune;
 

@@ -37,6 +37,7 @@
 #include "si_core.h"
 
 #include <hip/hip_ext.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <atomic>
@@ -666,9 +667,11 @@ __device__ __forceinline__ void label_body(
     int* __restrict__ key, unsigned long long* __restrict__ dig, unsigned long long* __restrict__ tr,
     int trace_cap, int loop_form, int trace, JumpCnt* __restrict__ jc,
     const uint8_t* __restrict__ blive, const int* __restrict__ hi,
-    unsigned long long* __restrict__ chg, unsigned long long* __restrict__ chg0, int check_rounds)
+    unsigned long long* __restrict__ chg, unsigned long long* __restrict__ chg0, int check_rounds,
+    unsigned* __restrict__ ccnt)
 {
     const int last = jc->last;
+    unsigned wchg = 0;                            // this wave's changed vertices (ccnt)
     const int F = last + 1;
     if (check_rounds && blockIdx.x == 0 && threadIdx.x == 0) {
         int stop, cyc, halt;
@@ -803,6 +806,7 @@ __device__ __forceinline__ void label_body(
                     // step in the previous step.
                     const unsigned long long cm = __ballot(ch);
                     const unsigned long long clr = __ballot(live && clr0[q] != 0);
+                    wchg += (unsigned)__popcll(cm);
                     if (lane == 0) {
                         chg[v >> 6] = cm;
                         chg0[v >> 6] = (c0w[q] & ~clr) | cm;
@@ -824,6 +828,14 @@ __device__ __forceinline__ void label_body(
             }
         }
     }
+    if (ccnt) {                                   // one atomic per workgroup, a line per slot
+        __shared__ unsigned wg_chg;
+        if (threadIdx.x == 0) wg_chg = 0;
+        __syncthreads();
+        if (lane == 0 && wchg) atomicAdd(&wg_chg, wchg);
+        __syncthreads();
+        if (threadIdx.x == 0 && wg_chg) atomicAdd(&ccnt[(blockIdx.x & (kSlots - 1)) * kCntStride], wg_chg);
+    }
 }
 
 __global__ void __launch_bounds__(kBS) k_label(
@@ -837,7 +849,7 @@ __global__ void __launch_bounds__(kBS) k_label(
     unsigned long long* __restrict__ chg, unsigned long long* __restrict__ chg0)
 {
     label_body(N, maxn, lmax, Sl, Zl, blast, gid, gc, strat, solved, lab, dep, key, dig, tr, trace_cap,
-               loop_form, trace, jc, blive, hi, chg, chg0, 1);
+               loop_form, trace, jc, blive, hi, chg, chg0, 1, nullptr);
 }
 
 // Exact compare of two finite vertices whose path maxima are equal.  The jump
@@ -1010,7 +1022,7 @@ __device__ __forceinline__ void switch_body(
     const int* __restrict__ par, const int2* __restrict__ Sl, const JumpCnt* __restrict__ jc,
     int loop_form, int* __restrict__ flag, unsigned long long* __restrict__ ties,
     const int* __restrict__ hi, const unsigned long long* __restrict__ chg,
-    const unsigned long long* __restrict__ chg0)
+    const unsigned long long* __restrict__ chg0, uint8_t* __restrict__ dq, int use_d)
 {
     // Loop form: a vertex left at the scan's fixed point stays there unless
     // the valuation of its current successor or of a candidate changed since
@@ -1030,6 +1042,18 @@ __device__ __forceinline__ void switch_body(
     const int lane = (int)(threadIdx.x & (kWave - 1));
     for (int w = (int)(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave); w < nruns; w += nw) {
         const int i = w * kWave + lane;
+        // Change marks (dq, one game): an entry is marked when the valuation
+        // of one of its successors changed since its last scan.  The marks
+        // are consumed here; with use_d only marked entries are visited, and
+        // they are exactly the entries the change bits would select (with
+        // the current successor among the successors), plus at most entries
+        // whose scan then compares no candidate.
+        bool dv = false;
+        if (dq) {
+            dv = i < lim && dq[i] != 0;
+            if (dv) dq[i] = 0;
+            if (use_d && !__any(dv)) continue;
+        }
         bool sw = false, live = false;
         // Only the successors whose valuation changed can beat a current
         // successor whose valuation did not (the others were no better than it
@@ -1037,7 +1061,7 @@ __device__ __forceinline__ void switch_body(
         // just those, in CSR order: the result is the full scan's.
         bool only_changed = false;
         int g = -1, c0 = -1, eb = 0, ee = 0;
-        const int v = i < lim ? own[i] : -1;
+        const int v = i < lim && (!use_d || dv) ? own[i] : -1;
         if (v >= 0) {
             g = gid ? gid[v] : 0;                 // no gid: one game
             c0 = strat[v];                        // loaded together with the game's control
@@ -1047,6 +1071,7 @@ __device__ __forceinline__ void switch_body(
             if (live && incremental && !(player == 1 && gc[g].reset_pending)) {
                 bool dirty = c0 >= 0 && ((cb[c0 >> 6] >> (c0 & 63)) & 1);
                 only_changed = !dirty;
+                if (use_d) dirty = true;          // marked: some successor changed
                 const int e1 = ee;
                 for (int e0 = eb; e0 < e1 && !dirty; e0 += 8) {
                     const int cnt = min(8, e1 - e0);
@@ -1194,7 +1219,7 @@ __global__ void __launch_bounds__(kBS) k_switch(
     const unsigned long long* __restrict__ chg0)
 {
     switch_body(player, N, own, own_n, vinfo, off, tgt, gid, gc, strat, key, dep, par, Sl, jc,
-                loop_form, flag, ties, hi, chg, chg0);
+                loop_form, flag, ties, hi, chg, chg0, nullptr, 0);
 }
 
 // mark_solved(0) of the games that take the outer step (fused form of
@@ -1311,6 +1336,8 @@ __device__ __forceinline__ void ctl0_body(int G, GameCtl* __restrict__ gc, StepP
         if (pg && c.done) atomicSub(&pg->live, 1);
         gc[g] = c;
     }
+    // the switch(0) pass before this launch consumed its marks
+    if (pg && blockIdx.x == 0 && threadIdx.x == 0) pg->full0 = 0;
 }
 
 __global__ void k_ctl0(int G, GameCtl* gc) { ctl0_body(G, gc, nullptr); }
@@ -1326,8 +1353,44 @@ __global__ void k_ctl0(int G, GameCtl* gc) { ctl0_body(G, gc, nullptr); }
 // phase that no game takes; a graph of these launches can stop anywhere and
 // the next graph continues.  Once no game is live every launch exits at once.
 enum Phase {
-    PH_START = 0, PH_INIT, PH_JUMP, PH_LABEL, PH_SW1, PH_CTL1, PH_MARK0, PH_SW0, PH_CTL0, PH_DONE, PH_ACT
+    PH_START = 0, PH_INIT, PH_JUMP, PH_LABEL, PH_SW1, PH_CTL1, PH_MARK0, PH_SW0, PH_CTL0, PH_DONE, PH_ACT,
+    PH_PUSH
 };
+
+// One game, after the labels: when at most push_max vertices changed their
+// valuation, mark the owner-list entries of their in-neighbours (plain byte
+// stores of 1: writers of one mark agree) for the next switch(1) (player 1)
+// and switch(0) (player 0, kept until that pass); otherwise the next
+// switch(1) and switch(0) scan every entry.
+__device__ __forceinline__ void push_body(long long N, const unsigned long long* __restrict__ chg,
+                                          const int* __restrict__ rin_off,
+                                          const int* __restrict__ rin_src, uint8_t* __restrict__ d0,
+                                          uint8_t* __restrict__ d1, long long push_max,
+                                          StepProg* __restrict__ pg)
+{
+    unsigned t = pg->ccnt[(threadIdx.x & (kSlots - 1)) * kCntStride];
+    for (int o = kSlots / 2; o; o >>= 1) t += __shfl_xor(t, o);
+    if ((long long)t > push_max) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            pg->full1 = 1;
+            pg->full0 = 1;
+        }
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) pg->full1 = 0;
+    const long long nw = (N + 63) / 64;
+    for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < nw;
+         w += (long long)gridDim.x * blockDim.x) {
+        for (unsigned long long x = chg[w]; x; x &= x - 1) {
+            const int u = (int)(w * 64 + __ffsll((long long)x) - 1);
+            const int e1 = rin_off[u + 1];
+            for (int e = rin_off[u]; e < e1; e++) {
+                const int c = rin_src[e];
+                (c & 1 ? d1 : d0)[c >> 1] = 1;
+            }
+        }
+    }
+}
 
 // Serving pool: games the host admitted (done, activate set) become live,
 // at a step boundary; the occupied bound grows to cover them.
@@ -1348,6 +1411,9 @@ struct StepArgs {
     int G, maxn, lmax, trace, trace_cap;
     int two_levels;  // jump launches advance two levels (else one)
     const int *vinfo, *off, *tgt, *gid, *own0, *own1, *hi;
+    const int *rin_off, *rin_src;
+    uint8_t *dm0, *dm1;   // change marks of the owner lists
+    long long push_max;   // 0: no marks (PH_PUSH never runs)
     GameCtl* gc;
     int *strat, *par, *dep, *key, *flag, *succ;
     int8_t *solved, *lab;
@@ -1392,8 +1458,9 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         const int hw = reinterpret_cast<const int*>(pg)[min(lane, kProgHeadWords - 1)];
         const unsigned long long x = pg->pslot[(i + 3) & 3][lane];
         const int maxn = a.maxn, lmax = a.lmax, two = a.two_levels;
+        const int push = a.push_max != 0;
         // all these loads issued before any is waited for
-        asm volatile("" ::"v"(hw), "v"(x), "s"(maxn), "s"(lmax), "s"(two));
+        asm volatile("" ::"v"(hw), "v"(x), "s"(maxn), "s"(lmax), "s"(two), "s"(push));
         const int p = (i + 1) & 1;
         // lane k's word as a scalar (the same in every lane)
         auto head = [&](int k) { return __builtin_amdgcn_readlane(hw, k); };
@@ -1407,7 +1474,8 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         switch (ph) {
         case PH_INIT: ph = PH_JUMP; r = 0; dpar = 0; break;
         case PH_JUMP: r++; break;
-        case PH_LABEL: ph = PH_SW1; break;
+        case PH_LABEL: ph = push ? PH_PUSH : PH_SW1; break;
+        case PH_PUSH: ph = PH_SW1; break;
         case PH_SW1: ph = PH_CTL1; break;
         case PH_CTL1: ph = any_outer ? PH_MARK0 : PH_INIT; break;
         case PH_MARK0: ph = PH_CTL0; break;   // switch(0) ran in the same launch
@@ -1484,7 +1552,11 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         label_body(a.N, a.maxn, a.lmax, glb(a.Sl), glb(a.Zl), glb(a.blast), gid1, glb(a.gc),
                    glb(a.strat), glb(a.solved), glb(a.lab), glb(a.dep), glb(a.key), glb(a.dig),
                    glb(a.tr), a.trace_cap, 1, a.trace, glb(a.jc), glb(a.blive), glb(a.hi), glb(a.chg),
-                   glb(a.chg0), 0);
+                   glb(a.chg0), 0, a.push_max ? pg->ccnt : nullptr);
+        break;
+    case PH_PUSH:
+        push_body(a.N, glb(a.chg), glb(a.rin_off), glb(a.rin_src), glb(a.dm0), glb(a.dm1), a.push_max,
+                  pg);
         break;
     case PH_MARK0:
         // mark_solved(0), then switch(0) in the same launch: the switch reads
@@ -1497,10 +1569,20 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
     case PH_SW1:
     case PH_SW0: {                                // one inlined copy of the switch
         const int pl = ph == PH_SW1 ? 1 : 0;
+        // change marks (one game): only marked entries unless a valuation
+        // since this player's last pass set none, or player 1's strategy
+        // was reset (then every entry is scanned)
+        uint8_t* dq = nullptr;
+        int use_d = 0;
+        if (a.push_max) {
+            dq = pl ? glb(a.dm1) : glb(a.dm0);
+            use_d = !(pl ? pg->full1 : pg->full0) && !(pl && glb(a.gc)[0].reset_pending);
+            if (pl && blockIdx.x == 0 && threadIdx.x < kSlots) pg->ccnt[threadIdx.x * kCntStride] = 0;
+        }
         switch_body(pl, a.N, pl ? glb(a.own1) : glb(a.own0), pl ? a.own_n1 : a.own_n0, glb(a.vinfo),
                     glb(a.off), glb(a.tgt), gid1, glb(a.gc), glb(a.strat), glb(a.key),
                     glb(a.dep), glb(a.par), glb(a.Sl), glb(a.jc), 1, glb(a.flag), nullptr, glb(a.hi),
-                    glb(a.chg), glb(a.chg0));
+                    glb(a.chg), glb(a.chg0), dq, use_d);
         break;
     }
     case PH_CTL1: ctl1_body(a.G, glb(a.gc), pg); break;
@@ -1527,8 +1609,11 @@ __global__ void k_prog_reset(int G, const GameCtl* __restrict__ gc, StepProg* __
         pg->psum[0] = pg->psum[1] = 0;
         pg->any_outer = 0;
         pg->live = acc;
+        pg->full1 = 1;             // marks start over: the first passes scan everything
+        pg->full0 = 1;
     }
     for (int k = threadIdx.x; k < 4 * kSlots; k += blockDim.x) pg->pslot[k / kSlots][k % kSlots] = 0;
+    for (int k = threadIdx.x; k < kSlots; k += blockDim.x) pg->ccnt[k * kCntStride] = 0;
 }
 
 __global__ void k_count_live(int G, const GameCtl* gc, int* out)
@@ -1699,6 +1784,79 @@ __global__ void k_owner_scatter(long long N, const int* __restrict__ vinfo, cons
     }
 }
 
+// In-edge lists of one game (the switch's change marks): in-degree counts,
+// an exclusive scan, then each edge placed through an atomic cursor per
+// target (the order within a list does not matter); an in-edge is stored as
+// its source's owner-list entry, index << 1 | owner.
+__global__ void k_rin_count(long long N, const int* __restrict__ off, const int* __restrict__ tgt,
+                            int* __restrict__ cnt)
+{
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    for (int e = off[v]; e < off[v + 1]; e++) atomicAdd(&cnt[tgt[e]], 1);
+}
+
+__global__ void k_owner_index(long long n, const int* __restrict__ own, int* __restrict__ idx)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) idx[own[i]] = (int)i;
+}
+
+__global__ void k_rin_fill(long long N, const int* __restrict__ vinfo, const int* __restrict__ off,
+                           const int* __restrict__ tgt, const int* __restrict__ idx,
+                           int* __restrict__ cur, int* __restrict__ src)
+{
+    long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const int code = (idx[v] << 1) | owner_of(vinfo[v]);
+    for (int e = off[v]; e < off[v + 1]; e++) src[atomicAdd(&cur[tgt[e]], 1)] = code;
+}
+
+// In-edge lists and change marks of a one-game core; on failure (memory)
+// the core runs without marks.
+static void build_in_edges(Core& c, hipStream_t s)
+{
+    const long long N = c.N, M = c.M;
+    if (N <= 0 || N >= (1LL << 30)) return;
+    int *cnt = nullptr, *idx = nullptr;
+    unsigned char* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    bool ok = dalloc(&c.rin_off, N + 1) == PSI_OK && dalloc(&c.rin_src, std::max(M, 1LL)) == PSI_OK &&
+              dalloc(&c.dmark[0], c.own_n[0] + 64) == PSI_OK &&
+              dalloc(&c.dmark[1], c.own_n[1] + 64) == PSI_OK && dalloc(&cnt, N + 1) == PSI_OK &&
+              dalloc(&idx, N) == PSI_OK &&
+              hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, c.rin_off, (int)(N + 1), s) ==
+                  hipSuccess &&
+              dalloc(&tmp, (long long)tmp_bytes) == PSI_OK;
+    const unsigned nb = nblocks(N, kBS);
+    if (ok) {
+        ok = hipMemsetAsync(cnt, 0, (N + 1) * 4, s) == hipSuccess &&
+             hipMemsetAsync(c.dmark[0], 0, c.own_n[0] + 64, s) == hipSuccess &&
+             hipMemsetAsync(c.dmark[1], 0, c.own_n[1] + 64, s) == hipSuccess;
+        k_rin_count<<<nb, kBS, 0, s>>>(N, c.off, c.tgt, cnt);
+        ok = ok && hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, c.rin_off, (int)(N + 1), s) ==
+                       hipSuccess;
+        ok = ok && hipMemcpyAsync(cnt, c.rin_off, N * 4, hipMemcpyDeviceToDevice, s) == hipSuccess;
+        for (int p = 0; p < 2; p++)
+            if (c.own_n[p] > 0)
+                k_owner_index<<<nblocks(c.own_n[p], kBS), kBS, 0, s>>>(c.own_n[p], c.own[p], idx);
+        k_rin_fill<<<nb, kBS, 0, s>>>(N, c.vinfo, c.off, c.tgt, idx, cnt, c.rin_src);
+        ok = ok && hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+    }
+    (void)hipFree(cnt);
+    (void)hipFree(idx);
+    (void)hipFree(tmp);
+    if (!ok) {
+        (void)hipGetLastError();
+        for (void* p : {(void*)c.rin_off, (void*)c.rin_src, (void*)c.dmark[0], (void*)c.dmark[1]})
+            if (p) (void)hipFree(p);
+        c.rin_off = c.rin_src = nullptr;
+        c.dmark[0] = c.dmark[1] = nullptr;
+        return;
+    }
+    c.push_ok = true;
+}
+
 int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap, hipStream_t s)
 {
     PhaseTimer pt;
@@ -1833,6 +1991,10 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
     (void)hipFree(d_ebase);
     (void)hipFree(d_cnt);
     pt.mark("pack on device");
+    if (G == 1) {
+        build_in_edges(c, s);
+        pt.mark("in-edge lists");
+    }
     return PSI_OK;
 }
 
@@ -2051,7 +2213,8 @@ void core_destroy(Core& c)
     if (c.pgraph) (void)hipGraphExecDestroy(c.pgraph);
     void* ptrs[] = {c.vinfo, c.off, c.tgt, c.gid, c.strat, c.par, c.dep, c.key, c.solved, c.lab, c.blive,
                     c.bact, c.blast, c.Sl, c.own[0], c.own[1], c.Zl, c.dig, c.tr, c.gctl, c.jc, c.flag,
-                    c.cnt_scratch, c.chg, c.succ, c.prog, c.pargs};
+                    c.cnt_scratch, c.chg, c.succ, c.prog, c.pargs, c.rin_off, c.rin_src, c.dmark[0],
+                    c.dmark[1]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c = Core();
@@ -2357,6 +2520,13 @@ int core_launch_steps(Core& c, int steps_per_check)
     return PSI_OK;
 }
 
+// Change marks are set when at most this many vertices changed (0: never).
+static long long push_max_of(const Core& c)
+{
+    if (!c.rin_off || !c.push_ok || g_si_tune.push <= 0) return 0;
+    return std::max(1LL, c.N / g_si_tune.push);
+}
+
 static StepArgs step_args(Core& c)
 {
     StepArgs a;
@@ -2376,6 +2546,11 @@ static StepArgs step_args(Core& c)
     a.gid = c.gid;
     a.own0 = c.own[0];
     a.own1 = c.own[1];
+    a.rin_off = c.rin_off;
+    a.rin_src = c.rin_src;
+    a.dm0 = c.dmark[0];
+    a.dm1 = c.dmark[1];
+    a.push_max = push_max_of(c);
     a.hi = c.flag + 2;
     a.gc = c.gctl;
     a.strat = c.strat;
@@ -2411,6 +2586,7 @@ static int upload_step_args(Core& c, StepArgs** out)
     PSI_HIP(hipMemcpyAsync(c.pargs, &a, sizeof a, hipMemcpyHostToDevice, c.s));
     PSI_HIP(hipStreamSynchronize(c.s));
     c.pargs_two = a.two_levels;
+    c.pargs_push = a.push_max;
     *out = (StepArgs*)c.pargs;
     return PSI_OK;
 }
@@ -2428,7 +2604,8 @@ static unsigned program_grid(const Core& c)
 // captured on first use.
 static int launch_program(Core& c, int launches)
 {
-    if (c.pgraph && c.pargs_two != g_si_tune.two_levels) {   // tuned since the capture
+    if (c.pgraph && (c.pargs_two != g_si_tune.two_levels || c.pargs_push != push_max_of(c))) {
+        // tuned (or a state loaded) since the capture
         StepArgs* a = nullptr;
         if (int rc = upload_step_args(c, &a)) return rc;
     }
@@ -2505,6 +2682,7 @@ int core_solve_program_profiled(Core& c, double* times, long long* counts)
         case PH_LABEL: cls = PSI_K_LABEL; break;
         case PH_SW1: cls = PSI_K_SWITCH1; break;
         case PH_MARK0: case PH_SW0: cls = PSI_K_SWITCH0; break;
+        case PH_PUSH: cls = PSI_K_PUSH; break;
         default: break;
         }
         times[cls] += ms;

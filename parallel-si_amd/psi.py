@@ -284,6 +284,7 @@ def list_rank_status():
 
 SIT_JUMP_GRID, SIT_STEP_GRID, SIT_POOL_GRID, SIT_PHASE_GRID, SIT_PROGRAM = 1, 2, 3, 4, 5
 SIT_TWO_LEVELS = 6
+SIT_PUSH, SIT_PUSH_DEFAULT = 7, 16
 _sig("psi_si_tune", C.c_int, C.c_int, C.c_int)
 
 
@@ -472,7 +473,7 @@ class Solver:
         return buf[: 4 * k.value].reshape(-1, 4)
 
 K_NAMES = ["val_init", "jump", "jump_idle", "label", "switch1", "switch0", "ctl", "total",
-           "gathers", "ties", "scans", "scan_steps"]
+           "gathers", "ties", "scans", "scan_steps", "push"]
 _sig("psi_solve_profiled", C.c_int, _vp, C.c_int, C.c_int,
      np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS"), _i64p)
 
@@ -523,7 +524,7 @@ def program_launches(solver, cap=1 << 16):
 
 
 PH_NAMES = ["start", "init", "jump", "label", "switch1", "ctl1", "mark0", "switch0", "ctl0", "done",
-            "act"]
+            "act", "push"]
 
 _sig("psi_get_jump_stats", C.c_int, _vp, _i32p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int))
 

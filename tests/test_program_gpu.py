@@ -16,6 +16,7 @@ def modes(gpu):
     yield gpu
     gpu.si_tune(gpu.SIT_PROGRAM, 1)
     gpu.si_tune(gpu.SIT_TWO_LEVELS, 1)
+    gpu.si_tune(gpu.SIT_PUSH, gpu.SIT_PUSH_DEFAULT)
 
 
 def _run(gpu, games, program, steps_per_sync=4, reset=False):
@@ -215,3 +216,51 @@ def test_program_grid_retuned_on_one_handle(modes):
     finally:
         gpu.si_tune(gpu.SIT_PHASE_GRID, 2048)
         s.close()
+
+
+# Change marks (one game): push 1 marks after every valuation, 16 (default)
+# after those with at most n/16 changes, 1 << 20 almost never, 0 never; the
+# switch passes then visit marked entries only.  Results equal the oracle's
+# and the program's without marks, round by round.
+@pytest.mark.parametrize("idx", range(3))
+@pytest.mark.parametrize("push", [1, 16, 1 << 20])
+def test_program_change_marks(modes, idx, push):
+    gpu = modes
+    g = _games(gpu)[idx]
+    print(f"change marks push={push}, game {idx}", flush=True)
+    gpu.si_tune(gpu.SIT_PUSH, 0)
+    a = _run(gpu, g, 1)
+    gpu.si_tune(gpu.SIT_PUSH, push)
+    b = _run(gpu, g, 1)
+    _same(a, b)
+    r = oracle.solve(g, trace=True)
+    assert b[0][0][:2] == (r.iter, r.br)
+    assert np.array_equal(b[0][1], r.strat) and np.array_equal(b[0][2], r.inf)
+    assert np.array_equal(b[0][4], r.hashes[: len(b[0][4])])
+
+
+@pytest.mark.parametrize("push", [1, 16])
+def test_program_change_marks_reset_mode(modes, push):
+    gpu = modes
+    g = gpu.gen_random_game(21, 40000, 30000, 2, 6)
+    gpu.si_tune(gpu.SIT_PUSH, push)
+    b = _run(gpu, g, 1, reset=True)
+    r = oracle.solve(g, reset=True, trace=True)
+    assert b[0][0][:2] == (r.iter, r.br)
+    assert np.array_equal(b[0][1], r.strat) and np.array_equal(b[0][2], r.inf)
+    assert np.array_equal(b[0][4], r.hashes[: len(b[0][4])])
+
+
+def test_program_change_marks_repeated_solves(modes):
+    """Marks left by one solve are harmless to the next on the same handle,
+    with the marks threshold retuned in between."""
+    gpu = modes
+    g = gpu.gen_random_game(22, 50000, 1 << 20, 2, 8)
+    r = oracle.solve(g)
+    s = gpu.Solver(g)
+    for push in (1, 16, 1, 0, 4):
+        gpu.si_tune(gpu.SIT_PUSH, push)
+        s.solve()
+        assert s.stats()[:2] == (r.iter, r.br), push
+        assert np.array_equal(s.strategy(), r.strat), push
+    s.close()
