@@ -264,3 +264,22 @@ def test_program_change_marks_repeated_solves(modes):
         assert s.stats()[:2] == (r.iter, r.br), push
         assert np.array_equal(s.strategy(), r.strat), push
     s.close()
+
+
+def test_program_marks_ties_and_duplicates(modes):
+    """Few priorities (many equal keys: exact compares) and a game with
+    repeated successors, with and without change marks."""
+    gpu = modes
+    games = [gpu.gen_random_game(31, 20000, 4, 2, 8), gpu.gen_random_game(32, 30000, 16, 1, 3)]
+    dup = gpu.gen_random_game(33, 5000, 50, 2, 5)
+    t = dup.targets.copy()
+    t[1::3] = t[0::3][: len(t[1::3])]        # some successors listed twice
+    games.append(gpu.Game(dup.owner, dup.priority, dup.offsets, t))
+    for g in games:
+        r = oracle.solve(g, trace=True)
+        for push in (0, 1, 16):
+            gpu.si_tune(gpu.SIT_PUSH, push)
+            b = _run(gpu, g, 1)
+            assert b[0][0][:2] == (r.iter, r.br), push
+            assert np.array_equal(b[0][1], r.strat) and np.array_equal(b[0][2], r.inf), push
+            assert np.array_equal(b[0][4], r.hashes[: len(b[0][4])]), push
