@@ -342,6 +342,10 @@ int psi_pool_info(psi_pool* p, int* running, int* queued, int64_t* vertex_cap, i
  * passes visit only marked entries (else every entry); 0 never marks.
  * Results are identical. */
 #define PSI_SIT_PUSH 7
+/* PSI_SIT_PUSH_MIN_N: the marks of PSI_SIT_PUSH on games of at least this
+ * many vertices (default 1048576; their in-edge lists are built when the
+ * handle is created, so this applies to handles created afterwards). */
+#define PSI_SIT_PUSH_MIN_N 8
 int psi_si_tune(int param, int value);
 
 /* ---- seeded generators (host only, no device calls) --------------------- */

@@ -215,6 +215,11 @@ extern "C" int psi_si_tune(int param, int value)
         g_si_tune.two_levels = value;
         return PSI_OK;
     }
+    if (param == PSI_SIT_PUSH_MIN_N) {
+        if (value < 0) return fail(PSI_ERR_INVALID, "psi_si_tune: push_min_n is 0 or positive");
+        g_si_tune.push_min_n = value;
+        return PSI_OK;
+    }
     if (param == PSI_SIT_PUSH) {
         if (value < 0) return fail(PSI_ERR_INVALID, "psi_si_tune: push is 0 or positive");
         g_si_tune.push = value;

@@ -87,8 +87,8 @@ struct StepProg {
     unsigned long long pslot[4][kSlots];
     unsigned ccnt[kSlots * kCntStride]; // vertices whose valuation changed at the last
                                         // valuation, over kSlots slots a cache line apart
-    int full1;      // the next switch(1) scans every entry (the last valuation set no marks)
-    int full0;      // the next switch(0) does (some valuation since the last one set none)
+    int full0;      // the next switch(0) scans every entry (a valuation since its last pass
+                    // set no marks)
 };
 
 struct Core {
@@ -127,6 +127,7 @@ struct Core {
     int* rin_off = nullptr;            // N+1
     int* rin_src = nullptr;            // M
     uint8_t* dmark[2] = {nullptr, nullptr};
+    unsigned char* rin_mem = nullptr;  // the allocation these live in
     bool push_ok = false;              // marks usable (cleared when a state is loaded)
     unsigned long long* Zl = nullptr;  // trace: digests per jump level (as Sl)
     unsigned long long* dig = nullptr;
@@ -168,6 +169,8 @@ struct SiTuning {
     int two_levels = 1;          // step program: jump launches advance two levels
     int push = 16;               // step program, one game: marks when at most N / push
                                  // valuations changed (0: never)
+    int push_min_n = 1 << 20;    // ... on games of at least this many vertices (the in-edge
+                                 // lists are built at creation for those only)
 };
 extern SiTuning g_si_tune;
 

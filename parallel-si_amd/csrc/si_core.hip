@@ -1357,27 +1357,17 @@ enum Phase {
     PH_PUSH
 };
 
-// One game, after the labels: when at most push_max vertices changed their
-// valuation, mark the owner-list entries of their in-neighbours (plain byte
-// stores of 1: writers of one mark agree) for the next switch(1) (player 1)
-// and switch(0) (player 0, kept until that pass); otherwise the next
-// switch(1) and switch(0) scan every entry.
+// One game, after the labels, when at most push_max vertices changed their
+// valuation (the step program runs this phase only then): mark the
+// owner-list entries of their in-neighbours (plain byte stores of 1: writers
+// of one mark agree) for the next switch(1) (player 1) and switch(0) (player
+// 0, kept until that pass).  After a valuation with more changes, the next
+// switch(1) and switch(0) scan every entry instead.
 __device__ __forceinline__ void push_body(long long N, const unsigned long long* __restrict__ chg,
                                           const int* __restrict__ rin_off,
                                           const int* __restrict__ rin_src, uint8_t* __restrict__ d0,
-                                          uint8_t* __restrict__ d1, long long push_max,
-                                          StepProg* __restrict__ pg)
+                                          uint8_t* __restrict__ d1)
 {
-    unsigned t = pg->ccnt[(threadIdx.x & (kSlots - 1)) * kCntStride];
-    for (int o = kSlots / 2; o; o >>= 1) t += __shfl_xor(t, o);
-    if ((long long)t > push_max) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            pg->full1 = 1;
-            pg->full0 = 1;
-        }
-        return;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) pg->full1 = 0;
     const long long nw = (N + 63) / 64;
     for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < nw;
          w += (long long)gridDim.x * blockDim.x) {
@@ -1448,7 +1438,7 @@ __global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8, 8))
 k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
 {
     const StepArgs& a = *ap;
-    __shared__ int ph_s, r_s, dbl_s, dpar_s;
+    __shared__ int ph_s, r_s, dbl_s, dpar_s, marked_s;
     __shared__ unsigned int cnt[2];
     if (threadIdx.x < kWave) {                    // the first wave decides
         const int lane = threadIdx.x;
@@ -1458,9 +1448,12 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         const int hw = reinterpret_cast<const int*>(pg)[min(lane, kProgHeadWords - 1)];
         const unsigned long long x = pg->pslot[(i + 3) & 3][lane];
         const int maxn = a.maxn, lmax = a.lmax, two = a.two_levels;
-        const int push = a.push_max != 0;
+        const long long push_max = a.push_max;
+        // the last valuation's changed-vertex count, one slot per lane
+        // (written by its label launch, zeroed by the next val_init)
+        const unsigned cc = pg->ccnt[lane * kCntStride];
         // all these loads issued before any is waited for
-        asm volatile("" ::"v"(hw), "v"(x), "s"(maxn), "s"(lmax), "s"(two), "s"(push));
+        asm volatile("" ::"v"(hw), "v"(x), "v"(cc), "s"(maxn), "s"(lmax), "s"(two));
         const int p = (i + 1) & 1;
         // lane k's word as a scalar (the same in every lane)
         auto head = [&](int k) { return __builtin_amdgcn_readlane(hw, k); };
@@ -1474,7 +1467,14 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         switch (ph) {
         case PH_INIT: ph = PH_JUMP; r = 0; dpar = 0; break;
         case PH_JUMP: r++; break;
-        case PH_LABEL: ph = push ? PH_PUSH : PH_SW1; break;
+        case PH_LABEL: {
+            // marks for the switches when few valuations changed (every
+            // workgroup sums the same slots, so all agree)
+            unsigned t = cc;
+            for (int o = kSlots / 2; o; o >>= 1) t += __shfl_xor(t, o);
+            ph = push_max && (long long)t <= push_max ? PH_PUSH : PH_SW1;
+            break;
+        }
         case PH_PUSH: ph = PH_SW1; break;
         case PH_SW1: ph = PH_CTL1; break;
         case PH_CTL1: ph = any_outer ? PH_MARK0 : PH_INIT; break;
@@ -1521,6 +1521,7 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
             r_s = r;
             dbl_s = dbl;
             dpar_s = dpar;
+            marked_s = (prev & 0x7f) == PH_PUSH;  // this switch(1) has marks
             cnt[0] = cnt[1] = 0;
         }
     }
@@ -1530,6 +1531,9 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
     const int* gid1 = a.G == 1 ? nullptr : glb(a.gid);
     switch (ph) {
     case PH_INIT:
+        // the coming valuation counts its changes afresh (every reader of the
+        // last count ran before this launch)
+        if (a.push_max && blockIdx.x == 0 && threadIdx.x < kSlots) pg->ccnt[threadIdx.x * kCntStride] = 0;
         val_init_body(a.N, glb(a.vinfo), glb(a.off), glb(a.tgt), gid1, glb(a.gc), glb(a.strat),
                       glb(a.solved), glb(a.lab), glb(a.par), glb(a.Sl), glb(a.Zl), glb(a.jc),
                       glb(a.blive), glb(a.bact), glb(a.blast), a.trace, glb(a.hi));
@@ -1555,8 +1559,7 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
                    glb(a.chg0), 0, a.push_max ? pg->ccnt : nullptr);
         break;
     case PH_PUSH:
-        push_body(a.N, glb(a.chg), glb(a.rin_off), glb(a.rin_src), glb(a.dm0), glb(a.dm1), a.push_max,
-                  pg);
+        push_body(a.N, glb(a.chg), glb(a.rin_off), glb(a.rin_src), glb(a.dm0), glb(a.dm1));
         break;
     case PH_MARK0:
         // mark_solved(0), then switch(0) in the same launch: the switch reads
@@ -1576,8 +1579,9 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         int use_d = 0;
         if (a.push_max) {
             dq = pl ? glb(a.dm1) : glb(a.dm0);
-            use_d = !(pl ? pg->full1 : pg->full0) && !(pl && glb(a.gc)[0].reset_pending);
-            if (pl && blockIdx.x == 0 && threadIdx.x < kSlots) pg->ccnt[threadIdx.x * kCntStride] = 0;
+            use_d = (pl ? marked_s : !pg->full0) && !(pl && glb(a.gc)[0].reset_pending);
+            // a valuation without marks: switch(0) scans everything at its next pass
+            if (pl && !marked_s && blockIdx.x == 0 && threadIdx.x == 0) pg->full0 = 1;
         }
         switch_body(pl, a.N, pl ? glb(a.own1) : glb(a.own0), pl ? a.own_n1 : a.own_n0, glb(a.vinfo),
                     glb(a.off), glb(a.tgt), gid1, glb(a.gc), glb(a.strat), glb(a.key),
@@ -1609,8 +1613,7 @@ __global__ void k_prog_reset(int G, const GameCtl* __restrict__ gc, StepProg* __
         pg->psum[0] = pg->psum[1] = 0;
         pg->any_outer = 0;
         pg->live = acc;
-        pg->full1 = 1;             // marks start over: the first passes scan everything
-        pg->full0 = 1;
+        pg->full0 = 1;             // marks start over: the first passes scan everything
     }
     for (int k = threadIdx.x; k < 4 * kSlots; k += blockDim.x) pg->pslot[k / kSlots][k % kSlots] = 0;
     for (int k = threadIdx.x; k < kSlots; k += blockDim.x) pg->ccnt[k * kCntStride] = 0;
@@ -1818,38 +1821,43 @@ static void build_in_edges(Core& c, hipStream_t s)
 {
     const long long N = c.N, M = c.M;
     if (N <= 0 || N >= (1LL << 30)) return;
-    int *cnt = nullptr, *idx = nullptr;
-    unsigned char* tmp = nullptr;
+    // One allocation for the lists, the marks and the build's scratch (kept
+    // until the handle is destroyed: no free, hence no wait, at creation).
     size_t tmp_bytes = 0;
-    bool ok = dalloc(&c.rin_off, N + 1) == PSI_OK && dalloc(&c.rin_src, std::max(M, 1LL)) == PSI_OK &&
-              dalloc(&c.dmark[0], c.own_n[0] + 64) == PSI_OK &&
-              dalloc(&c.dmark[1], c.own_n[1] + 64) == PSI_OK && dalloc(&cnt, N + 1) == PSI_OK &&
-              dalloc(&idx, N) == PSI_OK &&
-              hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, c.rin_off, (int)(N + 1), s) ==
-                  hipSuccess &&
-              dalloc(&tmp, (long long)tmp_bytes) == PSI_OK;
-    const unsigned nb = nblocks(N, kBS);
-    if (ok) {
-        ok = hipMemsetAsync(cnt, 0, (N + 1) * 4, s) == hipSuccess &&
-             hipMemsetAsync(c.dmark[0], 0, c.own_n[0] + 64, s) == hipSuccess &&
-             hipMemsetAsync(c.dmark[1], 0, c.own_n[1] + 64, s) == hipSuccess;
-        k_rin_count<<<nb, kBS, 0, s>>>(N, c.off, c.tgt, cnt);
-        ok = ok && hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, c.rin_off, (int)(N + 1), s) ==
-                       hipSuccess;
-        ok = ok && hipMemcpyAsync(cnt, c.rin_off, N * 4, hipMemcpyDeviceToDevice, s) == hipSuccess;
-        for (int p = 0; p < 2; p++)
-            if (c.own_n[p] > 0)
-                k_owner_index<<<nblocks(c.own_n[p], kBS), kBS, 0, s>>>(c.own_n[p], c.own[p], idx);
-        k_rin_fill<<<nb, kBS, 0, s>>>(N, c.vinfo, c.off, c.tgt, idx, cnt, c.rin_src);
-        ok = ok && hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, (int*)nullptr, (int*)nullptr, (int)(N + 1), s) !=
+        hipSuccess)
+        return;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t b_off = al((N + 1) * 4), b_src = al(std::max(M, 1LL) * 4);
+    const size_t b_d0 = al(c.own_n[0] + 64), b_d1 = al(c.own_n[1] + 64);
+    const size_t b_cnt = al((N + 1) * 4), b_idx = al(N * 4), b_tmp = al(std::max<size_t>(tmp_bytes, 1));
+    if (dalloc(&c.rin_mem, (long long)(b_off + b_src + b_d0 + b_d1 + b_cnt + b_idx + b_tmp)) != PSI_OK) {
+        (void)hipGetLastError();
+        c.rin_mem = nullptr;
+        return;
     }
-    (void)hipFree(cnt);
-    (void)hipFree(idx);
-    (void)hipFree(tmp);
+    unsigned char* q = c.rin_mem;
+    c.rin_off = (int*)q;            q += b_off;
+    c.rin_src = (int*)q;            q += b_src;
+    c.dmark[0] = q;                 q += b_d0;
+    c.dmark[1] = q;                 q += b_d1;
+    int* cnt = (int*)q;             q += b_cnt;
+    int* idx = (int*)q;             q += b_idx;
+    unsigned char* tmp = q;
+    const unsigned nb = nblocks(N, kBS);
+    bool ok = hipMemsetAsync(cnt, 0, (N + 1) * 4, s) == hipSuccess &&
+              hipMemsetAsync(c.dmark[0], 0, b_d0 + b_d1, s) == hipSuccess;
+    k_rin_count<<<nb, kBS, 0, s>>>(N, c.off, c.tgt, cnt);
+    ok = ok && hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, c.rin_off, (int)(N + 1), s) == hipSuccess;
+    ok = ok && hipMemcpyAsync(cnt, c.rin_off, N * 4, hipMemcpyDeviceToDevice, s) == hipSuccess;
+    for (int p = 0; p < 2; p++)
+        if (c.own_n[p] > 0) k_owner_index<<<nblocks(c.own_n[p], kBS), kBS, 0, s>>>(c.own_n[p], c.own[p], idx);
+    k_rin_fill<<<nb, kBS, 0, s>>>(N, c.vinfo, c.off, c.tgt, idx, cnt, c.rin_src);
+    ok = ok && hipGetLastError() == hipSuccess;
     if (!ok) {
         (void)hipGetLastError();
-        for (void* p : {(void*)c.rin_off, (void*)c.rin_src, (void*)c.dmark[0], (void*)c.dmark[1]})
-            if (p) (void)hipFree(p);
+        (void)hipFree(c.rin_mem);
+        c.rin_mem = nullptr;
         c.rin_off = c.rin_src = nullptr;
         c.dmark[0] = c.dmark[1] = nullptr;
         return;
@@ -1991,7 +1999,7 @@ int core_create(Core& c, int G, const GameView* games, bool trace, int trace_cap
     (void)hipFree(d_ebase);
     (void)hipFree(d_cnt);
     pt.mark("pack on device");
-    if (G == 1) {
+    if (G == 1 && g_si_tune.push > 0 && N >= g_si_tune.push_min_n) {
         build_in_edges(c, s);
         pt.mark("in-edge lists");
     }
@@ -2213,8 +2221,7 @@ void core_destroy(Core& c)
     if (c.pgraph) (void)hipGraphExecDestroy(c.pgraph);
     void* ptrs[] = {c.vinfo, c.off, c.tgt, c.gid, c.strat, c.par, c.dep, c.key, c.solved, c.lab, c.blive,
                     c.bact, c.blast, c.Sl, c.own[0], c.own[1], c.Zl, c.dig, c.tr, c.gctl, c.jc, c.flag,
-                    c.cnt_scratch, c.chg, c.succ, c.prog, c.pargs, c.rin_off, c.rin_src, c.dmark[0],
-                    c.dmark[1]};
+                    c.cnt_scratch, c.chg, c.succ, c.prog, c.pargs, c.rin_mem};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c = Core();
@@ -2509,8 +2516,11 @@ int core_launch_steps(Core& c, int steps_per_check)
         PSI_HIP(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
         int rc = core_enqueue_steps(c, steps_per_check);
         hipError_t e = hipStreamEndCapture(c.s, &gr);
-        if (rc) return rc;
-        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        if (e != hipSuccess) return rc ? rc : hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        if (rc) {                                 // the captured graph is not used
+            (void)hipGraphDestroy(gr);
+            return rc;
+        }
         e = hipGraphInstantiate(&c.graph, gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);
         if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
@@ -2623,8 +2633,14 @@ static int launch_program(Core& c, int launches)
         for (int i = 0; i < launches; i++) k_phase<<<grid, kBS, 0, c.s>>>(i, a, c.prog);
         hipError_t e = hipGetLastError();
         hipError_t e2 = hipStreamEndCapture(c.s, &gr);
-        if (e != hipSuccess) return hip_fail(e, "k_phase launch", __FILE__, __LINE__);
-        if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture", __FILE__, __LINE__);
+        if (e2 != hipSuccess) {
+            return e != hipSuccess ? hip_fail(e, "k_phase launch", __FILE__, __LINE__)
+                                   : hip_fail(e2, "hipStreamEndCapture", __FILE__, __LINE__);
+        }
+        if (e != hipSuccess) {                    // the captured graph is not used
+            (void)hipGraphDestroy(gr);
+            return hip_fail(e, "k_phase launch", __FILE__, __LINE__);
+        }
         e = hipGraphInstantiate(&c.pgraph, gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);
         if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);

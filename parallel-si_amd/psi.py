@@ -285,6 +285,7 @@ def list_rank_status():
 SIT_JUMP_GRID, SIT_STEP_GRID, SIT_POOL_GRID, SIT_PHASE_GRID, SIT_PROGRAM = 1, 2, 3, 4, 5
 SIT_TWO_LEVELS = 6
 SIT_PUSH, SIT_PUSH_DEFAULT = 7, 16
+SIT_PUSH_MIN_N, SIT_PUSH_MIN_N_DEFAULT = 8, 1 << 20
 _sig("psi_si_tune", C.c_int, C.c_int, C.c_int)
 
 

@@ -13,7 +13,9 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture
 def modes(gpu):
+    gpu.si_tune(gpu.SIT_PUSH_MIN_N, 0)      # change marks on these small games too
     yield gpu
+    gpu.si_tune(gpu.SIT_PUSH_MIN_N, gpu.SIT_PUSH_MIN_N_DEFAULT)
     gpu.si_tune(gpu.SIT_PROGRAM, 1)
     gpu.si_tune(gpu.SIT_TWO_LEVELS, 1)
     gpu.si_tune(gpu.SIT_PUSH, gpu.SIT_PUSH_DEFAULT)
