@@ -10,7 +10,8 @@
  *
  * Threading: a handle is used by one host thread at a time; distinct handles
  * may be used concurrently.  The standalone list-ranking entry points share
- * one library workspace and must not be called concurrently.
+ * one library workspace: concurrent calls of psi_list_rank take turns, and
+ * concurrent calls of psi_list_rank_noalloc race (its own note).
  */
 #ifndef PSI_H
 #define PSI_H

@@ -876,11 +876,15 @@ static int run(const int* d_succ, int n, const int* d_heads, int n_heads, int* d
 
 using namespace psi;
 
+// psi_list_rank's calls take turns (one workspace, on the library stream).
+static std::mutex g_lr_mu;
+
 extern "C" int psi_list_rank(const int* d_succ, int n, const int* d_heads, int n_heads, int* d_rank,
                              int method)
 {
     if (n < 0 || n_heads < 0) return fail(PSI_ERR_INVALID, "psi_list_rank: negative size");
     if (n == 0) return PSI_OK;
+    std::lock_guard<std::mutex> lk(g_lr_mu);
     if (int rc = lr::ensure_ctl()) return rc;
     return lr::run(d_succ, n, d_heads, n_heads, d_rank, method, lr::g_arena, lr::g_ctl, true,
                    stream());
@@ -987,6 +991,7 @@ extern "C" int psi_list_rank_host(const int* succ, int n, const int* heads, int 
 
 extern "C" int psi_list_rank_tune(int param, int value)
 {
+    std::lock_guard<std::mutex> lk(g_lr_mu);     // not while psi_list_rank plans with it
     auto& t = lr::g_tune;
     switch (param) {
     case PSI_LRT_LOG_D0: if (value < 1 || value > 16) break; t.log_d0 = value; return PSI_OK;

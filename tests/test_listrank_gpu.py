@@ -171,3 +171,31 @@ def test_full_size_forest(gpu):
     for method in ("AUTO", "HJ", "WYLLIE"):
         print(f"forest 16M method={method}", flush=True)
         _check(psi, succ, heads, method, expect)
+
+
+def test_concurrent_calls_take_turns(gpu):
+    """psi_list_rank_host from several host threads at once (ctypes releases
+    the GIL): the calls share one workspace and take turns; every result is
+    exact."""
+    import threading
+    rng = np.random.default_rng(77)
+    cases = [perm_list(rng, n) for n in (50000, 200000, 1000, 300001, 4097, 120000)]
+    expect = [oracle.list_rank(s, h) for s, h in cases]
+    got = [None] * len(cases)
+    errors = []
+
+    def work(k):
+        try:
+            for _ in range(3):
+                got[k] = gpu.list_rank_host(cases[k][0], cases[k][1], gpu.LR_AUTO)
+        except Exception as e:          # reported below
+            errors.append(e)
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(len(cases))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors
+    for k in range(len(cases)):
+        assert np.array_equal(got[k], expect[k]), k
