@@ -75,8 +75,9 @@ struct JumpCnt {
 // launches, each running the next phase of the loop), indexed by launch number
 // i so that launch i+1 finds launch i's words without a dependent load:
 // st[i & 1] = the phase launch i ran (phase | round << 8); psum[i & 1] = its
-// jump-round decision word; pslot[i & 3] = its jump round's counters (zeroed
-// by launch i-2).  A graph of these launches holds a multiple of 4.
+// jump-round decision word; pslot[i & 3] = its jump round's counters and
+// pmis[i & 3] its cycle-label mismatch flags (both zeroed by launch i-2).  A
+// graph of these launches holds a multiple of 4.
 struct StepProg {
     int st[2];
     int any_outer;  // some game takes the outer step (set by the control phase)
@@ -85,6 +86,7 @@ struct StepProg {
     int dev_seq[2]; // by launch parity: admissions activated as of that launch
     unsigned long long psum[2];
     unsigned long long pslot[4][kSlots];
+    unsigned pmis[4][kSlots];
     unsigned ccnt[kSlots * kCntStride]; // vertices whose valuation changed at the last
                                         // valuation, over kSlots slots a cache line apart
     int full0;      // the next switch(0) scans every entry (a valuation since its last pass

@@ -99,17 +99,23 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, unsigned l
     }
 }
 
-// Phase bookkeeping for round r: stop = first round that resolved nothing
-// (-1 if none yet), cyc = vertices still jumping after it (all lead into
-// cycles).  The state through round r-2 comes from the summary written by
-// round r-1; round r-1 itself is read from its slots.  No kernel reads a word
-// that its own blocks write.
+// Round summary word (written by the launch that decided round r, read by the
+// next one): bit 63 halt, bit 62 the launch counted cycle-label mismatches,
+// bits 32..60 stop + 1, bits 0..31 cyc.
+constexpr int kSumChecked = 62;
+__device__ __forceinline__ int sum_stop(unsigned long long sm) { return (int)((sm >> 32) & 0x1FFFFFFFu) - 1; }
+
+// Phase bookkeeping for round r: stop = first round that resolved nothing, or
+// after which nothing was still jumping (-1 if none yet), cyc = vertices still
+// jumping after it (all lead into cycles).  The state through round r-2 comes
+// from the summary written by round r-1; round r-1 itself is read from its
+// slots.  No kernel reads a word that its own blocks write.
 __device__ __forceinline__ void round_state(int r, const JumpCnt* jc, int* stop, int* cyc, int* halt)
 {
     // summary[(r-1)&1] was written by block 0 of round r-1 (a previous kernel)
     unsigned long long sm = r >= 1 ? jc->summary[(r - 1) & 1] : 0ull;
     *halt = (int)(sm >> 63);
-    int st = (int)((sm >> 32) & 0x7FFFFFFFu) - 1, cy = (int)(sm & 0xFFFFFFFFu);
+    int st = sum_stop(sm), cy = (int)(sm & 0xFFFFFFFFu);
     if (r >= 1 && st < 0 && !*halt) {
         unsigned long long res = 0, act = 0;
         for (int k = 0; k < kSlots; k++) {
@@ -117,7 +123,7 @@ __device__ __forceinline__ void round_state(int r, const JumpCnt* jc, int* stop,
             res += x >> 32;
             act += x & 0xFFFFFFFFu;
         }
-        if (res == 0) {
+        if (res == 0 || act == 0) {
             st = r - 1;
             cy = (int)act;
         }
@@ -148,15 +154,32 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x)
 
 // decide_round by a whole wave: sm = the summary word of round r-1, x = this
 // lane's slot word of round r-1 (lane k: slot k; kSlots == kWave), both loaded
-// together.  Every lane returns the same.
-__device__ __forceinline__ int decide_wave(int r, unsigned long long sm, unsigned long long x, int maxn,
-                                           int* stop, int* cyc)
+// together; mis = nonzero if launch r-1 found a cycle-label mismatch (the same
+// in every lane).  Every lane returns the same.
+//
+// Cycle labels (two-level launches, jump2_body).  For a vertex v still jumping
+// at the launch's end, let lambda(v) be the parity of the largest priority in
+// the level-r window that starts at x_r(v), v's 2^r-th successor: the label a
+// valuation ending at level r gives v.  The launch's first two gather steps
+// read the level-r records at x_r(v) and at x_r(x_r(v)), so it compares
+// lambda(v) with lambda(x_r(v)) at no cost and flags any difference.  Once the
+// stop has happened by the launch's end, the vertices still jumping are
+// exactly the cycle-bound ones, and with no difference every label is already
+// right: along a cycle the level-r windows at x_r(v), x_r(x_r(v)), ... follow
+// one another and cover the cycle, so one of them holds the cycle's largest
+// priority and the equal bits carry its parity to every cycle vertex; a tail
+// vertex reaches the cycle through x_r, so it has the same bit.  A window at a
+// later level is a union of such windows, so the labels stay right at every
+// later level, and the rounds stop.
+__device__ __forceinline__ int decide_wave(int r, unsigned long long sm, unsigned long long x, unsigned mis,
+                                           int maxn, int* stop, int* cyc)
 {
     const int halt = (int)(sm >> 63);
-    int st = (int)((sm >> 32) & 0x7FFFFFFFu) - 1, cy = (int)(sm & 0xFFFFFFFFu);
+    const int checked = (int)((sm >> kSumChecked) & 1);
+    int st = sum_stop(sm), cy = (int)(sm & 0xFFFFFFFFu);
     if (r >= 1 && st < 0 && !halt) {               // uniform: sm is the same in every lane
         const unsigned long long res = wave_sum_u64(x >> 32), act = wave_sum_u64(x & 0xFFFFFFFFu);
-        if (res == 0) {
+        if (res == 0 || act == 0) {
             st = r - 1;
             cy = (int)act;
         }
@@ -165,13 +188,14 @@ __device__ __forceinline__ int decide_wave(int r, unsigned long long sm, unsigne
     *cyc = cy;
     if (halt || r >= kMaxRounds) return 0;
     if (r == 0 || st < 0) return 1;
+    if (checked && !mis) return 0;                // cycle labels consistent
     return (r < 30 && (1 << r) < min(cy, maxn)) ? 1 : 0;
 }
 
-__device__ __forceinline__ unsigned long long summary_word(int run, int stop, int cyc)
+__device__ __forceinline__ unsigned long long summary_word(int run, int stop, int cyc, int checked = 0)
 {
-    return ((unsigned long long)(run ? 0 : 1) << 63) | ((unsigned long long)(stop + 1) << 32) |
-           (unsigned)cyc;
+    return ((unsigned long long)(run ? 0 : 1) << 63) | ((unsigned long long)checked << kSumChecked) |
+           ((unsigned long long)(stop + 1) << 32) | (unsigned)cyc;
 }
 
 // ---------------------------------------------------------------------------
@@ -495,7 +519,9 @@ __device__ __forceinline__ int2 join_rec(int2 s, int2 t, int span)
 // level as one round (levels r, r + 1 and r + 2 are in different buffers).
 // The counts are round r + 1's (resolved at level r + 2,
 // still jumping after it): a stop seen there is a stop at r or r + 1, and
-// only whether there was one is ever used.
+// only whether there was one is ever used.  A vertex still jumping at level
+// r + 2 whose cycle-label bit differs from that of its 2^r-th successor
+// (decide_wave) sets mis_row[block % kSlots]; cnt[2] is a zeroed shared flag.
 __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict__ Sl,
                                            unsigned long long* __restrict__ Zl,
                                            unsigned long long* __restrict__ slot_row,
@@ -503,7 +529,8 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
                                            const uint8_t* __restrict__ bact_in,
                                            uint8_t* __restrict__ bact_out,
                                            uint8_t* __restrict__ blast, int trace,
-                                           const int* __restrict__ hi, unsigned int* cnt)
+                                           const int* __restrict__ hi, unsigned int* cnt,
+                                           unsigned* __restrict__ mis_row)
 {
     unsigned int& cnt_res = cnt[0];
     unsigned int& cnt_act = cnt[1];
@@ -555,6 +582,7 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
             // once (unconditional, clamped indices, as in one round).
             int2 s[kRows];
             bool act1[kRows];                     // still jumping at level r + 1
+            int lp = 0;                           // bit q: row q's lambda(v) xor lambda(x_r(v))
 #pragma unroll
             for (int q = 0; q < kRows; q++) {
                 const long long v = base + q * kWave;
@@ -580,6 +608,10 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
                     }
                 }
                 const int sp = (step == 2 ? 3 : step + 1) * span;   // vertices s spans so far
+                // the level-r windows at x_r(v) (step 0) and x_r(x_r(v)) (step 1)
+                if (step < 2)
+#pragma unroll
+                    for (int q = 0; q < kRows; q++) lp ^= (t[q].y & 1) << q;
 #pragma unroll
                 for (int q = 0; q < kRows; q++)
                     if (s[q].x >= 0) s[q] = join_rec(s[q], t[q], sp);
@@ -596,6 +628,7 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
                 if (!__any(more)) break;
             }
             unsigned long long any = 0;
+            bool mis = false;
 #pragma unroll
             for (int q = 0; q < kRows; q++) {
                 const long long v = base + q * kWave;
@@ -607,7 +640,9 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
                 my_res += (unsigned)__popcll(__ballot(res));
                 my_act += (unsigned)__popcll(ma);
                 any |= ma;
+                mis |= act && ((lp >> q) & 1);
             }
+            if (__any(mis) && lane == 0) cnt[2] = 1;
             if (lane == 0) {
                 bact_out[b] = (uint8_t)(any != 0);
                 blast[b] = (uint8_t)(r + 2);
@@ -621,6 +656,7 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
     __syncthreads();
     if (threadIdx.x == 0 && (cnt_res | cnt_act))
         atomicAdd(&slot_row[blockIdx.x & (kSlots - 1)], ((unsigned long long)cnt_res << 32) | cnt_act);
+    if (threadIdx.x == 0 && cnt[2]) mis_row[blockIdx.x & (kSlots - 1)] = 1;
 }
 
 __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int2* __restrict__ Sl,
@@ -638,7 +674,7 @@ __global__ void __launch_bounds__(kBS) k_jump(int r, long long N, int maxn, int2
         const unsigned long long sm = r >= 1 ? jc->summary[(r - 1) & 1] : 0ull;
         const unsigned long long x = r >= 1 ? jc->slot[r - 1][threadIdx.x] : 0ull;
         int stop, cyc;
-        const int run = decide_wave(r, sm, x, maxn, &stop, &cyc);
+        const int run = decide_wave(r, sm, x, 1u, maxn, &stop, &cyc);
         if (threadIdx.x == 0) {
             run_s = run;
             cnt[0] = cnt[1] = 0;
@@ -1439,7 +1475,7 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
 {
     const StepArgs& a = *ap;
     __shared__ int ph_s, r_s, dbl_s, dpar_s, marked_s;
-    __shared__ unsigned int cnt[2];
+    __shared__ unsigned int cnt[3];
     if (threadIdx.x < kWave) {                    // the first wave decides
         const int lane = threadIdx.x;
         // Every word launch i-1 left (phase, round summary, control words)
@@ -1447,13 +1483,14 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         // holds word k of the state's head, every lane its slot word.
         const int hw = reinterpret_cast<const int*>(pg)[min(lane, kProgHeadWords - 1)];
         const unsigned long long x = pg->pslot[(i + 3) & 3][lane];
+        const unsigned mi = pg->pmis[(i + 3) & 3][lane];
         const int maxn = a.maxn, lmax = a.lmax, two = a.two_levels;
         const long long push_max = a.push_max;
         // the last valuation's changed-vertex count, one slot per lane
         // (written by its label launch, zeroed by the next val_init)
         const unsigned cc = pg->ccnt[lane * kCntStride];
         // all these loads issued before any is waited for
-        asm volatile("" ::"v"(hw), "v"(x), "v"(cc), "s"(maxn), "s"(lmax), "s"(two));
+        asm volatile("" ::"v"(hw), "v"(x), "v"(mi), "v"(cc), "s"(maxn), "s"(lmax), "s"(two));
         const int p = (i + 1) & 1;
         // lane k's word as a scalar (the same in every lane)
         auto head = [&](int k) { return __builtin_amdgcn_readlane(hw, k); };
@@ -1494,7 +1531,9 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
             // round r's decision from round r-1 (launch i-1); a round at lmax
             // would mean the valuation needs more rounds than it has levels
             int stop, cyc;
-            int run = decide_wave(r, r >= 1 ? sm : 0ull, x, maxn, &stop, &cyc);
+            unsigned mo = mi;                     // launch i-1's mismatch flags, any slot
+            for (int o = kSlots / 2; o; o >>= 1) mo |= __shfl_xor(mo, o);
+            int run = decide_wave(r, r >= 1 ? sm : 0ull, x, mo, maxn, &stop, &cyc);
             if (run && r >= lmax) {
                 if (blockIdx.x == 0 && lane == 0) a.jc->err = 1;
                 run = 0;
@@ -1503,13 +1542,14 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
             // r + 2 exists; the next launch then takes round r + 2.
             dbl = run && two && r + 2 <= lmax;
             if (blockIdx.x == 0 && lane == 0) {
-                pg->psum[i & 1] = summary_word(run, stop, cyc);
+                pg->psum[i & 1] = summary_word(run, stop, cyc, dbl);
                 if (run) a.jc->last = r + dbl;
             }
             if (!run) ph = PH_LABEL;
         }
         if (blockIdx.x == 0) {
             pg->pslot[(i + 2) & 3][lane] = 0;     // launch i+2 counts its round there
+            pg->pmis[(i + 2) & 3][lane] = 0;
             if (lane == 0) {
                 pg->st[i & 1] = ph | ((dpar ^ dbl) << 7) | ((r + dbl) << 8);
                 if (ph == PH_SW1) pg->any_outer = 0;   // set again by CTL1
@@ -1522,7 +1562,7 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
             dbl_s = dbl;
             dpar_s = dpar;
             marked_s = (prev & 0x7f) == PH_PUSH;  // this switch(1) has marks
-            cnt[0] = cnt[1] = 0;
+            cnt[0] = cnt[1] = cnt[2] = 0;
         }
     }
     __syncthreads();
@@ -1545,7 +1585,7 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         if (dbl_s)
             jump2_body(r, a.N, glb(a.Sl), glb(a.Zl), pg->pslot[i & 3], glb(a.blive),
                        glb(a.bact) + pin * a.nb, glb(a.bact) + (pin ^ 1) * a.nb, glb(a.blast),
-                       a.trace, glb(a.hi), cnt);
+                       a.trace, glb(a.hi), cnt, pg->pmis[i & 3]);
         else
             jump_body(r, a.N, glb(a.Sl), glb(a.Zl), pg->pslot[i & 3], glb(a.blive),
                       glb(a.bact) + pin * a.nb, glb(a.bact) + (pin ^ 1) * a.nb, glb(a.blast),
@@ -1615,7 +1655,10 @@ __global__ void k_prog_reset(int G, const GameCtl* __restrict__ gc, StepProg* __
         pg->live = acc;
         pg->full0 = 1;             // marks start over: the first passes scan everything
     }
-    for (int k = threadIdx.x; k < 4 * kSlots; k += blockDim.x) pg->pslot[k / kSlots][k % kSlots] = 0;
+    for (int k = threadIdx.x; k < 4 * kSlots; k += blockDim.x) {
+        pg->pslot[k / kSlots][k % kSlots] = 0;
+        pg->pmis[k / kSlots][k % kSlots] = 0;
+    }
     for (int k = threadIdx.x; k < kSlots; k += blockDim.x) pg->ccnt[k * kCntStride] = 0;
 }
 
