@@ -85,6 +85,8 @@ struct StepProg {
     int host_seq;   // serving pool: admissions written by the host (between graphs)
     int dev_seq[2]; // by launch parity: admissions activated as of that launch
     unsigned long long psum[2];
+    int pushed;     // the last switch(1) ran on change marks (few valuations changed)
+    int bf0;        // game 0's mode (bf of its control block) as the program started
     unsigned long long pslot[4][kSlots];
     unsigned pmis[4][kSlots];
     unsigned ccnt[kSlots * kCntStride]; // vertices whose valuation changed at the last
@@ -129,6 +131,16 @@ struct Core {
     int* rin_off = nullptr;            // N+1
     int* rin_src = nullptr;            // M
     uint8_t* dmark[2] = {nullptr, nullptr};
+    // Step program, one game: d1 marks (a byte per vertex, the valuation's
+    // tag) the vertices whose link changed for the coming valuation and
+    // their link-children; l12ok
+    // (per block) says the block's level-1 and level-2 jump records were
+    // written by the last valuation's first jump launch.  A first jump launch
+    // after a switch on change marks takes a vertex's level-1 and level-2
+    // records from the last valuation unless the vertex or its 2nd successor
+    // is marked (then the first four vertices of its path are the same).
+    uint8_t* d1 = nullptr;
+    uint8_t* l12ok = nullptr;
     unsigned char* rin_mem = nullptr;  // the allocation these live in
     bool push_ok = false;              // marks usable (cleared when a state is loaded)
     unsigned long long* Zl = nullptr;  // trace: digests per jump level (as Sl)

@@ -249,6 +249,40 @@ __device__ __forceinline__ void issued(int a, int b, int c, int d, int e, int f)
     asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d), "v"(e), "v"(f));
 }
 
+// Tag of the d1 marks of a valuation (game 0's br at its INIT): 1..255, so
+// marks left by earlier valuations need no clearing (a stale mark whose tag
+// recurs only adds work: the vertex is computed rather than kept).
+__host__ __device__ __forceinline__ int d1_tag(int br) { return br % 255 + 1; }
+
+// Marks v (whose link changed for the coming valuation) and its
+// link-children in the d1 map with the valuation's tag: the in-edge sources
+// s of v whose link is v (unsolved, strategy v; the strategies are final
+// during an INIT).  Plain byte stores: writers of one byte agree.
+__device__ __forceinline__ void d1_mark(int v, int tag, uint8_t* __restrict__ d1, const int* __restrict__ rin_off,
+                                        const int* __restrict__ rin_src, const int* __restrict__ own0,
+                                        const int* __restrict__ own1, const int* __restrict__ strat,
+                                        const int8_t* __restrict__ solved)
+{
+    d1[v] = (uint8_t)tag;
+    const int e1 = rin_off[v + 1];
+    for (int e = rin_off[v]; e < e1; e += 8) {
+        int src[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) src[j] = e + j < e1 ? rin_src[e + j] : -1;
+#pragma unroll
+        for (int j = 0; j < 8; j++) src[j] = src[j] < 0 ? -1 : ((src[j] & 1) ? own1 : own0)[src[j] >> 1];
+        int st[8], so[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            st[j] = src[j] >= 0 ? strat[src[j]] : -1;
+            so[j] = src[j] >= 0 ? solved[src[j]] : 1;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (st[j] == v && !so[j]) d1[src[j]] = (uint8_t)tag;
+    }
+}
+
 // Level-0 jump records.  S = (next or terminal code, max priority, count, -).
 // Terminal codes: -1 sink, -2 solved +inf, -3 solved -inf, -4 finished game.
 __device__ __forceinline__ void val_init_body(
@@ -257,7 +291,9 @@ __device__ __forceinline__ void val_init_body(
     int* __restrict__ strat, const int8_t* __restrict__ solved, const int8_t* __restrict__ lab,
     int* __restrict__ par, int2* __restrict__ S0, unsigned long long* __restrict__ Z0,
     JumpCnt* __restrict__ jc, uint8_t* __restrict__ blive, uint8_t* __restrict__ bact0,
-    uint8_t* __restrict__ blast, int trace, const int* __restrict__ hi)
+    uint8_t* __restrict__ blast, int trace, const int* __restrict__ hi, uint8_t* __restrict__ d1 = nullptr,
+    const int* __restrict__ rin_off = nullptr, const int* __restrict__ rin_src = nullptr,
+    const int* __restrict__ own0 = nullptr, const int* __restrict__ own1 = nullptr)
 {
     if (blockIdx.x == 0) {
         unsigned long long* sl = &jc->slot[0][0];
@@ -281,6 +317,7 @@ __device__ __forceinline__ void val_init_body(
         const long long base = b * kBS + lane;
         unsigned long long mlive[kRows];
         unsigned long long any = 0, anyj = 0;
+        unsigned linkchg = 0;                     // bit per row: this lane's vertex has a new link
 #pragma unroll
         for (int h = 0; h < kRows; h += kInitRows) {
             int g[kInitRows], vi[kInitRows], s[kInitRows], old[kInitRows], sv[kInitRows], lv[kInitRows];
@@ -328,9 +365,11 @@ __device__ __forceinline__ void val_init_body(
                         const int code = term_code(lv[q] == 1 ? 1 : 2, 0);
                         S0[v] = make_int2(code, (first | (old[q] != code)) ? kChangedBit : 0);
                         if (old[q] != code) par[v] = code;   // (unchanged links: no store)
+                        if (old[q] != code) linkchg |= 1u << (h + q);
                         if (trace) Z0[v] = 0;
                     } else {
                         if (old[q] != sq) par[v] = sq;
+                        if (old[q] != sq) linkchg |= 1u << (h + q);
                         S0[v] = make_int2(sq >= 0 ? sq : term_code(0, 1),
                                           pri_of(vi[q]) | ((first | (old[q] != sq)) ? kChangedBit : 0));
                         if (trace) Z0[v] = splitmix64((unsigned long long)(uint32_t)pri_of(vi[q]));
@@ -360,6 +399,11 @@ __device__ __forceinline__ void val_init_body(
             bact0[b] = (uint8_t)(anyj != 0);
             blast[b] = 0;
         }
+        if (d1)
+#pragma unroll 1
+            for (unsigned m = linkchg; m; m &= m - 1)
+                d1_mark((int)(base + (__ffs(m) - 1) * kWave), d1_tag(gc[0].br), d1, rin_off, rin_src, own0, own1,
+                        strat, solved);
     }
 }
 
@@ -530,8 +574,17 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
                                            uint8_t* __restrict__ bact_out,
                                            uint8_t* __restrict__ blast, int trace,
                                            const int* __restrict__ hi, unsigned int* cnt,
-                                           unsigned* __restrict__ mis_row)
+                                           unsigned* __restrict__ mis_row, const uint8_t* __restrict__ d1 = nullptr,
+                                           uint8_t* __restrict__ l12ok = nullptr, int tag = 0)
 {
+    // Round 0 with l12ok (one game, step program): the launch records in
+    // l12ok which blocks it wrote levels 1 and 2 of; with d1 as well (the
+    // INIT marked the vertices with a new link and their link-children), a
+    // vertex of such a block whose level-1 and level-2 windows hold no new
+    // link (it is unmarked and so is its 2nd successor, read from its
+    // level-1 record) keeps both records from the last valuation, with the
+    // change bit cleared.
+    const bool rec_ok = r == 0 && l12ok != nullptr;
     unsigned int& cnt_res = cnt[0];
     unsigned int& cnt_act = cnt[1];
     const int2* Sin = Sl + (size_t)level_slot(r) * N;
@@ -551,12 +604,18 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
         const long long bq = b0 + (long long)lane * G;
         const long long bc = min(bq, nbk - 1);
         const int f_live = blive[bc], f_act = bact_in[bc];
+        const int f_ok = rec_ok && d1 ? l12ok[bc] : 0;
         const bool on = bq < nbk && f_live && f_act;
-        if (threadIdx.x < kWave && bq < nbk && !on) bact_out[bq] = 0;
+        if (threadIdx.x < kWave && bq < nbk && !on) {
+            bact_out[bq] = 0;
+            if (rec_ok) l12ok[bq] = 0;
+        }
         unsigned long long m = __ballot(on);
+        const unsigned long long mok = __ballot(f_ok != 0);
         for (int k = 0; m; m &= m - 1, k++) {
             if ((k & (kRows - 1)) != wid) continue;
-            const long long b = b0 + (long long)(__ffsll((long long)m) - 1) * G;
+            const int bi = __ffsll((long long)m) - 1;
+            const long long b = b0 + (long long)bi * G;
             const long long base = b * kBS + lane;
             if (trace) {
                 // trace digests, row by row, from level r alone (as below)
@@ -583,11 +642,39 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
             int2 s[kRows];
             bool act1[kRows];                     // still jumping at level r + 1
             int lp = 0;                           // bit q: row q's lambda(v) xor lambda(x_r(v))
+            unsigned cmp = (1u << kRows) - 1;     // bit q: row q's vertex is computed here
 #pragma unroll
             for (int q = 0; q < kRows; q++) {
                 const long long v = base + q * kWave;
                 s[q] = Sin[min(v, N - 1)];
                 if (v >= N) s[q].x = -1;
+            }
+            if ((mok >> bi) & 1) {
+                // the marks and level-1 records of the rows, then the marks
+                // of the 2nd successors
+                int d[kRows];
+                int2 o1[kRows];
+#pragma unroll
+                for (int q = 0; q < kRows; q++) {
+                    const long long vc = min(base + q * kWave, N - 1);
+                    d[q] = d1[vc] == tag;
+                    o1[q] = Sl[(size_t)level_slot(1) * N + vc];
+                }
+#pragma unroll
+                for (int q = 0; q < kRows; q++)
+                    if (o1[q].x >= 0) d[q] |= d1[o1[q].x] == tag;
+#pragma unroll
+                for (int q = 0; q < kRows; q++) {
+                    const long long v = base + q * kWave;
+                    if (v < N && !d[q]) {
+                        cmp &= ~(1u << q);
+                        const int2 o2 = Sl[(size_t)level_slot(2) * N + v];
+                        if (o1[q].y & kChangedBit) Sl[(size_t)level_slot(1) * N + v].y = o1[q].y & ~kChangedBit;
+                        if (o2.y & kChangedBit) Sl[(size_t)level_slot(2) * N + v].y = o2.y & ~kChangedBit;
+                        act1[q] = o1[q].x >= 0;
+                        s[q] = o2;
+                    }
+                }
             }
 #pragma unroll 1
             for (int step = 0; step < 3; step++) {
@@ -595,13 +682,13 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
                 int bl[kRows];
 #pragma unroll
                 for (int q = 0; q < kRows; q++) {
-                    const int x = max(s[q].x, 0);
+                    const int x = (cmp >> q) & 1 ? max(s[q].x, 0) : 0;
                     bl[q] = blast[x / kBS];
                     t[q] = Sl[(size_t)level_slot(r) * N + x];
                 }
 #pragma unroll
                 for (int q = 0; q < kRows; q++) {
-                    const bool again = s[q].x >= 0 && bl[q] < r;
+                    const bool again = (cmp >> q) & 1 && s[q].x >= 0 && bl[q] < r;
                     if (__any(again)) {
                         const int2 u = Sl[(size_t)level_slot(again ? bl[q] : r) * N + (again ? s[q].x : 0)];
                         if (again) t[q] = u;
@@ -614,17 +701,19 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
                     for (int q = 0; q < kRows; q++) lp ^= (t[q].y & 1) << q;
 #pragma unroll
                 for (int q = 0; q < kRows; q++)
-                    if (s[q].x >= 0) s[q] = join_rec(s[q], t[q], sp);
+                    if ((cmp >> q) & 1 && s[q].x >= 0) s[q] = join_rec(s[q], t[q], sp);
                 if (step == 0)
 #pragma unroll
                     for (int q = 0; q < kRows; q++) {
                         const long long v = base + q * kWave;
-                        if (v < N) Sout1[v] = s[q];           // level r + 1
-                        act1[q] = s[q].x >= 0;
+                        if ((cmp >> q) & 1) {
+                            if (v < N) Sout1[v] = s[q];       // level r + 1
+                            act1[q] = s[q].x >= 0;
+                        }
                     }
                 bool more = false;
 #pragma unroll
-                for (int q = 0; q < kRows; q++) more |= s[q].x >= 0;
+                for (int q = 0; q < kRows; q++) more |= (cmp >> q) & 1 && s[q].x >= 0;
                 if (!__any(more)) break;
             }
             unsigned long long any = 0;
@@ -632,7 +721,7 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
 #pragma unroll
             for (int q = 0; q < kRows; q++) {
                 const long long v = base + q * kWave;
-                if (v < N) Sout2[v] = s[q];                   // level r + 2
+                if (v < N && (cmp >> q) & 1) Sout2[v] = s[q];   // level r + 2
                 const bool act = s[q].x >= 0;
                 // resolved in round r + 1: jumping at level r + 1, not at r + 2
                 const bool res = act1[q] && !act;
@@ -640,12 +729,14 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
                 my_res += (unsigned)__popcll(__ballot(res));
                 my_act += (unsigned)__popcll(ma);
                 any |= ma;
-                mis |= act && ((lp >> q) & 1);
+                // a kept record has no cycle-label check: counted as a mismatch
+                mis |= act && (((lp >> q) & 1) || !((cmp >> q) & 1));
             }
             if (__any(mis) && lane == 0) cnt[2] = 1;
             if (lane == 0) {
                 bact_out[b] = (uint8_t)(any != 0);
                 blast[b] = (uint8_t)(r + 2);
+                if (rec_ok) l12ok[b] = 1;
             }
         }
     }
@@ -1440,6 +1531,7 @@ struct StepArgs {
     const int *rin_off, *rin_src;
     uint8_t *dm0, *dm1;   // change marks of the owner lists
     long long push_max;   // 0: no marks (PH_PUSH never runs)
+    uint8_t *d1, *l12ok;  // reuse of level-1/2 records (with push_max; else null)
     GameCtl* gc;
     int *strat, *par, *dep, *key, *flag, *succ;
     int8_t *solved, *lab;
@@ -1461,11 +1553,12 @@ template <class T> __device__ __forceinline__ T* glb(T* const& field)
 }
 
 // the head of StepProg as 32-bit words: st[2], any_outer, live, host_seq,
-// dev_seq[2], (padding), psum[2]
-constexpr int kProgHeadWords = 12;
+// dev_seq[2], (padding), psum[2], pushed, bf0
+constexpr int kProgHeadWords = 14;
 static_assert(offsetof(StepProg, any_outer) == 8 && offsetof(StepProg, live) == 12 &&
                   offsetof(StepProg, host_seq) == 16 && offsetof(StepProg, dev_seq) == 20 &&
-                  offsetof(StepProg, psum) == 32 && offsetof(StepProg, pslot) == 4 * kProgHeadWords,
+                  offsetof(StepProg, psum) == 32 && offsetof(StepProg, pushed) == 48 &&
+                  offsetof(StepProg, bf0) == 52 && offsetof(StepProg, pslot) == 4 * kProgHeadWords,
               "StepProg head layout");
 
 // The arguments live in device memory (one pointer argument): each phase
@@ -1474,12 +1567,12 @@ __global__ void __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8, 8))
 k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
 {
     const StepArgs& a = *ap;
-    __shared__ int ph_s, r_s, dbl_s, dpar_s, marked_s;
+    __shared__ int ph_s, r_s, dbl_s, dpar_s, marked_s, reuse_s;
     __shared__ unsigned int cnt[3];
     if (threadIdx.x < kWave) {                    // the first wave decides
         const int lane = threadIdx.x;
         // Every word launch i-1 left (phase, round summary, control words)
-        // and its slot row, in two vector loads issued together: lane k < 12
+        // and its slot row, in two vector loads issued together: lane k < 14
         // holds word k of the state's head, every lane its slot word.
         const int hw = reinterpret_cast<const int*>(pg)[min(lane, kProgHeadWords - 1)];
         const unsigned long long x = pg->pslot[(i + 3) & 3][lane];
@@ -1499,8 +1592,10 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         const int hseq = head(4), dseq = head(5 + p);                     // host_seq, dev_seq[p]
         const unsigned long long sm =
             ((unsigned long long)(unsigned)head(9 + 2 * p) << 32) | (unsigned)head(8 + 2 * p);   // psum[p]
-        // st = phase | bact parity offset << 7 | round << 8
+        // st = phase | bact parity offset << 7 | round << 8 (an INIT's
+        // "round" says that it marked the changed links in d1)
         int ph = prev & 0x7f, r = prev >> 8, dbl = 0, dpar = (prev >> 7) & 1;
+        const int reuse = ph == PH_INIT && (r & 1);   // this launch is that valuation's round 0
         switch (ph) {
         case PH_INIT: ph = PH_JUMP; r = 0; dpar = 0; break;
         case PH_JUMP: r++; break;
@@ -1526,6 +1621,11 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         if (ph == PH_INIT) {
             r = 0;
             if (live <= 0) ph = PH_DONE;
+            // after a switch(1) on change marks (few valuations changed) and
+            // no outer step: mark the changed links for the first round's
+            // record reuse (one game, plain semantics, no trace)
+            if (ph == PH_INIT && a.d1 && !a.trace && (prev & 0x7f) == PH_CTL1 && head(12) && head(13) == 0)
+                r = 1;
         }
         if (ph == PH_JUMP) {
             // round r's decision from round r-1 (launch i-1); a round at lmax
@@ -1552,7 +1652,10 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
             pg->pmis[(i + 2) & 3][lane] = 0;
             if (lane == 0) {
                 pg->st[i & 1] = ph | ((dpar ^ dbl) << 7) | ((r + dbl) << 8);
-                if (ph == PH_SW1) pg->any_outer = 0;   // set again by CTL1
+                if (ph == PH_SW1) {
+                    pg->any_outer = 0;                 // set again by CTL1
+                    pg->pushed = (prev & 0x7f) == PH_PUSH;
+                }
                 pg->dev_seq[i & 1] = ph == PH_ACT ? hseq : dseq;
             }
         }
@@ -1563,6 +1666,7 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
             dpar_s = dpar;
             marked_s = (prev & 0x7f) == PH_PUSH;  // this switch(1) has marks
             cnt[0] = cnt[1] = cnt[2] = 0;
+            reuse_s = reuse;
         }
     }
     __syncthreads();
@@ -1576,7 +1680,8 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         if (a.push_max && blockIdx.x == 0 && threadIdx.x < kSlots) pg->ccnt[threadIdx.x * kCntStride] = 0;
         val_init_body(a.N, glb(a.vinfo), glb(a.off), glb(a.tgt), gid1, glb(a.gc), glb(a.strat),
                       glb(a.solved), glb(a.lab), glb(a.par), glb(a.Sl), glb(a.Zl), glb(a.jc),
-                      glb(a.blive), glb(a.bact), glb(a.blast), a.trace, glb(a.hi));
+                      glb(a.blive), glb(a.bact), glb(a.blast), a.trace, glb(a.hi),
+                      r ? glb(a.d1) : nullptr, glb(a.rin_off), glb(a.rin_src), glb(a.own0), glb(a.own1));
         break;
     case PH_JUMP: {
         // block flags of round r: after an odd number of two-round launches
@@ -1585,7 +1690,8 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         if (dbl_s)
             jump2_body(r, a.N, glb(a.Sl), glb(a.Zl), pg->pslot[i & 3], glb(a.blive),
                        glb(a.bact) + pin * a.nb, glb(a.bact) + (pin ^ 1) * a.nb, glb(a.blast),
-                       a.trace, glb(a.hi), cnt, pg->pmis[i & 3]);
+                       a.trace, glb(a.hi), cnt, pg->pmis[i & 3], reuse_s ? glb(a.d1) : nullptr,
+                       a.l12ok && !a.trace ? glb(a.l12ok) : nullptr, reuse_s ? d1_tag(glb(a.gc)[0].br) : 0);
         else
             jump_body(r, a.N, glb(a.Sl), glb(a.Zl), pg->pslot[i & 3], glb(a.blive),
                       glb(a.bact) + pin * a.nb, glb(a.bact) + (pin ^ 1) * a.nb, glb(a.blast),
@@ -1637,8 +1743,11 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
 }
 
 // Step program start: the next launch runs INIT; live = unfinished games.
-__global__ void k_prog_reset(int G, const GameCtl* __restrict__ gc, StepProg* __restrict__ pg)
+__global__ void k_prog_reset(int G, const GameCtl* __restrict__ gc, StepProg* __restrict__ pg,
+                             uint8_t* __restrict__ l12ok = nullptr, long long nb = 0)
 {
+    // no block's level-1 and level-2 records are taken as the last valuation's
+    for (long long b = threadIdx.x; l12ok && b < nb; b += blockDim.x) l12ok[b] = 0;
     __shared__ int acc;
     if (threadIdx.x == 0) acc = 0;
     __syncthreads();
@@ -1654,6 +1763,8 @@ __global__ void k_prog_reset(int G, const GameCtl* __restrict__ gc, StepProg* __
         pg->any_outer = 0;
         pg->live = acc;
         pg->full0 = 1;             // marks start over: the first passes scan everything
+        pg->pushed = 0;
+        pg->bf0 = G >= 1 ? gc[0].bf : 0;
     }
     for (int k = threadIdx.x; k < 4 * kSlots; k += blockDim.x) {
         pg->pslot[k / kSlots][k % kSlots] = 0;
@@ -1874,7 +1985,9 @@ static void build_in_edges(Core& c, hipStream_t s)
     const size_t b_off = al((N + 1) * 4), b_src = al(std::max(M, 1LL) * 4);
     const size_t b_d0 = al(c.own_n[0] + 64), b_d1 = al(c.own_n[1] + 64);
     const size_t b_cnt = al((N + 1) * 4), b_idx = al(N * 4), b_tmp = al(std::max<size_t>(tmp_bytes, 1));
-    if (dalloc(&c.rin_mem, (long long)(b_off + b_src + b_d0 + b_d1 + b_cnt + b_idx + b_tmp)) != PSI_OK) {
+    const size_t b_m1 = al(N), b_ok = al(nblocks(N, kBS));   // d1 marks, l12ok flags
+    if (dalloc(&c.rin_mem, (long long)(b_off + b_src + b_d0 + b_d1 + b_m1 + b_ok + b_cnt + b_idx + b_tmp)) !=
+        PSI_OK) {
         (void)hipGetLastError();
         c.rin_mem = nullptr;
         return;
@@ -1884,12 +1997,14 @@ static void build_in_edges(Core& c, hipStream_t s)
     c.rin_src = (int*)q;            q += b_src;
     c.dmark[0] = q;                 q += b_d0;
     c.dmark[1] = q;                 q += b_d1;
+    c.d1 = q;                       q += b_m1;
+    c.l12ok = q;                    q += b_ok;
     int* cnt = (int*)q;             q += b_cnt;
     int* idx = (int*)q;             q += b_idx;
     unsigned char* tmp = q;
     const unsigned nb = nblocks(N, kBS);
     bool ok = hipMemsetAsync(cnt, 0, (N + 1) * 4, s) == hipSuccess &&
-              hipMemsetAsync(c.dmark[0], 0, b_d0 + b_d1, s) == hipSuccess;
+              hipMemsetAsync(c.dmark[0], 0, b_d0 + b_d1 + b_m1 + b_ok, s) == hipSuccess;
     k_rin_count<<<nb, kBS, 0, s>>>(N, c.off, c.tgt, cnt);
     ok = ok && hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, c.rin_off, (int)(N + 1), s) == hipSuccess;
     ok = ok && hipMemcpyAsync(cnt, c.rin_off, N * 4, hipMemcpyDeviceToDevice, s) == hipSuccess;
@@ -1903,6 +2018,7 @@ static void build_in_edges(Core& c, hipStream_t s)
         c.rin_mem = nullptr;
         c.rin_off = c.rin_src = nullptr;
         c.dmark[0] = c.dmark[1] = nullptr;
+        c.d1 = c.l12ok = nullptr;
         return;
     }
     c.push_ok = true;
@@ -2604,6 +2720,8 @@ static StepArgs step_args(Core& c)
     a.dm0 = c.dmark[0];
     a.dm1 = c.dmark[1];
     a.push_max = push_max_of(c);
+    a.d1 = a.push_max ? c.d1 : nullptr;
+    a.l12ok = a.push_max ? c.l12ok : nullptr;
     a.hi = c.flag + 2;
     a.gc = c.gctl;
     a.strat = c.strat;
@@ -2711,7 +2829,7 @@ int core_solve_program_profiled(Core& c, double* times, long long* counts)
     hipEvent_t e0, e1;
     PSI_HIP(hipEventCreate(&e0));
     PSI_HIP(hipEventCreate(&e1));
-    k_prog_reset<<<1, 256, 0, c.s>>>(c.G, c.gctl, c.prog);
+    k_prog_reset<<<1, 256, 0, c.s>>>(c.G, c.gctl, c.prog, c.l12ok, nblocks(c.N, kBS));
     int rc = PSI_OK;
     c.prof_phase.clear();
     c.prof_us.clear();
@@ -2764,7 +2882,7 @@ int core_solve(Core& c, int steps_per_check)
         // ~16 launches per SI step on random games; an even count keeps the
         // state slots alternating across graphs
         const int launches = 16 * std::max(1, steps_per_check);   // a multiple of 4
-        k_prog_reset<<<1, 256, 0, c.s>>>(c.G, c.gctl, c.prog);
+        k_prog_reset<<<1, 256, 0, c.s>>>(c.G, c.gctl, c.prog, c.l12ok, nblocks(c.N, kBS));
         PSI_LAUNCH_CHECK();
         for (;;) {
             if (int rc = launch_program(c, launches)) return rc;
