@@ -650,31 +650,39 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
                 if (v >= N) s[q].x = -1;
             }
             if ((mok >> bi) & 1) {
-                // the marks and level-1 records of the rows, then the marks
-                // of the 2nd successors
-                int d[kRows];
-                int2 o1[kRows];
+                // The marks and level-1 records of the rows, then the marks
+                // of the 2nd successors, then the level-2 records: every load
+                // unconditional (the vertex's own, or vertex 0's mark for a
+                // path that ends within two vertices), so that the rows'
+                // loads are in flight together.  A terminal keeps its records
+                // too (copies of its unchanged level-0 record).
+                int m0[kRows], m2[kRows];
+                int2 o1[kRows], o2[kRows];
 #pragma unroll
                 for (int q = 0; q < kRows; q++) {
                     const long long vc = min(base + q * kWave, N - 1);
-                    d[q] = d1[vc] == tag;
+                    m0[q] = d1[vc];
                     o1[q] = Sl[(size_t)level_slot(1) * N + vc];
                 }
 #pragma unroll
-                for (int q = 0; q < kRows; q++)
-                    if (o1[q].x >= 0) d[q] |= d1[o1[q].x] == tag;
+                for (int q = 0; q < kRows; q++) m2[q] = d1[max(o1[q].x, 0)];
+                unsigned keep = 0;
 #pragma unroll
                 for (int q = 0; q < kRows; q++) {
                     const long long v = base + q * kWave;
-                    if (v < N && !d[q]) {
-                        cmp &= ~(1u << q);
-                        const int2 o2 = Sl[(size_t)level_slot(2) * N + v];
-                        if (o1[q].y & kChangedBit) Sl[(size_t)level_slot(1) * N + v].y = o1[q].y & ~kChangedBit;
-                        if (o2.y & kChangedBit) Sl[(size_t)level_slot(2) * N + v].y = o2.y & ~kChangedBit;
-                        act1[q] = o1[q].x >= 0;
-                        s[q] = o2;
-                    }
+                    if (v < N && m0[q] != tag && (o1[q].x < 0 || m2[q] != tag)) keep |= 1u << q;
+                    o2[q] = Sl[(size_t)level_slot(2) * N + min(v, N - 1)];
                 }
+#pragma unroll
+                for (int q = 0; q < kRows; q++) {
+                    if (!((keep >> q) & 1)) continue;
+                    const long long v = base + q * kWave;
+                    if (o1[q].y & kChangedBit) Sl[(size_t)level_slot(1) * N + v].y = o1[q].y & ~kChangedBit;
+                    if (o2[q].y & kChangedBit) Sl[(size_t)level_slot(2) * N + v].y = o2[q].y & ~kChangedBit;
+                    act1[q] = o1[q].x >= 0;
+                    s[q] = o2[q];
+                }
+                cmp &= ~keep;
             }
 #pragma unroll 1
             for (int step = 0; step < 3; step++) {
