@@ -141,6 +141,12 @@ struct Core {
     // is marked (then the first four vertices of its path are the same).
     uint8_t* d1 = nullptr;
     uint8_t* l12ok = nullptr;
+    // bchg (per block, step program, one game): some record of the block
+    // ended this valuation with the change bit (set by the INIT for its
+    // terminals, by the jump launches for what they resolve); the label
+    // launch only writes the change words of a block without it and without
+    // a vertex still jumping after the last round.
+    uint8_t* bchg = nullptr;
     unsigned char* rin_mem = nullptr;  // the allocation these live in
     bool push_ok = false;              // marks usable (cleared when a state is loaded)
     unsigned long long* Zl = nullptr;  // trace: digests per jump level (as Sl)

@@ -293,7 +293,8 @@ __device__ __forceinline__ void val_init_body(
     JumpCnt* __restrict__ jc, uint8_t* __restrict__ blive, uint8_t* __restrict__ bact0,
     uint8_t* __restrict__ blast, int trace, const int* __restrict__ hi, uint8_t* __restrict__ d1 = nullptr,
     const int* __restrict__ rin_off = nullptr, const int* __restrict__ rin_src = nullptr,
-    const int* __restrict__ own0 = nullptr, const int* __restrict__ own1 = nullptr)
+    const int* __restrict__ own0 = nullptr, const int* __restrict__ own1 = nullptr,
+    uint8_t* __restrict__ bchg = nullptr)
 {
     if (blockIdx.x == 0) {
         unsigned long long* sl = &jc->slot[0][0];
@@ -318,6 +319,7 @@ __device__ __forceinline__ void val_init_body(
         unsigned long long mlive[kRows];
         unsigned long long any = 0, anyj = 0;
         unsigned linkchg = 0;                     // bit per row: this lane's vertex has a new link
+        unsigned long long tchg = 0;              // a terminal whose record has the change bit
 #pragma unroll
         for (int h = 0; h < kRows; h += kInitRows) {
             int g[kInitRows], vi[kInitRows], s[kInitRows], old[kInitRows], sv[kInitRows], lv[kInitRows];
@@ -363,6 +365,7 @@ __device__ __forceinline__ void val_init_body(
                     // never reads the solved flags (cpu_list_ranking.cpp:242-255)
                     if (sv[q] && lsem[q] != 3) {
                         const int code = term_code(lv[q] == 1 ? 1 : 2, 0);
+                        if (first | (old[q] != code)) tchg |= 1ull;
                         S0[v] = make_int2(code, (first | (old[q] != code)) ? kChangedBit : 0);
                         if (old[q] != code) par[v] = code;   // (unchanged links: no store)
                         if (old[q] != code) linkchg |= 1u << (h + q);
@@ -370,6 +373,7 @@ __device__ __forceinline__ void val_init_body(
                     } else {
                         if (old[q] != sq) par[v] = sq;
                         if (old[q] != sq) linkchg |= 1u << (h + q);
+                        if (sq < 0 && (first | (old[q] != sq))) tchg |= 1ull;
                         S0[v] = make_int2(sq >= 0 ? sq : term_code(0, 1),
                                           pri_of(vi[q]) | ((first | (old[q] != sq)) ? kChangedBit : 0));
                         if (trace) Z0[v] = splitmix64((unsigned long long)(uint32_t)pri_of(vi[q]));
@@ -394,10 +398,12 @@ __device__ __forceinline__ void val_init_body(
                 if (v < N && !((mlive[q] >> lane) & 1)) S0[v] = make_int2(term_code(3, 0), 0);
             }
         }
+        const bool tc = __any(tchg != 0);
         if (lane == 0) {
             blive[b] = (uint8_t)(any != 0);
             bact0[b] = (uint8_t)(anyj != 0);
             blast[b] = 0;
+            if (bchg) bchg[b] = (uint8_t)tc;      // the jump rounds add resolved vertices
         }
         if (d1)
 #pragma unroll 1
@@ -435,7 +441,8 @@ __device__ __forceinline__ void jump_body(int r, long long N, int2* __restrict__
                                           const uint8_t* __restrict__ bact_in,
                                           uint8_t* __restrict__ bact_out,
                                           uint8_t* __restrict__ blast, int trace,
-                                          const int* __restrict__ hi, unsigned int* cnt)
+                                          const int* __restrict__ hi, unsigned int* cnt,
+                                          uint8_t* __restrict__ bchg = nullptr)
 {
     unsigned int& cnt_res = cnt[0];
     unsigned int& cnt_act = cnt[1];
@@ -511,7 +518,7 @@ __device__ __forceinline__ void jump_body(int r, long long N, int2* __restrict__
                     if (again) t[q] = u;
                 }
             }
-            unsigned long long any = 0;
+            unsigned long long any = 0, tchg = 0;  // tchg: a record that ended with the change bit
 #pragma unroll
             for (int q = 0; q < kRows; q++) {
                 const long long v = base + q * kWave;
@@ -528,10 +535,12 @@ __device__ __forceinline__ void jump_body(int r, long long N, int2* __restrict__
                 my_res += (unsigned)__popcll(mr);
                 my_act += (unsigned)__popcll(ma);
                 any |= ma;
+                if (bchg) tchg |= __ballot(v < N && s[q].x < 0 && (s[q].y & kChangedBit));
             }
             if (lane == 0) {
                 bact_out[b] = (uint8_t)(any != 0);
                 blast[b] = (uint8_t)(r + 1);
+                if (tchg) bchg[b] = 1;
             }
         }
     }
@@ -575,7 +584,8 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
                                            uint8_t* __restrict__ blast, int trace,
                                            const int* __restrict__ hi, unsigned int* cnt,
                                            unsigned* __restrict__ mis_row, const uint8_t* __restrict__ d1 = nullptr,
-                                           uint8_t* __restrict__ l12ok = nullptr, int tag = 0)
+                                           uint8_t* __restrict__ l12ok = nullptr, int tag = 0,
+                                           uint8_t* __restrict__ bchg = nullptr)
 {
     // Round 0 with l12ok (one game, step program): the launch records in
     // l12ok which blocks it wrote levels 1 and 2 of; with d1 as well (the
@@ -724,7 +734,7 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
                 for (int q = 0; q < kRows; q++) more |= (cmp >> q) & 1 && s[q].x >= 0;
                 if (!__any(more)) break;
             }
-            unsigned long long any = 0;
+            unsigned long long any = 0, tchg = 0;  // tchg: a computed record that ended with the change bit
             bool mis = false;
 #pragma unroll
             for (int q = 0; q < kRows; q++) {
@@ -739,12 +749,14 @@ __device__ __forceinline__ void jump2_body(int r, long long N, int2* __restrict_
                 any |= ma;
                 // a kept record has no cycle-label check: counted as a mismatch
                 mis |= act && (((lp >> q) & 1) || !((cmp >> q) & 1));
+                if (bchg) tchg |= __ballot(v < N && (cmp >> q) & 1 && !act && (s[q].y & kChangedBit));
             }
             if (__any(mis) && lane == 0) cnt[2] = 1;
             if (lane == 0) {
                 bact_out[b] = (uint8_t)(any != 0);
                 blast[b] = (uint8_t)(r + 2);
                 if (rec_ok) l12ok[b] = 1;
+                if (tchg) bchg[b] = 1;
             }
         }
     }
@@ -803,8 +815,17 @@ __device__ __forceinline__ void label_body(
     int trace_cap, int loop_form, int trace, JumpCnt* __restrict__ jc,
     const uint8_t* __restrict__ blive, const int* __restrict__ hi,
     unsigned long long* __restrict__ chg, unsigned long long* __restrict__ chg0, int check_rounds,
-    unsigned* __restrict__ ccnt)
+    unsigned* __restrict__ ccnt, const uint8_t* __restrict__ bchg = nullptr,
+    const uint8_t* __restrict__ bfin = nullptr)
 {
+    // Quiet blocks (one game, plain semantics, not its first valuation, no
+    // trace; bchg and bfin given): a block none of whose records ended with
+    // the change bit (bchg: set by the INIT for its terminals and by the
+    // jump launches for what they resolved) and with no vertex still jumping
+    // after the last round (bfin: none cycle-bound) holds no vertex whose
+    // valuation, label or solved flag changes, so only its change words are
+    // written (chg zero; chg0 zero after an outer step, else as it was).
+    const int quiet_clr = bchg ? gc[0].clear0 : 0;
     const int last = jc->last;
     unsigned wchg = 0;                            // this wave's changed vertices (ccnt)
     const int F = last + 1;
@@ -823,11 +844,22 @@ __device__ __forceinline__ void label_body(
     // together (unconditional, clamped indices; no barrier between waves).
     for (long long b0 = blockIdx.x; b0 < nbk; b0 += (long long)kWave * G) {
         const long long bq = b0 + (long long)lane * G;
-        const bool bl_live = blive[min(bq, nbk - 1)] != 0;
+        const long long bc = min(bq, nbk - 1);
+        const bool bl_live = blive[bc] != 0;
+        const bool quiet = bchg && !bchg[bc] && !bfin[bc];
         unsigned long long m = __ballot(bq < nbk && bl_live);
+        const unsigned long long mq = __ballot(quiet);
         for (int k = 0; m; m &= m - 1, k++) {
             if ((k & (kRows - 1)) != wid) continue;
-            const long long b = b0 + (long long)(__ffsll((long long)m) - 1) * G;
+            const int bi = __ffsll((long long)m) - 1;
+            const long long b = b0 + (long long)bi * G;
+            if ((mq >> bi) & 1) {
+                if (lane < kRows) {
+                    chg[b * kRows + lane] = 0;
+                    if (quiet_clr) chg0[b * kRows + lane] = 0;
+                }
+                continue;
+            }
             // this block's final level (it may have stopped jumping earlier)
             const int Fb = min(F, (int)blast[b]);
             const int2* SF = Sl + (size_t)level_slot(Fb) * N;
@@ -1540,6 +1572,7 @@ struct StepArgs {
     uint8_t *dm0, *dm1;   // change marks of the owner lists
     long long push_max;   // 0: no marks (PH_PUSH never runs)
     uint8_t *d1, *l12ok;  // reuse of level-1/2 records (with push_max; else null)
+    uint8_t* bchg;        // blocks whose records ended with the change bit (with push_max)
     GameCtl* gc;
     int *strat, *par, *dep, *key, *flag, *succ;
     int8_t *solved, *lab;
@@ -1689,7 +1722,8 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
         val_init_body(a.N, glb(a.vinfo), glb(a.off), glb(a.tgt), gid1, glb(a.gc), glb(a.strat),
                       glb(a.solved), glb(a.lab), glb(a.par), glb(a.Sl), glb(a.Zl), glb(a.jc),
                       glb(a.blive), glb(a.bact), glb(a.blast), a.trace, glb(a.hi),
-                      r ? glb(a.d1) : nullptr, glb(a.rin_off), glb(a.rin_src), glb(a.own0), glb(a.own1));
+                      r ? glb(a.d1) : nullptr, glb(a.rin_off), glb(a.rin_src), glb(a.own0), glb(a.own1),
+                      a.bchg ? glb(a.bchg) : nullptr);
         break;
     case PH_JUMP: {
         // block flags of round r: after an odd number of two-round launches
@@ -1699,19 +1733,26 @@ k_phase(int i, const StepArgs* __restrict__ ap, StepProg* __restrict__ pg)
             jump2_body(r, a.N, glb(a.Sl), glb(a.Zl), pg->pslot[i & 3], glb(a.blive),
                        glb(a.bact) + pin * a.nb, glb(a.bact) + (pin ^ 1) * a.nb, glb(a.blast),
                        a.trace, glb(a.hi), cnt, pg->pmis[i & 3], reuse_s ? glb(a.d1) : nullptr,
-                       a.l12ok && !a.trace ? glb(a.l12ok) : nullptr, reuse_s ? d1_tag(glb(a.gc)[0].br) : 0);
+                       a.l12ok && !a.trace ? glb(a.l12ok) : nullptr, reuse_s ? d1_tag(glb(a.gc)[0].br) : 0,
+                       a.bchg ? glb(a.bchg) : nullptr);
         else
             jump_body(r, a.N, glb(a.Sl), glb(a.Zl), pg->pslot[i & 3], glb(a.blive),
                       glb(a.bact) + pin * a.nb, glb(a.bact) + (pin ^ 1) * a.nb, glb(a.blast),
-                      a.trace, glb(a.hi), cnt);
+                      a.trace, glb(a.hi), cnt, a.bchg ? glb(a.bchg) : nullptr);
         break;
     }
-    case PH_LABEL:
+    case PH_LABEL: {
+        // quiet blocks: one game with change flags, plain semantics, not its
+        // first valuation, no trace
+        const GameCtl* g0 = glb(a.gc);
+        const bool quiet = a.bchg && !a.trace && a.G == 1 && g0->bf == 0 && g0->br > 0;
         label_body(a.N, a.maxn, a.lmax, glb(a.Sl), glb(a.Zl), glb(a.blast), gid1, glb(a.gc),
                    glb(a.strat), glb(a.solved), glb(a.lab), glb(a.dep), glb(a.key), glb(a.dig),
                    glb(a.tr), a.trace_cap, 1, a.trace, glb(a.jc), glb(a.blive), glb(a.hi), glb(a.chg),
-                   glb(a.chg0), 0, a.push_max ? pg->ccnt : nullptr);
+                   glb(a.chg0), 0, a.push_max ? pg->ccnt : nullptr,
+                   quiet ? glb(a.bchg) : nullptr, glb(a.bact) + ((r + dpar_s) & 1) * a.nb);
         break;
+    }
     case PH_PUSH:
         push_body(a.N, glb(a.chg), glb(a.rin_off), glb(a.rin_src), glb(a.dm0), glb(a.dm1));
         break;
@@ -1994,7 +2035,8 @@ static void build_in_edges(Core& c, hipStream_t s)
     const size_t b_d0 = al(c.own_n[0] + 64), b_d1 = al(c.own_n[1] + 64);
     const size_t b_cnt = al((N + 1) * 4), b_idx = al(N * 4), b_tmp = al(std::max<size_t>(tmp_bytes, 1));
     const size_t b_m1 = al(N), b_ok = al(nblocks(N, kBS));   // d1 marks, l12ok flags
-    if (dalloc(&c.rin_mem, (long long)(b_off + b_src + b_d0 + b_d1 + b_m1 + b_ok + b_cnt + b_idx + b_tmp)) !=
+    const size_t b_bc = al(nblocks(N, kBS));                  // bchg flags
+    if (dalloc(&c.rin_mem, (long long)(b_off + b_src + b_d0 + b_d1 + b_m1 + b_ok + b_bc + b_cnt + b_idx + b_tmp)) !=
         PSI_OK) {
         (void)hipGetLastError();
         c.rin_mem = nullptr;
@@ -2007,12 +2049,13 @@ static void build_in_edges(Core& c, hipStream_t s)
     c.dmark[1] = q;                 q += b_d1;
     c.d1 = q;                       q += b_m1;
     c.l12ok = q;                    q += b_ok;
+    c.bchg = q;                     q += b_bc;
     int* cnt = (int*)q;             q += b_cnt;
     int* idx = (int*)q;             q += b_idx;
     unsigned char* tmp = q;
     const unsigned nb = nblocks(N, kBS);
     bool ok = hipMemsetAsync(cnt, 0, (N + 1) * 4, s) == hipSuccess &&
-              hipMemsetAsync(c.dmark[0], 0, b_d0 + b_d1 + b_m1 + b_ok, s) == hipSuccess;
+              hipMemsetAsync(c.dmark[0], 0, b_d0 + b_d1 + b_m1 + b_ok + b_bc, s) == hipSuccess;
     k_rin_count<<<nb, kBS, 0, s>>>(N, c.off, c.tgt, cnt);
     ok = ok && hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, c.rin_off, (int)(N + 1), s) == hipSuccess;
     ok = ok && hipMemcpyAsync(cnt, c.rin_off, N * 4, hipMemcpyDeviceToDevice, s) == hipSuccess;
@@ -2026,7 +2069,7 @@ static void build_in_edges(Core& c, hipStream_t s)
         c.rin_mem = nullptr;
         c.rin_off = c.rin_src = nullptr;
         c.dmark[0] = c.dmark[1] = nullptr;
-        c.d1 = c.l12ok = nullptr;
+        c.d1 = c.l12ok = c.bchg = nullptr;
         return;
     }
     c.push_ok = true;
@@ -2730,6 +2773,7 @@ static StepArgs step_args(Core& c)
     a.push_max = push_max_of(c);
     a.d1 = a.push_max ? c.d1 : nullptr;
     a.l12ok = a.push_max ? c.l12ok : nullptr;
+    a.bchg = a.push_max ? c.bchg : nullptr;
     a.hi = c.flag + 2;
     a.gc = c.gctl;
     a.strat = c.strat;
