@@ -8,8 +8,10 @@ valuation rounds (gpumain.cpp:34) and t_solve is one full solve_si
 (init_strat to convergence) with the game already resident in HBM.
 
 One JSON line on stdout.  Besides the contract keys it carries
-  roofline     - the dominant kernel of the solve (pointer-jumping round),
-                 its algorithmic bytes per launch over its HIP-event time;
+  roofline     - the dominant kernel of the solve, k_phase (every launch of
+                 the step program): the solve's algorithmic bytes over the
+                 HIP-event time of its launches; the pointer-jumping round
+                 k_jump beside it (jump_round);
   cpu_baseline - the oracle's sparse CPUParity restatement (1 core) on the
                  first rounds of the same game (a bounded sample);
   listrank     - 16,777,216-node list and 1,048,576-list forest rankings;
