@@ -76,13 +76,14 @@ template <class T> int fill(T* p, long long n, T v, hipStream_t s)
     return PSI_OK;
 }
 
-// Several int fills in one launch (each launch costs a few microseconds,
-// more than filling a level of the reduced list).
-constexpr int kFillSegs = 6;
+// Several fills of int or int2 arrays in one launch (each launch costs a few
+// microseconds, more than filling a level of the reduced list).
+constexpr int kFillSegs = 16;
 struct FillSet {
-    int* p[kFillSegs];
+    void* p[kFillSegs];
     long long start[kFillSegs + 1];
-    int v[kFillSegs];
+    long long v[kFillSegs];       // int2 (x, y) as y << 32 | x for 8-byte segments
+    int wide[kFillSegs];          // 1: 8-byte elements
     int k;
 };
 
@@ -93,7 +94,9 @@ __global__ void __launch_bounds__(256) fill_set_kernel(FillSet fs)
          i += (long long)gridDim.x * blockDim.x) {
         int j = 0;
         while (i >= fs.start[j + 1]) j++;
-        fs.p[j][i - fs.start[j]] = fs.v[j];
+        const long long e = i - fs.start[j];
+        if (fs.wide[j]) ((long long*)fs.p[j])[e] = fs.v[j];
+        else ((int*)fs.p[j])[e] = (int)fs.v[j];
     }
 }
 
@@ -101,16 +104,22 @@ struct Fills {
     FillSet fs{};
     hipStream_t s;
     explicit Fills(hipStream_t st) : s(st) { fs.start[0] = 0; }
-    int add(int* p, long long n, int v)
+    int seg(void* p, long long n, long long v, int wide)
     {
         if (n <= 0) return PSI_OK;
         if (fs.k == kFillSegs)
             if (int rc = launch()) return rc;
         fs.p[fs.k] = p;
         fs.v[fs.k] = v;
+        fs.wide[fs.k] = wide;
         fs.start[fs.k + 1] = fs.start[fs.k] + n;
         fs.k++;
         return PSI_OK;
+    }
+    int add(int* p, long long n, int v) { return seg(p, n, v, 0); }
+    int add(int2* p, long long n, int x, int y)
+    {
+        return seg(p, n, (long long)(((unsigned long long)(uint32_t)y << 32) | (uint32_t)x), 1);
     }
     int launch()
     {
@@ -126,14 +135,45 @@ struct Fills {
 
 // ctl[0]: overflow sublists allocated; ctl[1]: error bits (1 bad index, 2 overflow exhausted)
 
+// Head walks alone (no splitters): one lane per head walks its whole list and
+// writes each node's rank, its hop count from the head, at once; no reduced
+// list and no final pass.  A walk still going after N hops went round a
+// cycle reachable from a head (error bit 2); a successor or head out of range
+// sets bit 1.
+__global__ void __launch_bounds__(256) walk_heads(const int* __restrict__ succ, int N,
+                                                  const int* __restrict__ heads, int H,
+                                                  int* __restrict__ rank, int* __restrict__ ctl)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= H) return;
+    int cur = heads[k];
+    if ((unsigned)cur >= (unsigned)N) {
+        atomicOr(&ctl[1], 1);
+        return;
+    }
+    for (int j = 0;;) {
+        rank[cur] = j;
+        int nxt = succ[cur];
+        if (nxt < 0 || nxt >= N) {
+            if (nxt != -1) atomicOr(&ctl[1], 1);
+            return;
+        }
+        if (++j >= N) {
+            atomicOr(&ctl[1], 2);
+            return;
+        }
+        cur = nxt;
+    }
+}
+
 // Level-0 walk with unit weights.  One lane per sublist start; writes
-// packed[x] = (sublist << LB) | position-in-sublist for every node it visits.
+// packed[x] = (sublist << LB) | position-in-sublist for every node it visits,
+// and each sublist's (next sublist, length) to rnw.
 __global__ void __launch_bounds__(256) walk0_unit(const int* __restrict__ succ, int N,
                                                   const int* __restrict__ heads, int H, int B,
                                                   int log_d, uint32_t salt, int LB,
-                                                  int* __restrict__ packed, int* __restrict__ rnext,
-                                                  int* __restrict__ rlen, int* __restrict__ ctl,
-                                                  int over_base, int over_cap)
+                                                  int* __restrict__ packed, int2* __restrict__ rnw,
+                                                  int* __restrict__ ctl, int over_base, int over_cap)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= B + H) return;
@@ -149,8 +189,7 @@ __global__ void __launch_bounds__(256) walk0_unit(const int* __restrict__ succ, 
             return;
         }
         if (B > 0 && is_split((uint32_t)cur, log_d, salt)) {
-            rnext[rid] = cur >> log_d;
-            rlen[rid] = 0;
+            rnw[rid] = make_int2(cur >> log_d, 0);
             return;
         }
     }
@@ -161,26 +200,22 @@ __global__ void __launch_bounds__(256) walk0_unit(const int* __restrict__ succ, 
         int nxt = succ[cur];
         if (nxt < 0 || nxt >= N) {
             if (nxt != -1) atomicOr(&ctl[1], 1);
-            rnext[rid] = -1;
-            rlen[rid] = local;
+            rnw[rid] = make_int2(-1, local);
             return;
         }
         if (B > 0 && is_split((uint32_t)nxt, log_d, salt)) {
-            rnext[rid] = nxt >> log_d;
-            rlen[rid] = local;
+            rnw[rid] = make_int2(nxt >> log_d, local);
             return;
         }
         if (local == hc) {
             int o = atomicAdd(&ctl[0], 1);
             if (o >= over_cap) {
                 atomicOr(&ctl[1], 2);
-                rnext[rid] = -1;
-                rlen[rid] = local;
+                rnw[rid] = make_int2(-1, local);
                 return;
             }
             int nr = over_base + o;
-            rnext[rid] = nr;
-            rlen[rid] = local;
+            rnw[rid] = make_int2(nr, local);
             rid = nr;
             local = 0;
         }
@@ -188,12 +223,12 @@ __global__ void __launch_bounds__(256) walk0_unit(const int* __restrict__ succ, 
     }
 }
 
-// Weighted walk of a reduced level.  Heads are the slots [hb, hb+H).
-__global__ void __launch_bounds__(256) walk_w(const int* __restrict__ rnext_in,
-                                              const int* __restrict__ w_in, int R, int hb, int H,
+// Weighted walk of a reduced level.  Heads are the slots [hb, hb+H).  A node's
+// (next, weight) is one 8-byte record and its (sublist, offset) in the next
+// level another, so each hop is one random load and one random store.
+__global__ void __launch_bounds__(256) walk_w(const int2* __restrict__ nw_in, int R, int hb, int H,
                                               int B, int log_d, uint32_t salt, int hc,
-                                              int* __restrict__ rid_out, int* __restrict__ loc_out,
-                                              int* __restrict__ rnext_out, int* __restrict__ w_out,
+                                              int2* __restrict__ rl_out, int2* __restrict__ nw_out,
                                               int* __restrict__ ctl, int over_base, int over_cap)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -205,39 +240,34 @@ __global__ void __launch_bounds__(256) walk_w(const int* __restrict__ rnext_in,
     } else {
         cur = hb + (t - B);
         if (B > 0 && is_split((uint32_t)cur, log_d, salt)) {
-            rnext_out[rid] = cur >> log_d;
-            w_out[rid] = 0;
+            nw_out[rid] = make_int2(cur >> log_d, 0);
             return;
         }
     }
     int acc = 0, hops = 0;
     for (;;) {
-        rid_out[cur] = rid;
-        loc_out[cur] = acc;
-        acc += w_in[cur];
+        const int2 c = nw_in[cur];
+        rl_out[cur] = make_int2(rid, acc);
+        acc += c.y;
         hops++;
-        int nxt = rnext_in[cur];
+        const int nxt = c.x;
         if (nxt < 0 || nxt >= R) {
-            rnext_out[rid] = -1;
-            w_out[rid] = acc;
+            nw_out[rid] = make_int2(-1, acc);
             return;
         }
         if (B > 0 && is_split((uint32_t)nxt, log_d, salt)) {
-            rnext_out[rid] = nxt >> log_d;
-            w_out[rid] = acc;
+            nw_out[rid] = make_int2(nxt >> log_d, acc);
             return;
         }
         if (hops == hc) {
             int o = atomicAdd(&ctl[0], 1);
             if (o >= over_cap) {
                 atomicOr(&ctl[1], 2);
-                rnext_out[rid] = -1;
-                w_out[rid] = acc;
+                nw_out[rid] = make_int2(-1, acc);
                 return;
             }
             int nr = over_base + o;
-            rnext_out[rid] = nr;
-            w_out[rid] = acc;
+            nw_out[rid] = make_int2(nr, acc);
             rid = nr;
             acc = 0;
             hops = 0;
@@ -250,9 +280,9 @@ __global__ void __launch_bounds__(256) walk_w(const int* __restrict__ rnext_in,
 // pointer jumping along predecessors inside one workgroup's LDS.
 // pred state: >= 0 predecessor still to absorb, -1 resolved, -3 unreachable.
 constexpr int kTopCap = 16384;
-__global__ void __launch_bounds__(kTopThreads) top_lds(int R, const int* __restrict__ rnext,
-                                                       const int* __restrict__ w, int hb, int H,
-                                                       int* __restrict__ rok, int* __restrict__ roff)
+__global__ void __launch_bounds__(kTopThreads) top_lds(int R, const int2* __restrict__ nw,
+                                                       int hb, int H, int* __restrict__ rok,
+                                                       int* __restrict__ roff)
 {
     __shared__ int P[kTopCap];
     __shared__ int S[kTopCap];
@@ -261,14 +291,14 @@ __global__ void __launch_bounds__(kTopThreads) top_lds(int R, const int* __restr
     for (int i = tid; i < R; i += kTopThreads) P[i] = -3;
     __syncthreads();
     for (int i = tid; i < R; i += kTopThreads) {
-        int nx = rnext[i];
+        int nx = nw[i].x;
         if (nx >= 0 && nx < R) P[nx] = i;
     }
     __syncthreads();
     for (int i = tid; i < R; i += kTopThreads) {
         int p = P[i];
         if (p >= 0) {
-            S[i] = w[p];
+            S[i] = nw[p].y;
         } else {
             bool head = i >= hb && i < hb + H;
             P[i] = head ? -1 : -3;
@@ -400,12 +430,14 @@ __global__ void __launch_bounds__(kTopThreads) wy_lds0(const int* __restrict__ s
 // Wyllie pointer jumping along predecessors.  PS[i] = (pred-state, prefix).
 // hb < 0: every node without a predecessor is a head (level-0 Wyllie).
 // ctl (level 0 only, else null): bit 2 if a node has two predecessors.
-__global__ void wy_pred(const int* __restrict__ nxt, int R, int* __restrict__ pred,
+// nxt[i * stride]: the successor of node i (stride 2 reads the x of int2
+// records).
+__global__ void wy_pred(const int* __restrict__ nxt, int stride, int R, int* __restrict__ pred,
                         int* __restrict__ ctl)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= R) return;
-    int nx = nxt[i];
+    int nx = nxt[(long long)i * stride];
     if (nx >= 0 && nx < R) {
         if (ctl) {
             if (atomicExch(&pred[nx], i) != -1) atomicOr(&ctl[1], 2);
@@ -430,6 +462,7 @@ __global__ void wy_mark_heads(const int* __restrict__ heads, int H, int N, int* 
 
 // hmark (level 0 with given heads, else null): a node without a predecessor
 // is a head only if marked.
+// w[p * 2]: the weight of node p (the y of the level's int2 records).
 template <bool UNIT>
 __global__ void wy_init(const int* __restrict__ pred, const int* __restrict__ w, int R, int hb,
                         int H, const int* __restrict__ hmark, int2* __restrict__ ps)
@@ -439,7 +472,7 @@ __global__ void wy_init(const int* __restrict__ pred, const int* __restrict__ w,
     int p = pred[i];
     int2 v;
     if (p >= 0) {
-        v = make_int2(p, UNIT ? 1 : w[p]);
+        v = make_int2(p, UNIT ? 1 : w[(long long)p * 2]);
     } else {
         bool head = hmark ? hmark[i] != 0 : (hb < 0 ? true : (i >= hb && i < hb + H));
         v = make_int2(head ? -1 : -3, 0);
@@ -501,16 +534,15 @@ __global__ void wy_finish(const int2* __restrict__ a, const int2* __restrict__ b
     roff[i] = v.x == -1 ? v.y : -1;
 }
 
-__global__ void prop_kernel(int R, const int* __restrict__ rid, const int* __restrict__ loc,
-                            const int* __restrict__ rok_up, const int* __restrict__ roff_up,
+__global__ void prop_kernel(int R, const int2* __restrict__ rl, const int* __restrict__ roff_up,
                             int* __restrict__ rok, int* __restrict__ roff)
 {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
-    int id = rid[r];
-    int o = id < 0 ? -1 : roff_up[id];
+    const int2 v = rl[r];
+    int o = v.x < 0 ? -1 : roff_up[v.x];
     rok[r] = o >= 0;
-    roff[r] = o >= 0 ? o + loc[r] : -1;
+    roff[r] = o >= 0 ? o + v.y : -1;
 }
 
 __global__ void final0(int N, int LB, int R1, int* __restrict__ rank, const int* __restrict__ rok1,
@@ -574,22 +606,32 @@ struct Arena {
     }
 };
 
+// How a reduced level is ranked.
+enum LevelKind { kTop, kWyllie, kWalk };
+
 struct Level {
     int R = 0, hb = 0, H = 0;
-    int *rnext = nullptr, *w = nullptr;           // this level's nodes
-    int *rid = nullptr, *loc = nullptr;           // filled when this level is walked
-    int *rok = nullptr, *roff = nullptr;          // final offsets of this level's nodes
+    int2* nw = nullptr;                            // this level's nodes: (next, weight)
+    int2* rl = nullptr;                            // (sublist, offset) in the next level, if walked
+    int *rok = nullptr, *roff = nullptr;           // final offsets of this level's nodes
+    LevelKind kind = kTop;
+    int B = 0, log_d = 0, hc = 0, over = 0;        // kWalk: splitters, hop cap, overflow slots
+    int* pred = nullptr;                           // kWyllie: predecessors and ping-pong pairs
+    int2 *ps0 = nullptr, *ps1 = nullptr;
+    long long maxlen = 0;                          // kWyllie: longest list of the level
 };
 
 struct Plan {
     int N, H, LB, B0, over0;
     bool blocks0;
     std::vector<Level> lv;                         // lv[0] is reduced level 1
-    std::vector<int> B, over, log_d;               // walk parameters of each reduced level
-    bool top_wyllie = false;
 };
 
-static size_t level_bytes(long long R) { return 6 * al256((size_t)std::max(R, 1LL) * 4); }
+static size_t level_bytes(long long R)
+{
+    const size_t r = (size_t)std::max(R, 1LL);
+    return 2 * al256(r * 8) + 2 * al256(r * 4);
+}
 
 Arena g_arena;           // psi_list_rank's workspace, grown with hipMalloc
 int* g_ctl = nullptr;    // its control words (2 ints per level + 64 Wyllie round counters)
@@ -605,8 +647,8 @@ static int ensure_ctl()
     return PSI_OK;
 }
 
-// Workspace rank_level takes below a level of R nodes and H heads (the
-// level's own arrays excluded), following the same choices it makes.
+// Workspace the levels below a level of R nodes and H heads take (the level's
+// own arrays excluded), following the same choices plan_levels makes.
 static size_t rank_level_bytes(long long R, long long H, int k)
 {
     const Tuning& tu = g_tune;
@@ -617,80 +659,88 @@ static size_t rank_level_bytes(long long R, long long H, int k)
     return level_bytes(R2) + rank_level_bytes(R2, H, k + 1);
 }
 
-// Ranks the reduced level lv[k] (offsets into lv[k].rok/roff), recursively.
-// Workspace from `ar`, control words from ctl0.
-static int rank_level(Plan& pl, int k, Arena& ar, int* ctl0, hipStream_t s)
+// Chooses how each reduced level from lv[k] down is ranked, takes its arrays
+// from `ar` and adds every fill they need to `f`, so that one launch clears
+// all levels before the level-0 walk.
+static int plan_levels(Plan& pl, int k, Arena& ar, int* ctl0, Fills& f)
+{
+    const Tuning& tu = g_tune;
+    for (;; k++) {
+        Level& L = pl.lv[k];
+        if (L.R <= kTopCap) {
+            L.kind = kTop;
+            return PSI_OK;
+        }
+        if ((long long)L.H * 4 >= L.R || k + 1 >= kMaxLevels) {
+            // Global Wyllie on this level.  A list of this level has at most
+            // R nodes; after head walks alone (level 1 of PSI_LR_WALK) it is one
+            // head slot and the overflow slots its walk was cut into.
+            L.kind = kWyllie;
+            L.pred = ar.take<int>(L.R);
+            L.ps0 = ar.take<int2>(L.R);
+            L.ps1 = ar.take<int2>(L.R);
+            if (ar.over) return fail(PSI_ERR_INVALID, "list ranking: workspace plan exceeded");
+            L.maxlen = L.R;
+            if (k == 0 && pl.B0 == 0) L.maxlen = std::min<long long>(L.maxlen, 1LL + pl.over0);
+            if (int rc = f.add(L.pred, L.R, -1)) return rc;
+            return f.add(ctl0 + kMaxLevels * 2, kWyRounds * kWySlots + 1, 0);
+        }
+        // Walk this level into a new reduced level.
+        L.kind = kWalk;
+        L.log_d = tu.log_dk;
+        L.B = (int)((L.R + (1LL << L.log_d) - 1) >> L.log_d);
+        L.hc = tu.hop_cap_k;
+        L.over = L.R / L.hc + 1;
+        Level nx;
+        nx.R = L.B + L.H + L.over;
+        nx.hb = L.B;
+        nx.H = L.H;
+        nx.nw = ar.take<int2>(nx.R);
+        nx.rl = ar.take<int2>(nx.R);
+        nx.rok = ar.take<int>(nx.R);
+        nx.roff = ar.take<int>(nx.R);
+        if (ar.over) return fail(PSI_ERR_INVALID, "list ranking: workspace plan exceeded");
+        if (int rc = f.add(nx.nw, nx.R, -2, 0)) return rc;   // unused slots: no successor
+        if (int rc = f.add(L.rl, L.R, -1, 0)) return rc;     // not reached by this level's walks
+        pl.lv.push_back(nx);
+    }
+}
+
+// Ranks the reduced level lv[k] and the levels below it as planned (offsets
+// into lv[k].rok/roff); control words from ctl0, zeroed before the walks.
+static int rank_level(Plan& pl, int k, int* ctl0, hipStream_t s)
 {
     Level& L = pl.lv[k];
-    const Tuning& tu = g_tune;
-    if (L.R <= kTopCap) {
-        top_lds<<<1, kTopThreads, 0, s>>>(L.R, L.rnext, L.w, L.hb, L.H, L.rok, L.roff);
+    if (L.kind == kTop) {
+        top_lds<<<1, kTopThreads, 0, s>>>(L.R, L.nw, L.hb, L.H, L.rok, L.roff);
         PSI_LAUNCH_CHECK();
         return PSI_OK;
     }
-    bool heads_dominate = (long long)L.H * 4 >= L.R || k + 1 >= kMaxLevels;
-    if (heads_dominate) {
-        // Global Wyllie on this level.
-        int* pred = ar.take<int>(L.R);
-        int2* ps0 = ar.take<int2>(L.R);
-        int2* ps1 = ar.take<int2>(L.R);
-        if (ar.over) return fail(PSI_ERR_INVALID, "list ranking: workspace plan exceeded");
+    if (L.kind == kWyllie) {
         int* active = ctl0 + kMaxLevels * 2;
-        // A list of this level has at most R nodes; after head walks alone
-        // (level 1 of PSI_LR_WALK) it is one head slot and the overflow slots
-        // its walk was cut into, so at most 1 + over0 nodes.
-        long long maxlen = L.R;
-        if (k == 0 && pl.B0 == 0) maxlen = std::min<long long>(maxlen, 1LL + pl.over0);
-        int rounds = ceil_log2(maxlen) + 1;
-        Fills f(s);
-        f.add(pred, L.R, -1);
-        f.add(active, kWyRounds * kWySlots + 1, 0);
-        if (int rc = f.launch()) return rc;
-        wy_pred<<<nblocks(L.R, 256), 256, 0, s>>>(L.rnext, L.R, pred, nullptr);
-        wy_init<false><<<nblocks(L.R, 256), 256, 0, s>>>(pred, L.w, L.R, L.hb, L.H, nullptr, ps0);
+        int rounds = ceil_log2(L.maxlen) + 1;
+        wy_pred<<<nblocks(L.R, 256), 256, 0, s>>>((const int*)L.nw, 2, L.R, L.pred, nullptr);
+        wy_init<false><<<nblocks(L.R, 256), 256, 0, s>>>(L.pred, (const int*)L.nw + 1, L.R, L.hb,
+                                                         L.H, nullptr, L.ps0);
         const unsigned wg = (unsigned)std::min<long long>(nblocks(L.R, 256), kWyGrid);
         for (int r = 0; r < rounds; r++)
-            wy_round<<<wg, 256, 0, s>>>((r & 1) ? ps1 : ps0, (r & 1) ? ps0 : ps1, L.R, active, r);
-        wy_finish<<<nblocks(L.R, 256), 256, 0, s>>>(ps0, ps1, L.R, active, L.rok, L.roff);
+            wy_round<<<wg, 256, 0, s>>>((r & 1) ? L.ps1 : L.ps0, (r & 1) ? L.ps0 : L.ps1, L.R,
+                                        active, r);
+        wy_finish<<<nblocks(L.R, 256), 256, 0, s>>>(L.ps0, L.ps1, L.R, active, L.rok, L.roff);
         PSI_LAUNCH_CHECK();
         return PSI_OK;
     }
-    // Walk this level into a new reduced level.
-    int log_d = tu.log_dk;
-    int B = (int)((L.R + (1LL << log_d) - 1) >> log_d);
-    int hc = tu.hop_cap_k;
-    int over = L.R / hc + 1;
-    Level nx;
-    nx.R = B + L.H + over;
-    nx.hb = B;
-    nx.H = L.H;
-    nx.rnext = ar.take<int>(nx.R);
-    nx.w = ar.take<int>(nx.R);
-    nx.rid = ar.take<int>(nx.R);
-    nx.loc = ar.take<int>(nx.R);
-    nx.rok = ar.take<int>(nx.R);
-    nx.roff = ar.take<int>(nx.R);
-    if (ar.over) return fail(PSI_ERR_INVALID, "list ranking: workspace plan exceeded");
     int* ctl = ctl0 + 2 * (k + 1);
-    {
-        Fills f(s);
-        f.add(ctl, 2, 0);
-        f.add(nx.rnext, nx.R, -2);
-        f.add(nx.w, nx.R, 0);
-        f.add(L.rid, L.R, -1);
-        if (int rc = f.launch()) return rc;
-    }
-    if (B + L.H > 0)
-    walk_w<<<nblocks(B + L.H, 256), 256, 0, s>>>(L.rnext, L.w, L.R, L.hb, L.H, B, log_d, tu.salt,
-                                                 hc, L.rid, L.loc, nx.rnext, nx.w, ctl, B + L.H,
-                                                 over);
+    const Level& N2 = pl.lv[k + 1];
+    if (L.B + L.H > 0)
+        walk_w<<<nblocks(L.B + L.H, 256), 256, 0, s>>>(L.nw, L.R, L.hb, L.H, L.B, L.log_d,
+                                                       g_tune.salt, L.hc, L.rl, N2.nw, ctl,
+                                                       L.B + L.H, L.over);
     PSI_LAUNCH_CHECK();
-    pl.lv.push_back(nx);
-    if (int rc = rank_level(pl, k + 1, ar, ctl0, s)) return rc;
-    Level& L2 = pl.lv[k];
-    Level& N2 = pl.lv[k + 1];
-    prop_kernel<<<nblocks(L2.R, 256), 256, 0, s>>>(L2.R, L2.rid, L2.loc, N2.rok, N2.roff, L2.rok,
-                                                   L2.roff);
+    if (int rc = rank_level(pl, k + 1, ctl0, s)) return rc;
+    const Level& L2 = pl.lv[k];
+    const Level& N3 = pl.lv[k + 1];
+    prop_kernel<<<nblocks(L2.R, 256), 256, 0, s>>>(L2.R, L2.rl, N3.roff, L2.rok, L2.roff);
     PSI_LAUNCH_CHECK();
     return PSI_OK;
 }
@@ -776,7 +826,7 @@ static int wyllie0(const int* succ, int N, const int* heads, int H, bool use_hea
         if (int rc = f.launch()) return rc;
     }
     if (use_heads && H > 0) wy_mark_heads<<<nblocks(H, 256), 256, 0, s>>>(heads, H, N, hmark, ctl0);
-    wy_pred<<<nblocks(N, 256), 256, 0, s>>>(succ, N, pred, ctl0);
+    wy_pred<<<nblocks(N, 256), 256, 0, s>>>(succ, 1, N, pred, ctl0);
     wy_init<true><<<nblocks(N, 256), 256, 0, s>>>(pred, nullptr, N, -1, 0, hmark, ps0);
     const unsigned wg = (unsigned)std::min<long long>(nblocks(N, 256), kWyGrid);
     for (int r = 0; r < rounds; r++)
@@ -786,17 +836,33 @@ static int wyllie0(const int* succ, int N, const int* heads, int H, bool use_hea
     return PSI_OK;
 }
 
-static int hj0(const int* succ, int N, const int* heads, int H, int* rank, bool blocks, bool checked,
+// Head walks alone (PSI_LR_WALK): ranks written during the walk, no workspace.
+static int walk_direct(const int* succ, int N, const int* heads, int H, int* rank, bool checked,
+                       int* ctl0, hipStream_t s)
+{
+    Fills f(s);
+    if (int rc = f.add(ctl0, kMaxLevels * 2, 0)) return rc;
+    if (checked)
+        if (int rc = f.add(rank, N, -1)) return rc;
+    if (int rc = f.launch()) return rc;
+    if (H > 0) {
+        walk_heads<<<nblocks(H, 256), 256, 0, s>>>(succ, N, heads, H, rank, ctl0);
+        PSI_LAUNCH_CHECK();
+    }
+    return PSI_OK;
+}
+
+static int hj0(const int* succ, int N, const int* heads, int H, int* rank, bool checked,
                Arena& ar, int* ctl0, bool grow, hipStream_t s)
 {
     const Tuning& tu = g_tune;
     Hj0Shape sh;
-    if (int rc = hj0_shape(N, H, blocks, &sh)) return rc;
+    if (int rc = hj0_shape(N, H, true, &sh)) return rc;
     if (int rc = arena_prepare(ar, hj0_bytes(sh, H), grow)) return rc;
     Plan pl;
     pl.N = N;
     pl.H = H;
-    pl.blocks0 = blocks;
+    pl.blocks0 = true;
     const int log_d = tu.log_d0;
     pl.B0 = sh.B0;
     pl.over0 = sh.over0;
@@ -807,37 +873,37 @@ static int hj0(const int* succ, int N, const int* heads, int H, int* rank, bool 
     L1.R = (int)R1;
     L1.hb = pl.B0;
     L1.H = H;
-    L1.rnext = ar.take<int>(R1);
-    L1.w = ar.take<int>(R1);
-    L1.rid = ar.take<int>(R1);
-    L1.loc = ar.take<int>(R1);
+    L1.nw = ar.take<int2>(R1);
+    L1.rl = ar.take<int2>(R1);
     L1.rok = ar.take<int>(R1);
     L1.roff = ar.take<int>(R1);
     if (ar.over) return fail(PSI_ERR_INVALID, "list ranking: workspace plan exceeded");
     pl.lv.push_back(L1);
-    int* ctl = ctl0;
     {
+        // one launch clears the control words, every level's arrays and (checked) rank
         Fills f(s);
-        f.add(ctl0, kMaxLevels * 2, 0);
-        f.add(L1.rnext, R1, -2);
-        f.add(L1.w, R1, 0);
-        if (checked) f.add(rank, N, -1);
+        if (int rc = f.add(ctl0, kMaxLevels * 2, 0)) return rc;
+        if (int rc = f.add(pl.lv[0].nw, R1, -2, 0)) return rc;
+        if (checked)
+            if (int rc = f.add(rank, N, -1)) return rc;
+        if (int rc = plan_levels(pl, 0, ar, ctl0, f)) return rc;
         if (int rc = f.launch()) return rc;
     }
     if (pl.B0 + H > 0) {
         walk0_unit<<<nblocks((long long)pl.B0 + H, 256), 256, 0, s>>>(
-            succ, N, heads, H, pl.B0, log_d, tu.salt, LB, rank, L1.rnext, L1.w, ctl, pl.B0 + H, pl.over0);
+            succ, N, heads, H, pl.B0, log_d, tu.salt, LB, rank, pl.lv[0].nw, ctl0, pl.B0 + H,
+            pl.over0);
         PSI_LAUNCH_CHECK();
     }
-    if (int rc = rank_level(pl, 0, ar, ctl0, s)) return rc;
+    if (int rc = rank_level(pl, 0, ctl0, s)) return rc;
+    const int* roff1 = pl.lv[0].roff;
     if (((uintptr_t)rank & 15) == 0 && N >= 1024) {
         const int N4 = N / 4;
-        final0_v4<<<nblocks(N4, 256), 256, 0, s>>>(N4, LB, (int)R1, (int4*)rank, pl.lv[0].roff);
+        final0_v4<<<nblocks(N4, 256), 256, 0, s>>>(N4, LB, (int)R1, (int4*)rank, roff1);
         if (N4 * 4 < N)
-            final0<<<1, 256, 0, s>>>(N - N4 * 4, LB, (int)R1, rank + N4 * 4, pl.lv[0].rok,
-                                     pl.lv[0].roff);
+            final0<<<1, 256, 0, s>>>(N - N4 * 4, LB, (int)R1, rank + N4 * 4, pl.lv[0].rok, roff1);
     } else {
-        final0<<<nblocks(N, 256), 256, 0, s>>>(N, LB, (int)R1, rank, pl.lv[0].rok, pl.lv[0].roff);
+        final0<<<nblocks(N, 256), 256, 0, s>>>(N, LB, (int)R1, rank, pl.lv[0].rok, roff1);
     }
     PSI_LAUNCH_CHECK();
     return PSI_OK;
@@ -864,8 +930,8 @@ static int run(const int* d_succ, int n, const int* d_heads, int n_heads, int* d
         method = mean <= g_tune.walk_mean_max ? PSI_LR_WALK : PSI_LR_HJ;
     }
     switch (method) {
-    case PSI_LR_HJ: return hj0(d_succ, n, d_heads, n_heads, d_rank, true, checked, ar, ctl0, grow, s);
-    case PSI_LR_WALK: return hj0(d_succ, n, d_heads, n_heads, d_rank, false, checked, ar, ctl0, grow, s);
+    case PSI_LR_HJ: return hj0(d_succ, n, d_heads, n_heads, d_rank, checked, ar, ctl0, grow, s);
+    case PSI_LR_WALK: return walk_direct(d_succ, n, d_heads, n_heads, d_rank, checked, ctl0, s);
     case PSI_LR_WYLLIE: return wyllie0(d_succ, n, d_heads, n_heads, false, d_rank, ar, ctl0, grow, s);
     default: return fail(PSI_ERR_INVALID, "psi_list_rank: unknown method");
     }

@@ -121,13 +121,22 @@ def test_auto_wyllie_reports_two_predecessors(gpu, n):
         gpu.list_rank_tune(gpu.LRT_WYLLIE_MAX, gpu.LRT_WYLLIE_MAX_DEFAULT)
 
 
-def test_long_list_head_walk_overflows_hop_cap(gpu):
-    # one list of 50,000 nodes ranked by head walks only: the walk is cut every
-    # 2^local_bits hops into overflow sublists that the reduced level links up
+def test_long_list_head_walk_ranks_directly(gpu):
+    # one list of 50,000 nodes ranked by head walks only: each node's rank is
+    # its hop count, written during the walk (no overflow sublists, no reduced
+    # level)
     succ, heads = perm_list(np.random.default_rng(9), 50000)
     _check(gpu, succ, heads, "WALK")
     bits, over = gpu.list_rank_status()
-    assert bits == 0 and over >= 50000 // 4096
+    assert bits == 0 and over == 0
+
+
+def test_head_walk_reports_cycle_reached_from_head(gpu):
+    # 0 -> 1 -> 2 -> 0: the head's walk never ends; it stops after n hops
+    # with the "walk did not end" error instead of ranking silently
+    succ = np.array([1, 2, 0, -1], np.int32)
+    with pytest.raises(gpu.PsiError):
+        gpu.list_rank_host(succ, np.array([0], np.int32), gpu.LR_WALK)
 
 
 @pytest.mark.parametrize("log_d0,local_bits", [(3, 8), (6, 12), (8, 10)])
