@@ -19,7 +19,7 @@ import psi
 SI_DEFAULT = {"phase_grid": 2048, "steps_per_sync": 4, "two_levels": 1}
 SI_CANDIDATES = {"phase_grid": [1024, 1536, 2048], "steps_per_sync": [2, 4, 8],
                  "two_levels": [0, 1]}
-LR_DEFAULT = {"log_d0": 4, "local_bits": 12, "log_dk": 3}
+LR_DEFAULT = {"log_d0": 5, "local_bits": 12, "log_dk": 2}
 LR_CANDIDATES = {"log_d0": [3, 4, 5, 6], "local_bits": [10, 12, 14], "log_dk": [2, 3, 4]}
 _LR_PARAM = {"log_d0": psi.LRT_LOG_D0, "local_bits": psi.LRT_LOCAL_BITS, "log_dk": psi.LRT_LOG_DK}
 

@@ -31,15 +31,15 @@ namespace lr {
 
 constexpr int kTopThreads = 1024;
 
-// Defaults measured on the 16,777,216-node list (MI355X, us per ranking):
-// level-0 / reduced-level splitter densities 2^-4 / 2^-4: 808, 2^-5 / 2^-4:
-// 812, 2^-6 / 2^-4: 819, 2^-5 / 2^-5: 854, 2^-3 / 2^-4: 882, 2^-2 / 2^-4: 1013;
-// 2^-4 / 2^-3: 778 (and one list of 2^20 nodes 137, against 281 at
-// 2^-6 / 2^-4: the reduced levels' walks are bound by their longest sublist).
+// Defaults measured on the 16,777,216-node list (MI355X, us per ranking,
+// level-0 / reduced-level splitter densities): 2^-4 / 2^-3: 753-779,
+// 2^-4 / 2^-2: 768, 2^-4 / 2^-4: 785, 2^-5 / 2^-3: 760, 2^-5 / 2^-2: 743-747,
+// 2^-5 / 2^-4: 802, 2^-3 / 2^-3: 794.  The reduced levels' walks are bound by
+// their longest sublist, the level-0 walk by its random line traffic.
 struct Tuning {
-    int log_d0 = 4;      // level-0 splitter density: one per 2^log_d0 ids
+    int log_d0 = 5;      // level-0 splitter density: one per 2^log_d0 ids
     int local_bits = 12; // level-0 hop cap 2^local_bits (also the packed local field)
-    int log_dk = 3;      // reduced-level splitter density
+    int log_dk = 2;      // reduced-level splitter density
     int hop_cap_k = 1024;
     int walk_mean_max = 64; // AUTO: mean list length at or below this -> head walks only
     // AUTO: at most this many nodes -> Wyllie (heads as given).  Measured on

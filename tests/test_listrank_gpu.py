@@ -148,7 +148,7 @@ def test_tuning_knobs_keep_results(gpu, log_d0, local_bits):
     try:
         _check(gpu, succ, heads, "HJ", expect)
     finally:
-        gpu.list_rank_tune(gpu.LRT_LOG_D0, 4)
+        gpu.list_rank_tune(gpu.LRT_LOG_D0, 5)
         gpu.list_rank_tune(gpu.LRT_LOCAL_BITS, 12)
 
 
