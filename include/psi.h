@@ -104,7 +104,7 @@ int psi_tour_path_sums(const int* d_link, int n, const uint64_t* d_w, uint64_t* 
                        int method);
 
 /* Tuning knobs (the autotuner sets these). */
-#define PSI_LRT_LOG_D0 1        /* level-0 splitter density 2^-v                 */
+#define PSI_LRT_LOG_D0 1        /* level-0 density 2^-v (2^24+ nodes; denser below) */
 #define PSI_LRT_LOCAL_BITS 2    /* level-0 hop cap 2^v                           */
 #define PSI_LRT_LOG_DK 3        /* reduced-level splitter density 2^-v           */
 #define PSI_LRT_HOP_CAP_K 4     /* reduced-level hop cap                         */

@@ -37,15 +37,15 @@ constexpr int kTopThreads = 1024;
 // 2^-5 / 2^-4: 802, 2^-3 / 2^-3: 794.  The reduced levels' walks are bound by
 // their longest sublist, the level-0 walk by its random line traffic.
 struct Tuning {
-    int log_d0 = 5;      // level-0 splitter density: one per 2^log_d0 ids
+    int log_d0 = 5;      // level-0 splitter density: one per 2^log_d0 ids (2^24+ nodes; level0_log_d)
     int local_bits = 12; // level-0 hop cap 2^local_bits (also the packed local field)
     int log_dk = 2;      // reduced-level splitter density
     int hop_cap_k = 1024;
     int walk_mean_max = 64; // AUTO: mean list length at or below this -> head walks only
     // AUTO: at most this many nodes -> Wyllie (heads as given).  Measured on
     // one list (MI355X, us per ranking, HJ with the densities above / Wyllie):
-    // 1024 nodes 26 / 11, 4096: 36 / 19, 16384: 48 / 51, 65536: 57 / 80,
-    // 2^20: 137 / 420.
+    // 1024 nodes 28 / 11, 4096: 42 / 19, 8192: 41 / 28, 16384: 45 / 50,
+    // 65536: 54 / 74, 2^20: 128 / 404.
     int wyllie_max = 8192;
     uint32_t salt = 0x2545F491u;
 };
@@ -288,23 +288,28 @@ __global__ void __launch_bounds__(kTopThreads) top_lds(int R, const int2* __rest
     __shared__ int S[kTopCap];
     constexpr int per = kTopCap / kTopThreads;
     const int tid = threadIdx.x;
-    for (int i = tid; i < R; i += kTopThreads) P[i] = -3;
-    __syncthreads();
+    int2 v[per];
+#pragma unroll
+    for (int k = 0; k < per; k++) {                // a lane's loads in flight together
+        const int i = tid + k * kTopThreads;
+        if (i < R) v[k] = nw[i];
+    }
     for (int i = tid; i < R; i += kTopThreads) {
-        int nx = nw[i].x;
-        if (nx >= 0 && nx < R) P[nx] = i;
+        P[i] = -3;
+        S[i] = 0;
     }
     __syncthreads();
-    for (int i = tid; i < R; i += kTopThreads) {
-        int p = P[i];
-        if (p >= 0) {
-            S[i] = nw[p].y;
-        } else {
-            bool head = i >= hb && i < hb + H;
-            P[i] = head ? -1 : -3;
-            S[i] = 0;
+#pragma unroll
+    for (int k = 0; k < per; k++) {                // a node's weight starts its successor's prefix
+        const int i = tid + k * kTopThreads;
+        if (i < R && v[k].x >= 0 && v[k].x < R) {
+            P[v[k].x] = i;
+            S[v[k].x] = v[k].y;
         }
     }
+    __syncthreads();
+    for (int i = tid; i < R; i += kTopThreads)
+        if (P[i] < 0) P[i] = (i >= hb && i < hb + H) ? -1 : -3;
     __syncthreads();
     int rounds = ceil_log2_dev(R) + 1;
     for (int r = 0; r < rounds; r++) {
@@ -342,6 +347,156 @@ __global__ void __launch_bounds__(kTopThreads) top_lds(int R, const int2* __rest
     }
 }
 
+// Top level by sublist walks inside one workgroup's LDS (for at most
+// kTopHjHeads heads): splitters one per aligned block of 2^kTopLogD slots and
+// the head slots start sublists; each lane walks its two sublists side by
+// side, leaving in every node its sublist (as -(s + 3)) and its offset in it;
+// the at most 2048 sublists are then ranked by pointer jumping and each
+// node's offset is its sublist's plus its own.  Work O(R) instead of top_lds'
+// O(R log R) LDS accesses.  A walk longer than R hops (a cycle a head reaches)
+// is cut; its nodes after the cut read -1.
+constexpr int kTopLogD = 4;
+constexpr int kTopHjHeads = 1024;
+constexpr int kTopSubs = (kTopCap >> kTopLogD) + kTopHjHeads;
+static_assert(kTopSubs == 2 * kTopThreads, "two sublists per lane");
+__global__ void __launch_bounds__(kTopThreads) top_hj(int R, const int2* __restrict__ nw, int hb,
+                                                      int H, uint32_t salt, int* __restrict__ rok,
+                                                      int* __restrict__ roff)
+{
+    __shared__ int NX[kTopCap];     // successor, then -(sublist + 3) once walked
+    __shared__ int WL[kTopCap];     // weight, then offset inside the sublist
+    __shared__ int P[kTopSubs];     // sublist pointer jumping: predecessor state
+    __shared__ int S[kTopSubs];     // and prefix; at the end the sublist's offset
+    const int tid = threadIdx.x;
+    const int NB = (R + (1 << kTopLogD) - 1) >> kTopLogD;
+    const int M = NB + H;
+    {
+        // all of a lane's loads in flight together, then into LDS
+        constexpr int per = kTopCap / kTopThreads;
+        int2 v[per];
+#pragma unroll
+        for (int k = 0; k < per; k++) {
+            const int i = tid + k * kTopThreads;
+            if (i < R) v[k] = nw[i];
+        }
+#pragma unroll
+        for (int k = 0; k < per; k++) {
+            const int i = tid + k * kTopThreads;
+            if (i < R) {
+                NX[i] = v[k].x;
+                WL[i] = v[k].y;
+            }
+        }
+    }
+    for (int s = tid; s < kTopSubs; s += kTopThreads) {
+        P[s] = -3;
+        S[s] = 0;
+    }
+    __syncthreads();
+    int cur[2], sn[2], acc[2], hops[2];
+    bool act[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int s = tid + k * kTopThreads;
+        act[k] = false;
+        sn[k] = -2;                                // no sublist
+        acc[k] = 0;
+        hops[k] = 0;
+        cur[k] = 0;
+        if (s < NB) {
+            cur[k] = block_splitter(s, kTopLogD, salt);
+            act[k] = cur[k] < R;
+        } else if (s < M) {
+            cur[k] = hb + (s - NB);
+            if (is_split((uint32_t)cur[k], kTopLogD, salt))
+                sn[k] = cur[k] >> kTopLogD;        // the head is a splitter: its sublist follows
+            else
+                act[k] = true;
+        }
+    }
+    while (act[0] || act[1]) {
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            if (!act[k]) continue;
+            const int s = tid + k * kTopThreads;
+            const int c = cur[k];
+            const int x = NX[c];
+            const int w = WL[c];
+            NX[c] = -(s + 3);
+            WL[c] = acc[k];
+            acc[k] += w;
+            if (x < 0 || x >= R || ++hops[k] > R) {
+                sn[k] = -1;
+                act[k] = false;
+            } else if (is_split((uint32_t)x, kTopLogD, salt)) {
+                sn[k] = x >> kTopLogD;
+                act[k] = false;
+            } else {
+                cur[k] = x;
+            }
+        }
+    }
+    __syncthreads();
+    // predecessors of the sublists; a sublist's prefix starts with the weight
+    // of the one before it
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int s = tid + k * kTopThreads;
+        if (s < M && sn[k] >= 0 && sn[k] < NB) {
+            P[sn[k]] = s;
+            S[sn[k]] = acc[k];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int s = tid + k * kTopThreads;
+        if (s >= NB && s < M && P[s] == -3) P[s] = -1;   // a head: offset 0
+    }
+    __syncthreads();
+    const int rounds = ceil_log2_dev(M) + 1;
+    for (int r = 0; r < rounds; r++) {
+        int np[2], ns[2];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int s = tid + k * kTopThreads;
+            np[k] = -3;
+            ns[k] = 0;
+            if (s < M) {
+                int p = P[s], v = S[s];
+                if (p >= 0) {
+                    v += S[p];
+                    p = P[p];
+                }
+                np[k] = p;
+                ns[k] = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int s = tid + k * kTopThreads;
+            if (s < M) {
+                P[s] = np[k];
+                S[s] = ns[k];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int s = tid + k * kTopThreads;
+        if (s < M && P[s] != -1) S[s] = -1;       // not reached from a head
+    }
+    __syncthreads();
+    for (int i = tid; i < R; i += kTopThreads) {
+        const int c = NX[i];
+        const int o = c <= -3 ? S[-c - 3] : -1;
+        rok[i] = o >= 0 ? 1 : 0;
+        roff[i] = o >= 0 ? o + WL[i] : -1;
+    }
+}
+
 // Level-0 Wyllie for a list of at most kTopCap nodes inside one workgroup's
 // LDS: predecessors and prefixes never leave LDS, ceil(log2 N) + 1 rounds,
 // one launch (the gpu mode's pointer jumping, gpu_parity.cu:79-106, on a
@@ -360,10 +515,19 @@ __global__ void __launch_bounds__(kTopThreads) wy_lds0(const int* __restrict__ s
     const int tid = threadIdx.x;
     int bits = 0;
     if (tid < ctl_zero) ctl[tid] = 0;             // this call's control words (one workgroup)
+    int nxv[per];
+#pragma unroll
+    for (int k = 0; k < per; k++) {                // a lane's loads in flight together
+        const int i = tid + k * kTopThreads;
+        nxv[k] = i < N ? succ[i] : -1;
+    }
     for (int i = tid; i < N; i += kTopThreads) P[i] = -3;
     __syncthreads();
-    for (int i = tid; i < N; i += kTopThreads) {
-        int nx = succ[i];
+#pragma unroll
+    for (int k = 0; k < per; k++) {
+        const int i = tid + k * kTopThreads;
+        const int nx = nxv[k];
+        if (i >= N) continue;
         if (nx >= 0 && nx < N) {
             if (atomicExch(&P[nx], i) != -3) bits |= 2;
         } else if (nx != -1) {
@@ -570,14 +734,16 @@ __device__ __forceinline__ int final_one(int p, int LB, int R1, const int* __res
 __global__ void __launch_bounds__(256) final0_v4(int N4, int LB, int R1, int4* __restrict__ rank4,
                                                  const int* __restrict__ roff1)
 {
+    typedef int v4i __attribute__((ext_vector_type(4)));
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N4) return;
-    int4 p = rank4[i];
-    p.x = final_one(p.x, LB, R1, roff1);
-    p.y = final_one(p.y, LB, R1, roff1);
-    p.z = final_one(p.z, LB, R1, roff1);
-    p.w = final_one(p.w, LB, R1, roff1);
-    rank4[i] = p;
+    // rank streams through once: non-temporal, so that it does not push the
+    // gathered offset table out of L2
+    v4i* q = (v4i*)rank4;
+    v4i p = __builtin_nontemporal_load(q + i);
+    v4i o = {final_one(p.x, LB, R1, roff1), final_one(p.y, LB, R1, roff1),
+             final_one(p.z, LB, R1, roff1), final_one(p.w, LB, R1, roff1)};
+    __builtin_nontemporal_store(o, q + i);
 }
 
 // ---------------------------------------------------------------------------
@@ -712,7 +878,10 @@ static int rank_level(Plan& pl, int k, int* ctl0, hipStream_t s)
 {
     Level& L = pl.lv[k];
     if (L.kind == kTop) {
-        top_lds<<<1, kTopThreads, 0, s>>>(L.R, L.nw, L.hb, L.H, L.rok, L.roff);
+        if (L.H <= kTopHjHeads)
+            top_hj<<<1, kTopThreads, 0, s>>>(L.R, L.nw, L.hb, L.H, g_tune.salt, L.rok, L.roff);
+        else
+            top_lds<<<1, kTopThreads, 0, s>>>(L.R, L.nw, L.hb, L.H, L.rok, L.roff);
         PSI_LAUNCH_CHECK();
         return PSI_OK;
     }
@@ -753,14 +922,27 @@ static size_t wyllie0_bytes(long long N, bool use_heads)
 }
 
 struct Hj0Shape {
-    int B0 = 0, LB = 0, over0 = 0;
+    int log_d = 0, B0 = 0, LB = 0, over0 = 0;
     long long R1 = 0;
 };
+
+// Level-0 splitter density for N nodes: the tuned one from 2^24 nodes on, one
+// step denser per halving below, at most two (shorter lists are bound by
+// their longest sublist rather than by the walk's line traffic; measured on
+// one list, us per ranking at level-0 densities 2^-3 / 2^-4 / 2^-5: 65,536
+// nodes 54 / 68 / 128, 2^20: 129 / 133 / 182, 2^22: 243 / 270 / 305,
+// 2^24: 792 / 741 / 727).
+static int level0_log_d(int N)
+{
+    const int ld = g_tune.log_d0 - std::max(0, std::min(2, 24 - ceil_log2(N)));
+    return std::max(1, ld);
+}
 
 static int hj0_shape(int N, int H, bool blocks, Hj0Shape* sh)
 {
     const Tuning& tu = g_tune;
-    sh->B0 = blocks ? (int)((N + (1LL << tu.log_d0) - 1) >> tu.log_d0) : 0;
+    sh->log_d = level0_log_d(N);
+    sh->B0 = blocks ? (int)((N + (1LL << sh->log_d) - 1) >> sh->log_d) : 0;
     int LB = tu.local_bits;
     for (;;) {
         sh->over0 = (int)(N >> LB) + 1;
@@ -863,7 +1045,7 @@ static int hj0(const int* succ, int N, const int* heads, int H, int* rank, bool 
     pl.N = N;
     pl.H = H;
     pl.blocks0 = true;
-    const int log_d = tu.log_d0;
+    const int log_d = sh.log_d;
     pl.B0 = sh.B0;
     pl.over0 = sh.over0;
     pl.LB = sh.LB;
