@@ -784,12 +784,10 @@ struct Level {
     int B = 0, log_d = 0, hc = 0, over = 0;        // kWalk: splitters, hop cap, overflow slots
     int* pred = nullptr;                           // kWyllie: predecessors and ping-pong pairs
     int2 *ps0 = nullptr, *ps1 = nullptr;
-    long long maxlen = 0;                          // kWyllie: longest list of the level
 };
 
 struct Plan {
     int N, H, LB, B0, over0;
-    bool blocks0;
     std::vector<Level> lv;                         // lv[0] is reduced level 1
 };
 
@@ -838,16 +836,12 @@ static int plan_levels(Plan& pl, int k, Arena& ar, int* ctl0, Fills& f)
             return PSI_OK;
         }
         if ((long long)L.H * 4 >= L.R || k + 1 >= kMaxLevels) {
-            // Global Wyllie on this level.  A list of this level has at most
-            // R nodes; after head walks alone (level 1 of PSI_LR_WALK) it is one
-            // head slot and the overflow slots its walk was cut into.
+            // Global Wyllie on this level (a list of it has at most R nodes).
             L.kind = kWyllie;
             L.pred = ar.take<int>(L.R);
             L.ps0 = ar.take<int2>(L.R);
             L.ps1 = ar.take<int2>(L.R);
             if (ar.over) return fail(PSI_ERR_INVALID, "list ranking: workspace plan exceeded");
-            L.maxlen = L.R;
-            if (k == 0 && pl.B0 == 0) L.maxlen = std::min<long long>(L.maxlen, 1LL + pl.over0);
             if (int rc = f.add(L.pred, L.R, -1)) return rc;
             return f.add(ctl0 + kMaxLevels * 2, kWyRounds * kWySlots + 1, 0);
         }
@@ -887,7 +881,7 @@ static int rank_level(Plan& pl, int k, int* ctl0, hipStream_t s)
     }
     if (L.kind == kWyllie) {
         int* active = ctl0 + kMaxLevels * 2;
-        int rounds = ceil_log2(L.maxlen) + 1;
+        int rounds = ceil_log2(L.R) + 1;
         wy_pred<<<nblocks(L.R, 256), 256, 0, s>>>((const int*)L.nw, 2, L.R, L.pred, nullptr);
         wy_init<false><<<nblocks(L.R, 256), 256, 0, s>>>(L.pred, (const int*)L.nw + 1, L.R, L.hb,
                                                          L.H, nullptr, L.ps0);
@@ -938,11 +932,11 @@ static int level0_log_d(int N)
     return std::max(1, ld);
 }
 
-static int hj0_shape(int N, int H, bool blocks, Hj0Shape* sh)
+static int hj0_shape(int N, int H, Hj0Shape* sh)
 {
     const Tuning& tu = g_tune;
     sh->log_d = level0_log_d(N);
-    sh->B0 = blocks ? (int)((N + (1LL << sh->log_d) - 1) >> sh->log_d) : 0;
+    sh->B0 = (int)((N + (1LL << sh->log_d) - 1) >> sh->log_d);
     int LB = tu.local_bits;
     for (;;) {
         sh->over0 = (int)(N >> LB) + 1;
@@ -1039,12 +1033,11 @@ static int hj0(const int* succ, int N, const int* heads, int H, int* rank, bool 
 {
     const Tuning& tu = g_tune;
     Hj0Shape sh;
-    if (int rc = hj0_shape(N, H, true, &sh)) return rc;
+    if (int rc = hj0_shape(N, H, &sh)) return rc;
     if (int rc = arena_prepare(ar, hj0_bytes(sh, H), grow)) return rc;
     Plan pl;
     pl.N = N;
     pl.H = H;
-    pl.blocks0 = true;
     const int log_d = sh.log_d;
     pl.B0 = sh.B0;
     pl.over0 = sh.over0;
