@@ -106,6 +106,28 @@ def test_auto_wyllie_with_given_heads(gpu, wy_max, n_lists, n_nodes):
         gpu.list_rank_tune(gpu.LRT_WYLLIE_MAX, gpu.LRT_WYLLIE_MAX_DEFAULT)
 
 
+@pytest.mark.parametrize("n_lists", [1024, 1025])
+def test_top_level_by_sublist_walks_and_by_jumping(gpu, n_lists):
+    """HJ on 60,000 nodes: the reduced list (about 8,600 slots) is ranked at
+    once by the top level, by LDS sublist walks with up to 1,024 heads and by
+    LDS pointer jumping beyond; about one head slot in 16 is itself a top-level
+    splitter. A third of the lists lose their head and one of those is closed
+    into a cycle, so unreached slots are part of the top level too."""
+    import psi
+    n = 60000
+    succ0, heads, _ = psi.gen_forest(n_lists * 13 + 5, n, n_lists, 4096)
+    print(f"top level lists={n_lists}", flush=True)
+    _check(gpu, succ0, heads, "HJ")
+    succ = succ0.copy()
+    keep = np.ones(n_lists, bool)
+    keep[1::3] = False
+    t = h = int(heads[1])                          # a dropped head's list becomes a cycle
+    while succ[t] != -1:
+        t = succ[t]
+    succ[t] = h
+    _check(gpu, succ, heads[keep], "HJ")
+
+
 @pytest.mark.parametrize("n", [10, 20000])
 def test_auto_wyllie_reports_two_predecessors(gpu, n):
     """A node with two predecessors (not a forest of lists) is reported by
