@@ -1258,7 +1258,10 @@ __device__ __forceinline__ void switch_body(
         // for the whole wave, so that exact compares whose segment maxima tie
         // are done by all lanes together (coop_scan), one at a time.
         int cur = live ? c0 : -1;
-        int kcur = live ? key_of(key, cur) : 0;
+        // The current successor is one of the successors, so its key arrives
+        // with the first batch's keys (no load of its own) unless it sits
+        // further on in a longer edge list.
+        int kcur = live && cur < 0 ? (1 << 30) : 0;
         // First the best candidate by key alone.  The exact order refines the
         // key order, and the scan ends at the first candidate (current
         // successor first) that is a maximum of the exact order, so its result
@@ -1270,18 +1273,32 @@ __device__ __forceinline__ void switch_body(
         {
             int e = live ? eb : 0;
             const int e1 = live ? ee : 0;
+            bool first = cur >= 0;
             while (e < e1) {
                 const int cnt = min(kSwBatch, e1 - e);
                 int us[kSwBatch], ks[kSwBatch];
 #pragma unroll
                 for (int k = 0; k < kSwBatch; k++) us[k] = k < cnt ? tgt[e + k] : -1;
-                if (only_changed) {
+                if (only_changed) {                   // the current successor stays: its key is needed
 #pragma unroll
                     for (int k = 0; k < kSwBatch; k++)
-                        if (us[k] >= 0 && !((cb[us[k] >> 6] >> (us[k] & 63)) & 1)) us[k] = -2;
+                        if (us[k] >= 0 && us[k] != cur && !((cb[us[k] >> 6] >> (us[k] & 63)) & 1))
+                            us[k] = -2;
                 }
 #pragma unroll
                 for (int k = 0; k < kSwBatch; k++) ks[k] = us[k] >= 0 ? key[us[k]] : 0;
+                if (first) {
+                    bool found = false;
+#pragma unroll
+                    for (int k = 0; k < kSwBatch; k++)
+                        if (us[k] == cur) {
+                            kcur = ks[k];
+                            found = true;
+                        }
+                    if (!found) kcur = key[cur];
+                    kbest = kcur;
+                    first = false;
+                }
 #pragma unroll
                 for (int k = 0; k < kSwBatch; k++) {
                     if (us[k] < 0) continue;
