@@ -18,6 +18,8 @@ One JSON line on stdout.  Besides the contract keys it carries
   batch        - 1,024 mixed-size games in one handle vs one at a time;
   serve        - Poisson arrivals into the serving pool: p50/p99 latency and
                  throughput;
+  serve_loaded - the same requests arriving 8x as fast (several games on the
+                 GPU at once);
   bf           - solve_bf (the cpu_bf role) at 1M vertices, and a 16K-vertex
                  sample timed on the GPU and with the oracle on one core.
 Progress goes to stderr.
@@ -557,6 +559,8 @@ def main():
                     help="list ranking: no sweep over list lengths (for profiling)")
     ap.add_argument("--requests", type=int, default=96, help="serving: requests")
     ap.add_argument("--rate", type=float, default=2.0, help="serving: arrivals per second")
+    ap.add_argument("--load-rate", type=float, default=16.0,
+                    help="serving under load: arrivals per second (0: skip)")
     ap.add_argument("--only", choices=["si", "listrank", "batch", "serve", "bf", "copy", "cpu"],
                     default=None,
                     help="run one workload only (for profiling)")
@@ -674,6 +678,9 @@ def main():
     if not a.no_serve:
         log("[bench] serving ...")
         line["serve"] = serve_bench(psi, a.requests, a.rate)
+        if a.load_rate > 0:
+            log(f"[bench] serving under load ({a.load_rate:g} requests/s) ...")
+            line["serve_loaded"] = serve_bench(psi, a.requests, a.load_rate)
     if not a.no_cpu:
         log("[bench] cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(g, 0 if a.cpu_full else a.cpu_rounds)
