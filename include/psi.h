@@ -87,7 +87,8 @@ int psi_list_rank_noalloc(const int* d_succ, int n, const int* d_heads, int n_he
                           int method);
 /* Device error bits of the last call of either entry point (1: index out of
  * range, 2: a walk did not end) and the number of overflow sublists the
- * level-0 walk created. */
+ * level-0 walk created (Helman-JaJa; head walks write ranks directly and
+ * create none). */
 int psi_list_rank_status(int* error_bits, int* overflow_used);
 
 /* Euler-tour path sums over a forest of links: the valuation engine of
