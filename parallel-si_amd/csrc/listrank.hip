@@ -920,16 +920,17 @@ struct Hj0Shape {
     long long R1 = 0;
 };
 
-// Level-0 splitter density for N nodes: the tuned one from 2^24 nodes on, one
-// step denser per halving below, at most two (shorter lists are bound by
-// their longest sublist rather than by the walk's line traffic; measured on
-// one list, us per ranking at level-0 densities 2^-3 / 2^-4 / 2^-5: 65,536
-// nodes 54 / 68 / 128, 2^20: 129 / 133 / 182, 2^22: 243 / 270 / 305,
-// 2^24: 792 / 741 / 727).
+// Level-0 splitter density for N nodes: the tuned one from 2^24 nodes on,
+// one step denser below 2^24 and two below 9 * 2^20 (shorter lists are bound by
+// their longest sublist rather than by the walk's line traffic).  Measured on
+// random lists, us per ranking at level-0 densities 2^-3 / 2^-4 / 2^-5:
+// 65,536 nodes 54 / 68 / 128, 2^20: 129 / 133 / 182, 2^22: 243 / 270 / 305,
+// 8,388,610: 406 / 451 / 451, 10 * 2^20: 508 / 483 / 507, 12 * 2^20:
+// 586 / 577 / 607, 14 * 2^20: 690 / 640 / 689, 2^24: 782-792 / 717-741 / 727-728.
 static int level0_log_d(int N)
 {
-    const int ld = g_tune.log_d0 - std::max(0, std::min(2, 24 - ceil_log2(N)));
-    return std::max(1, ld);
+    const int steps = N >= (1 << 24) ? 0 : N >= 9 * (1 << 20) ? 1 : 2;
+    return std::max(1, g_tune.log_d0 - steps);
 }
 
 static int hj0_shape(int N, int H, Hj0Shape* sh)
