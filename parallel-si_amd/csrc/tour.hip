@@ -9,7 +9,10 @@
 // Tour: vertex v has an enter node 2v and an exit node 2v+1; the root (the
 // sink) is vertex n with head 2n and tail 2n+1.  Each vertex pushes itself
 // onto its parent's child list with one atomic exchange (child order is the
-// exchange order; sums do not depend on it); enter(v) leads to its first
+// exchange order; sums do not depend on it), except the root's children,
+// which are many (every vertex whose link is -1) and would serialise on one
+// word: their list is the compacted set {v : link[v] = -1} in id order,
+// linked in one pass.  enter(v) leads to its first
 // child's enter node or to its own exit node; exit(v) leads to the next
 // sibling's enter node or to the parent's exit node.  The tour from the head
 // visits exactly the vertices whose links reach the root; a link chain that
@@ -20,6 +23,8 @@
 #include "common.h"
 
 #include <hipcub/hipcub.hpp>
+
+#include <algorithm>
 
 namespace psi {
 namespace {
@@ -39,14 +44,31 @@ __device__ __forceinline__ int tour_parent(const int* __restrict__ link, int n, 
     return l >= 0 ? l : (l == -1 ? n : -1);        // -1: a detached root
 }
 
+// Children of vertices (the root's are linked by k_tour_root).
 __global__ void k_tour_push(int n, const int* __restrict__ link, int* __restrict__ first,
                             int* __restrict__ sib)
 {
     int v = blockIdx.x * kTB + threadIdx.x;
     if (v >= n) return;
     const int p = tour_parent(link, n, v);
+    if (p == n) return;
     sib[v] = p >= 0 ? atomicExch(&first[p], v) : -1;
 }
+
+// The root's children: kids[0 .. *nk) are the vertices with link -1.
+__global__ void k_tour_root(int n, const int* __restrict__ kids, const int* __restrict__ nk,
+                            int* __restrict__ first, int* __restrict__ sib)
+{
+    const int i = blockIdx.x * kTB + threadIdx.x;
+    const int k = *nk;
+    if (i >= k) return;
+    sib[kids[i]] = i + 1 < k ? kids[i + 1] : -1;
+    if (i == 0) first[n] = kids[0];
+}
+
+struct IsRootChild {
+    __host__ __device__ bool operator()(const int& l) const { return l == -1; }
+};
 
 __global__ void k_tour_succ(int n, const int* __restrict__ link, const int* __restrict__ first,
                             const int* __restrict__ sib, int* __restrict__ succ)
@@ -112,14 +134,19 @@ extern "C" int psi_tour_path_sums(const int* d_link, int n, const uint64_t* d_w,
                                      "predecessor; use HJ, WALK or AUTO");
     hipStream_t s = stream();
     const long long L = 2LL * n + 2;
-    size_t scan_bytes = 0;
+    size_t scan_bytes = 0, sel_bytes = 0;
     PSI_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, scan_bytes, (const uint64_t*)nullptr,
                                              (uint64_t*)nullptr, (int)L, s));
+    const hipcub::CountingInputIterator<int> ids(0);
+    const hipcub::TransformInputIterator<bool, IsRootChild, const int*> is_kid(d_link, IsRootChild());
+    PSI_HIP(hipcub::DeviceSelect::Flagged(nullptr, sel_bytes, ids, is_kid, (int*)nullptr,
+                                          (int*)nullptr, n, s));
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t b_first = al((n + 1) * sizeof(int)), b_sib = al(n * sizeof(int));
     const size_t b_succ = al(L * sizeof(int)), b_rank = al(L * sizeof(int));
     const size_t b_a = al(L * sizeof(uint64_t)), b_s = al(L * sizeof(uint64_t));
-    const size_t need = b_first + b_sib + b_succ + b_rank + b_a + b_s + al(scan_bytes) + 4 * sizeof(int);
+    const size_t b_tmp = al(std::max(scan_bytes, sel_bytes));   // the select, then the scan
+    const size_t need = b_first + b_sib + b_succ + b_rank + b_a + b_s + b_tmp + 4 * sizeof(int);
     if (int rc = ensure(&g_tour.buf, &g_tour.cap, need)) return rc;
     char* p = (char*)g_tour.buf;
     int* first = (int*)p; p += b_first;
@@ -128,10 +155,14 @@ extern "C" int psi_tour_path_sums(const int* d_link, int n, const uint64_t* d_w,
     int* rank = (int*)p; p += b_rank;
     uint64_t* a = (uint64_t*)p; p += b_a;
     uint64_t* sc = (uint64_t*)p; p += b_s;
-    void* tmp = p; p += al(scan_bytes);
+    void* tmp = p; p += b_tmp;
     int* head = (int*)p;
+    int* nk = head + 1;
+    int* kids = rank;                               // free until the ranking (n <= L ints)
     k_tour_first<<<nblocks(n + 1, kTB), kTB, 0, s>>>(n, first, head);
     k_tour_push<<<nblocks(n, kTB), kTB, 0, s>>>(n, d_link, first, sib);
+    PSI_HIP(hipcub::DeviceSelect::Flagged(tmp, sel_bytes, ids, is_kid, kids, nk, n, s));
+    k_tour_root<<<nblocks(n, kTB), kTB, 0, s>>>(n, kids, nk, first, sib);
     k_tour_succ<<<nblocks(n + 1, kTB), kTB, 0, s>>>(n, d_link, first, sib, succ);
     k_tour_zero<uint64_t><<<nblocks(L, kTB), kTB, 0, s>>>(a, L);
     PSI_LAUNCH_CHECK();
