@@ -112,9 +112,18 @@ def si_bench(psi, n, steps, warmup, seed):
     # The phase of every launch in order: the timed graph solves run the same
     # launch sequence (then idle launches to the end of their last graph), so
     # this attributes per-dispatch profiler records to phases.
-    log("[bench] step-program launch phases: " +
-        " ".join(p for p, _, _ in psi.program_launches(s)))
-    return g, s, it, br, wall, dev_ms, parity, prof, pprof, prounds
+    launches = psi.program_launches(s)
+    log("[bench] step-program launch phases: " + " ".join(p for p, _, _ in launches))
+    # levels each valuation of the step program reached (its jump launches
+    # advance two levels each; the cycle-label check stops cycle-bound
+    # valuations early, which the kernel-by-kernel loop above does not)
+    levels = []
+    for ph, r, _ in launches:
+        if ph == "init":
+            levels.append(0)
+        elif ph == "jump" and levels:
+            levels[-1] = max(levels[-1], r + 1)
+    return g, s, it, br, wall, dev_ms, parity, prof, pprof, (levels, prounds)
 
 
 def listrank_bench(psi, reps=10, tune=True, sweep=True):
@@ -640,7 +649,8 @@ def main():
                            "avg_launch_us": round(jump_ms / max(jump_launches, 1) * 1e3, 2),
                            "launches": jump_launches,
                            "traffic": measured_traffic(g.n, a.seed, "k_jump")},
-            "rounds_per_valuation": prounds}
+            "rounds_per_valuation": prounds[0],
+            "rounds_per_valuation_kernel_by_kernel": prounds[1]}
     line = {
         "metric": "SI vertex-iterations/s on 4M-vertex game",
         "value": round(value, 1),
