@@ -318,26 +318,38 @@ def batch_bench(psi, n_games=1024, seed=3, lo=1000, hi=1000000, check_alone=True
                           "sample": "every 16th game of the batch"}}
 
 
-def serve_bench(psi, requests=96, rate=2.0, seed=4, budget_gib=64.0, steps=4, lookahead=64):
-    """Serving: `requests` games arrive as a seeded Poisson stream of `rate`
-    per second.  Request k: with probability 1/5 a chain game, else a random
-    game of the 4M-game generator (priority U[0, 2^20), out-degree U[2, 8]);
-    n log-uniform in [10^4, 4,194,304]; game seed = seed * 100000 + k.  Games
-    are generated ahead on host threads (outside the timing; a request whose
-    game is not ready at its arrival time waits, and the wait counts in its
-    latency).  Latency = harvest time - arrival time.  A sample of requests is
-    solved again alone afterwards and compared bit for bit."""
+STREAM_REQUESTS = 4096
+
+
+def serve_bench(psi, requests=96, rate=2.0, seed=4, budget_gib=64.0, steps=4, lookahead=64,
+                first=0, dump=None):
+    """Serving: requests [first, first + requests) of a seeded 4,096-request
+    Poisson stream of `rate` per second.  Request k: with probability 1/5 a
+    chain game, else a random game of the 4M-game generator (priority
+    U[0, 2^20), out-degree U[2, 8]); n log-uniform in [10^4, 4,194,304]; game
+    seed = seed * 100000 + k.  The whole stream's sizes, kinds and arrival
+    gaps are drawn first, so a slice is the same requests at the same
+    spacing whatever its length (its clock starts at its first arrival).
+    Games are generated ahead on host threads (outside the timing; a request
+    whose game is not ready at its arrival time waits, and the wait counts in
+    its latency).  Latency = harvest time - arrival time.  A sample of
+    requests is solved again alone afterwards and compared bit for bit.
+    dump: a path that receives every request's (k, chain, n, br, arrival,
+    latency) as JSON, so that slices run apart can be pooled."""
     from concurrent.futures import ThreadPoolExecutor
 
-    sizes = psi.gen_sizes(seed, requests, 1e4, 4194304)
+    total = max(STREAM_REQUESTS, first + requests)
+    sizes_all = psi.gen_sizes(seed, total, 1e4, 4194304)
     rng = np.random.default_rng(seed)
-    chain = rng.random(requests) < 0.2
-    arrival = np.cumsum(rng.exponential(1.0 / rate, requests))
-    arrival -= arrival[0]
+    chain_all = rng.random(total) < 0.2
+    arrival_all = np.cumsum(rng.exponential(1.0 / rate, total))
+    ks = np.arange(first, first + requests)
+    sizes, chain = sizes_all[ks], chain_all[ks]
+    arrival = arrival_all[ks] - arrival_all[first]
 
-    def make(k):
-        n = int(sizes[k])
-        if chain[k]:
+    def make(i):
+        n, k = int(sizes[i]), first + i
+        if chain[i]:
             return psi.gen_chain_game(seed * 100000 + k, n)
         return psi.gen_random_game(seed * 100000 + k, n, 1 << 20, 2, 8)
 
@@ -364,8 +376,8 @@ def serve_bench(psi, requests=96, rate=2.0, seed=4, budget_gib=64.0, steps=4, lo
     # again alone and compared
     fx = load_fixture("cfg4_stream.json")
     expect = {}
-    if fx is not None and fx["gen"]["seed"] == seed and fx["gen"]["requests"] == 4096:
-        expect = {r["k"]: r for r in fx["requests"] if r["k"] < requests}
+    if fx is not None and fx["gen"]["seed"] == seed and fx["gen"]["requests"] == STREAM_REQUESTS:
+        expect = {r["k"] - first: r for r in fx["requests"] if first <= r["k"] < first + requests}
     order = np.argsort(sizes)
     sample = set([int(k) for k in order if chain[k] and k not in expect][:2] +
                  [int(k) for k in order[::-1] if not chain[k] and k not in expect][:1])
@@ -427,9 +439,19 @@ def serve_bench(psi, requests=96, rate=2.0, seed=4, budget_gib=64.0, steps=4, lo
     vit = sum(n_of[k] * br_of[k] for k in range(requests))
     lat_c = lat[chain]
     lat_r = lat[~chain]
-    return {"requests": requests, "requests_run": requests, "requests_in_stream": 4096,
-            "reduced": requests < 4096,
-            "prefix_note": f"the first {requests} requests of the seeded 4,096-request stream",
+    if dump:
+        with open(dump, "w") as fh:
+            json.dump({"seed": seed, "rate_per_s": rate, "first": first, "requests": requests,
+                       "makespan_s": makespan, "vertex_iterations": int(vit),
+                       "served_equals_alone": bool(same), "checked_alone": len(kept),
+                       "checked_vs_oracle": vs_oracle["checked"],
+                       "equal_vs_oracle": vs_oracle["equal"],
+                       "rows": [[int(first + i), bool(chain[i]), int(n_of[i]), int(br_of[i]),
+                                 float(arrival[i]), float(lat[i])] for i in range(requests)]}, fh)
+    return {"requests": requests, "requests_run": requests, "requests_in_stream": STREAM_REQUESTS,
+            "reduced": requests < STREAM_REQUESTS,
+            "prefix_note": (f"requests {first}..{first + requests - 1} of the seeded "
+                            f"4,096-request stream"),
             "rate_per_s": rate, "seed": seed, "chain_requests": int(chain.sum()),
             "sum_n": int(sizes.sum()), "budget_gib": budget_gib, "steps_per_sync": steps,
             "makespan_s": round(makespan, 3),
@@ -448,6 +470,39 @@ def serve_bench(psi, requests=96, rate=2.0, seed=4, budget_gib=64.0, steps=4, lo
             "served_equals_oracle": (vs_oracle["equal"] == vs_oracle["checked"]
                                      if vs_oracle["checked"] else None),
             "checked_vs_oracle": vs_oracle["checked"]}
+
+
+def serve_merge(paths):
+    """Pools the per-request dumps of stream slices served in separate runs
+    (serve_bench(dump=...)): latency percentiles over every request, and the
+    sums of the slices' makespans and vertex-iterations."""
+    parts = [json.load(open(p)) for p in paths]
+    rows = [r for d in parts for r in d["rows"]]
+    ks = sorted(r[0] for r in rows)
+    lat = np.array([r[5] for r in rows])
+    ch = np.array([r[1] for r in rows])
+    span = sum(d["makespan_s"] for d in parts)
+    vit = sum(d["vertex_iterations"] for d in parts)
+
+    def pct(x, q):
+        return round(float(np.percentile(x, q)), 4) if len(x) else None
+
+    return {"requests": len(rows), "distinct": len(set(ks)), "first": ks[0], "last": ks[-1],
+            "slices": [[d["first"], d["requests"], round(d["makespan_s"], 1)] for d in parts],
+            "rate_per_s": sorted(set(d["rate_per_s"] for d in parts)),
+            "chain_requests": int(ch.sum()), "sum_n": int(sum(r[2] for r in rows)),
+            "vertex_iterations": int(vit), "br_max": int(max(r[3] for r in rows)),
+            "summed_makespan_s": round(span, 1),
+            "throughput_req_per_s": round(len(rows) / span, 3),
+            "vertex_iterations_per_s": round(vit / span, 1),
+            "latency_p50_s": pct(lat, 50), "latency_p99_s": pct(lat, 99),
+            "latency_mean_s": round(float(lat.mean()), 4),
+            "latency_p50_random_s": pct(lat[~ch], 50), "latency_p99_random_s": pct(lat[~ch], 99),
+            "latency_p50_chain_s": pct(lat[ch], 50), "latency_p99_chain_s": pct(lat[ch], 99),
+            "served_equals_alone": all(d["served_equals_alone"] for d in parts),
+            "checked_alone": sum(d["checked_alone"] for d in parts),
+            "checked_vs_oracle": sum(d["checked_vs_oracle"] for d in parts),
+            "equal_vs_oracle": sum(d["equal_vs_oracle"] for d in parts)}
 
 
 def bf_game(psi, n, seed):
@@ -568,12 +623,21 @@ def main():
                     help="list ranking: no sweep over list lengths (for profiling)")
     ap.add_argument("--requests", type=int, default=96, help="serving: requests")
     ap.add_argument("--rate", type=float, default=2.0, help="serving: arrivals per second")
+    ap.add_argument("--first", type=int, default=0,
+                    help="serving: index of the first request of the stream served")
+    ap.add_argument("--serve-dump", default=None,
+                    help="serving (--only serve): write per-request latencies to this JSON file")
     ap.add_argument("--load-rate", type=float, default=16.0,
                     help="serving under load: arrivals per second (0: skip)")
     ap.add_argument("--only", choices=["si", "listrank", "batch", "serve", "bf", "copy", "cpu"],
                     default=None,
                     help="run one workload only (for profiling)")
+    ap.add_argument("--serve-merge", nargs="+", default=None,
+                    help="pool the --serve-dump files of stream slices (no GPU work)")
     a = ap.parse_args()
+    if a.serve_merge:
+        print(json.dumps({"serve_merged": serve_merge(a.serve_merge)}), flush=True)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               f"(independent games would shard by game across GPUs, one process per GPU)",
@@ -597,7 +661,8 @@ def main():
         print(json.dumps({"batch": batch_bench(psi, a.batch_games)}), flush=True)
         return
     if a.only == "serve":
-        print(json.dumps({"serve": serve_bench(psi, a.requests, a.rate)}), flush=True)
+        print(json.dumps({"serve": serve_bench(psi, a.requests, a.rate, first=a.first,
+                                               dump=a.serve_dump)}), flush=True)
         return
     if a.only == "bf":
         print(json.dumps({"bf": bf_bench(psi)}), flush=True)
