@@ -119,7 +119,10 @@ int psi_list_rank_tune(int param, int value);
  * targets[m] (file order = the switch tie-break order), owner[n] in {0,1},
  * priority[n] in [0, 2^30), n < 2^28.  Unlike the reference, successor ids are range
  * checked and negative priorities are rejected (both are undefined behaviour
- * there).  The arrays are copied. */
+ * there).  The arrays are copied.  n = 0 (m = 0, offsets = {0}) is the game
+ * of a file with a header line only: a one-game solver of it has no device
+ * state and solves as the reference does (iter 1, br 1; solve_bf br 0).  A
+ * batch or a serving pool holds games of at least one vertex. */
 typedef struct psi_game psi_game;
 int psi_game_create(int n, int m, const int* owner, const int* priority, const int* offsets,
                     const int* targets, psi_game** out);

@@ -17,19 +17,28 @@ struct psi_game {
     int* d_priority = nullptr;
     int* d_offsets = nullptr;
     int* d_targets = nullptr;
+    bool uploaded = false;
 };
 
 struct psi_solver {
     Core core;
     bool trace = false;
     hipStream_t s = nullptr;   // the handle's own stream: handles solve concurrently
+    // A one-game handle of a game with no vertices has no device state.  The
+    // reference solves such a game at once: solve_si runs one valuation and
+    // one outer step (gpumain.cpp:18-53: iter 1, br 1, also under CPUBV and
+    // CPUList), and solve_bf's first best response counts no round
+    // (main.cpp:57-77, cpu_bf.cpp:25-47: br 0).
+    bool empty = false;
+    int e_iter = 0, e_br = 0;
 };
 
 extern "C" int psi_game_create(int n, int m, const int* owner, const int* priority,
                                const int* offsets, const int* targets, psi_game** out)
 {
     *out = nullptr;
-    if (n <= 0 || m <= 0) return fail(PSI_ERR_INVALID, "psi_game_create: empty game");
+    // n = 0 is the reference's game with no vertices (a header line only)
+    if (n < 0 || m < 0) return fail(PSI_ERR_INVALID, "psi_game_create: negative size");
     if (offsets[0] != 0 || offsets[n] != m)
         return fail(PSI_ERR_INVALID, "psi_game_create: offsets must run from 0 to m");
     auto* g = new psi_game;
@@ -57,9 +66,10 @@ extern "C" int psi_game_create(int n, int m, const int* owner, const int* priori
 extern "C" int psi_game_destroy(psi_game* g)
 {
     if (!g) return PSI_OK;
-    if (g->d_owner) {
+    if (g->uploaded) {
         std::lock_guard<std::recursive_mutex> lk(setup_mutex());
-        for (int* p : {g->d_owner, g->d_priority, g->d_offsets, g->d_targets}) (void)hipFree(p);
+        for (int* p : {g->d_owner, g->d_priority, g->d_offsets, g->d_targets})
+            if (p) (void)hipFree(p);
     }
     delete g;
     return PSI_OK;
@@ -73,12 +83,13 @@ extern "C" int psi_game_device_csr(psi_game* g, int** player, int** priority, in
                                    int** edges)
 {
     if (!g) return fail(PSI_ERR_INVALID, "psi_game_device_csr: null game");
-    if (!g->d_owner) {
+    if (!g->uploaded) {
         std::lock_guard<std::recursive_mutex> lk(setup_mutex());
         int* d[4] = {nullptr, nullptr, nullptr, nullptr};
         const std::vector<int>* h[4] = {&g->owner, &g->priority, &g->offsets, &g->targets};
         for (int i = 0; i < 4; i++) {
             size_t bytes = h[i]->size() * sizeof(int);
+            if (bytes == 0) continue;              // a game with no vertices: null arrays
             hipError_t e = hipMalloc((void**)&d[i], bytes);
             if (e == hipSuccess) e = hipMemcpy(d[i], h[i]->data(), bytes, hipMemcpyHostToDevice);
             if (e != hipSuccess) {
@@ -90,6 +101,7 @@ extern "C" int psi_game_device_csr(psi_game* g, int** player, int** priority, in
         g->d_priority = d[1];
         g->d_offsets = d[2];
         g->d_targets = d[3];
+        g->uploaded = true;
     }
     if (player) *player = g->d_owner;
     if (priority) *priority = g->d_priority;
@@ -115,8 +127,15 @@ static int create_from_views(int n_games, const GameView* views, int flags, int 
     std::lock_guard<std::recursive_mutex> lk(setup_mutex());
     auto* s = new psi_solver;
     s->trace = (flags & PSI_SOLVER_TRACE) != 0;
+    // one game with no vertices: a handle without device state (a batch
+    // holds games of at least one vertex, as core_create checks)
+    s->empty = n_games == 1 && views[0].n == 0 && views[0].m == 0;
     hipError_t e = hipStreamCreateWithFlags(&s->s, hipStreamNonBlocking);
     int rc = e == hipSuccess ? PSI_OK : hip_fail(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+    if (!rc && s->empty) {
+        *out = s;
+        return PSI_OK;
+    }
     if (!rc) rc = core_create(s->core, n_games, views, s->trace, trace_cap, s->s);
     if (!rc) rc = core_init_strat(s->core);
     if (rc) {
@@ -177,29 +196,47 @@ extern "C" int psi_solver_destroy(psi_solver* s)
 
 extern "C" int psi_solver_info(psi_solver* s, int* n_games, int64_t* n_total, int64_t* m_total)
 {
-    if (n_games) *n_games = s->core.G;
+    if (n_games) *n_games = s->empty ? 1 : s->core.G;
     if (n_total) *n_total = s->core.N;
     if (m_total) *m_total = s->core.M;
     return PSI_OK;
 }
 
-extern "C" int psi_init_strat(psi_solver* s) { return core_init_strat(s->core); }
+extern "C" int psi_init_strat(psi_solver* s) { return s->empty ? PSI_OK : core_init_strat(s->core); }
 
-extern "C" int psi_compute_valuation(psi_solver* s) { return core_valuation(s->core, false); }
+extern "C" int psi_compute_valuation(psi_solver* s)
+{
+    return s->empty ? PSI_OK : core_valuation(s->core, false);
+}
 
 extern "C" int psi_mark_solved(psi_solver* s, int player)
 {
     if (player != 0 && player != 1) return fail(PSI_ERR_INVALID, "player must be 0 or 1");
-    return core_mark_solved(s->core, player);
+    return s->empty ? PSI_OK : core_mark_solved(s->core, player);
 }
 
 extern "C" int psi_switch_strategy(psi_solver* s, int player, int* switched)
 {
     if (player != 0 && player != 1) return fail(PSI_ERR_INVALID, "player must be 0 or 1");
+    if (s->empty) {
+        *switched = 0;
+        return PSI_OK;
+    }
     return core_switch(s->core, player, switched);
 }
 
-extern "C" int psi_reset_opponent_strategy(psi_solver* s) { return core_reset_opp(s->core); }
+extern "C" int psi_reset_opponent_strategy(psi_solver* s)
+{
+    return s->empty ? PSI_OK : core_reset_opp(s->core);
+}
+
+// The solves of a game with no vertices.
+static int empty_solve(psi_solver* s, int br)
+{
+    s->e_iter = 1;
+    s->e_br = br;
+    return PSI_OK;
+}
 
 static int check_game(psi_solver* s, int game);
 
@@ -237,6 +274,7 @@ extern "C" int psi_si_tune(int param, int value)
 
 extern "C" int psi_solve_bf(psi_solver* s, int max_rounds, int steps_per_sync)
 {
+    if (s->empty) return empty_solve(s, 0);
     Core& c = s->core;
     if (int rc = core_init_strat(c)) return rc;
     if (int rc = core_set_params(c, 0, max_rounds, 1)) return rc;
@@ -258,6 +296,7 @@ extern "C" int psi_solve_bf(psi_solver* s, int max_rounds, int steps_per_sync)
 // the message and exits).
 extern "C" int psi_solve_bv(psi_solver* s, int max_rounds, int steps_per_sync)
 {
+    if (s->empty) return empty_solve(s, 1);
     Core& c = s->core;
     if (int rc = core_init_strat(c)) return rc;
     if (int rc = core_set_params(c, 0, max_rounds, 2)) return rc;
@@ -280,6 +319,7 @@ extern "C" int psi_solve_bv(psi_solver* s, int max_rounds, int steps_per_sync)
 // an odd cycle and reopen it): max_rounds caps them (PSI_ERR_CAP).
 extern "C" int psi_solve_list(psi_solver* s, int max_rounds, int steps_per_sync)
 {
+    if (s->empty) return empty_solve(s, 1);
     Core& c = s->core;
     if (int rc = core_init_strat(c)) return rc;
     if (int rc = core_set_params(c, 0, max_rounds, 3)) return rc;
@@ -296,6 +336,10 @@ extern "C" int psi_solve_list(psi_solver* s, int max_rounds, int steps_per_sync)
 extern "C" int psi_get_bf_status(psi_solver* s, int game, int* status)
 {
     if (int rc = check_game(s, game)) return rc;
+    if (s->empty) {
+        *status = 0;
+        return PSI_OK;
+    }
     Core& c = s->core;
     GameCtl gc;
     PSI_HIP(hipMemcpyAsync(&gc, c.gctl + game, sizeof gc, hipMemcpyDeviceToHost, c.s));
@@ -306,6 +350,7 @@ extern "C" int psi_get_bf_status(psi_solver* s, int game, int* status)
 
 extern "C" int psi_solve(psi_solver* s, int reset, int max_rounds, int steps_per_sync)
 {
+    if (s->empty) return empty_solve(s, 1);
     Core& c = s->core;
     if (int rc = core_init_strat(c)) return rc;
     if (int rc = core_set_params(c, reset, max_rounds)) return rc;
@@ -316,6 +361,11 @@ extern "C" int psi_solve(psi_solver* s, int reset, int max_rounds, int steps_per
 extern "C" int psi_solve_profiled(psi_solver* s, int reset, int max_rounds, double* times_ms,
                                   int64_t* counts)
 {
+    if (s->empty) {
+        std::fill(times_ms, times_ms + PSI_K_COUNT, 0.0);
+        std::fill(counts, counts + PSI_K_COUNT, (int64_t)0);
+        return empty_solve(s, 1);
+    }
     Core& c = s->core;
     if (int rc = core_init_strat(c)) return rc;
     if (int rc = core_set_params(c, reset, max_rounds)) return rc;
@@ -336,6 +386,11 @@ extern "C" int psi_get_profile_rounds(psi_solver* s, int* out, int cap, int* cou
 extern "C" int psi_solve_program_profiled(psi_solver* s, int reset, int max_rounds, double* times_ms,
                                           int64_t* counts)
 {
+    if (s->empty) {
+        std::fill(times_ms, times_ms + PSI_K_COUNT, 0.0);
+        std::fill(counts, counts + PSI_K_COUNT, (int64_t)0);
+        return empty_solve(s, 1);
+    }
     Core& c = s->core;
     if (int rc = core_init_strat(c)) return rc;
     if (int rc = core_set_params(c, reset, max_rounds)) return rc;
@@ -360,7 +415,8 @@ extern "C" int psi_get_program_launches(psi_solver* s, int* phase, float* us, in
 
 static int check_game(psi_solver* s, int game)
 {
-    if (game < 0 || game >= s->core.G) return fail(PSI_ERR_INVALID, "game index out of range");
+    if (game < 0 || game >= (s->empty ? 1 : s->core.G))
+        return fail(PSI_ERR_INVALID, "game index out of range");
     return PSI_OK;
 }
 
@@ -376,6 +432,12 @@ template <class T> static int copy_game(psi_solver* s, int game, const T* dev, T
 extern "C" int psi_get_stats(psi_solver* s, int game, int* iter, int* br, int* capped)
 {
     if (int rc = check_game(s, game)) return rc;
+    if (s->empty) {
+        if (iter) *iter = s->e_iter;
+        if (br) *br = s->e_br;
+        if (capped) *capped = 0;
+        return PSI_OK;
+    }
     Core& c = s->core;
     GameCtl gc;
     PSI_HIP(hipMemcpyAsync(&gc, c.gctl + game, sizeof gc, hipMemcpyDeviceToHost, c.s));
@@ -390,6 +452,7 @@ extern "C" int psi_get_stats(psi_solver* s, int game, int* iter, int* br, int* c
 extern "C" int psi_get_strategy(psi_solver* s, int game, int* out)
 {
     if (int rc = check_game(s, game)) return rc;
+    if (s->empty) return PSI_OK;
     if (int rc = copy_game(s, game, s->core.strat, out)) return rc;
     int vb = s->core.h_vbase[game], n = s->core.h_n[game];
     for (int v = 0; v < n; v++)
@@ -401,6 +464,7 @@ extern "C" int psi_get_strategy(psi_solver* s, int game, int* out)
 extern "C" int psi_get_labels(psi_solver* s, int game, int* out)
 {
     if (int rc = check_game(s, game)) return rc;
+    if (s->empty) return PSI_OK;
     int n = s->core.h_n[game];
     std::vector<int8_t> t(n);
     if (int rc = copy_game(s, game, s->core.lab, t.data())) return rc;
@@ -411,6 +475,7 @@ extern "C" int psi_get_labels(psi_solver* s, int game, int* out)
 extern "C" int psi_get_solved(psi_solver* s, int game, int* out)
 {
     if (int rc = check_game(s, game)) return rc;
+    if (s->empty) return PSI_OK;
     int n = s->core.h_n[game];
     std::vector<int8_t> t(n);
     if (int rc = copy_game(s, game, s->core.solved, t.data())) return rc;
@@ -422,6 +487,7 @@ extern "C" int psi_get_solved(psi_solver* s, int game, int* out)
 extern "C" int psi_get_depths(psi_solver* s, int game, int* out)
 {
     if (int rc = check_game(s, game)) return rc;
+    if (s->empty) return PSI_OK;
     if (int rc = copy_game(s, game, s->core.dep, out)) return rc;
     int n = s->core.h_n[game];
     std::vector<int8_t> t(n);
@@ -436,6 +502,7 @@ extern "C" int psi_get_digests(psi_solver* s, int game, uint64_t* out)
 {
     if (int rc = check_game(s, game)) return rc;
     if (!s->trace) return fail(PSI_ERR_INVALID, "digests need a solver created with PSI_SOLVER_TRACE");
+    if (s->empty) return PSI_OK;
     return copy_game(s, game, (const uint64_t*)s->core.dig, out);
 }
 
@@ -445,6 +512,13 @@ extern "C" int psi_get_trace(psi_solver* s, int game, uint64_t* hashes, int cap,
     if (int rc = check_game(s, game)) return rc;
     Core& c = s->core;
     if (!s->trace) return fail(PSI_ERR_INVALID, "trace needs a solver created with PSI_SOLVER_TRACE");
+    if (s->empty) {
+        // every round values empty arrays: each hash is the empty sum
+        const int k = std::max(0, std::min(s->e_br, cap));
+        std::fill(hashes, hashes + (size_t)k * 4, (uint64_t)0);
+        *count = k;
+        return PSI_OK;
+    }
     GameCtl gc;
     PSI_HIP(hipMemcpyAsync(&gc, c.gctl + game, sizeof gc, hipMemcpyDeviceToHost, c.s));
     PSI_HIP(hipStreamSynchronize(c.s));
@@ -463,6 +537,10 @@ extern "C" int psi_get_trace(psi_solver* s, int game, uint64_t* hashes, int cap,
 extern "C" int psi_get_winning(psi_solver* s, int game, int player, int* out, int* len)
 {
     if (int rc = check_game(s, game)) return rc;
+    if (s->empty) {
+        *len = 0;
+        return PSI_OK;
+    }
     int n = s->core.h_n[game];
     std::vector<int> lab(n);
     if (int rc = psi_get_labels(s, game, lab.data())) return rc;
@@ -478,6 +556,7 @@ extern "C" int psi_set_state(psi_solver* s, int game, const int* strat, const in
                              const int* labels)
 {
     if (int rc = check_game(s, game)) return rc;
+    if (s->empty) return PSI_OK;
     Core& c = s->core;
     int vb = c.h_vbase[game], n = c.h_n[game];
     std::vector<int> st(n);
@@ -501,7 +580,7 @@ extern "C" int psi_set_state(psi_solver* s, int game, const int* strat, const in
 
 extern "C" int psi_solver_synchronize(psi_solver* s)
 {
-    PSI_HIP(hipStreamSynchronize(s->core.s));
+    PSI_HIP(hipStreamSynchronize(s->s));
     return PSI_OK;
 }
 
@@ -509,7 +588,7 @@ extern "C" int psi_solver_synchronize(psi_solver* s)
 // its kernels run on.
 extern "C" int psi_solver_event_record(psi_solver* s, void* ev)
 {
-    PSI_HIP(hipEventRecord((hipEvent_t)ev, s->core.s));
+    PSI_HIP(hipEventRecord((hipEvent_t)ev, s->s));
     return PSI_OK;
 }
 
@@ -518,6 +597,12 @@ extern "C" int psi_solver_event_record(psi_solver* s, void* ev)
 // *last = last round that ran, *err = 1 if the launched rounds ran out.
 extern "C" int psi_get_jump_stats(psi_solver* s, int* out, int rounds, int* last, int* err)
 {
+    if (s->empty) {
+        std::fill(out, out + 2 * std::max(0, std::min(rounds, kMaxRounds)), 0);
+        *last = -1;
+        *err = 0;
+        return PSI_OK;
+    }
     Core& c = s->core;
     JumpCnt jc;
     PSI_HIP(hipMemcpyAsync(&jc, c.jc, sizeof jc, hipMemcpyDeviceToHost, c.s));
@@ -540,6 +625,14 @@ extern "C" int psi_get_jump_stats(psi_solver* s, int* out, int rounds, int* last
 // sink), labels, and per game (iter, br, capped).  Any pointer may be null.
 extern "C" int psi_get_results(psi_solver* s, int* strat_all, int* labels_all, int* stats_all)
 {
+    if (s->empty) {
+        if (stats_all) {
+            stats_all[0] = s->e_iter;
+            stats_all[1] = s->e_br;
+            stats_all[2] = 0;
+        }
+        return PSI_OK;
+    }
     Core& c = s->core;
     if (c.N * 4 >= (32LL << 20)) {
         // large: through pinned chunks, localised and widened on the host pool

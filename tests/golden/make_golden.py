@@ -154,6 +154,24 @@ def make_bv():
         print(name, g.n, res["final"]["status"] if res["final"] else res["message"])
 
 
+def make_empty():
+    """The game of a file with a header line only, in every mode; written to
+    edge/ so that the fixture sweeps over this folder do not take it."""
+    fx = {"name": "empty", "gm": "parity -1;\n", "modes": {}}
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "empty.gm")
+        with open(path, "w") as fh:
+            fh.write(fx["gm"])
+        for mode in ["cpu", "cpu-reset", "cpubv", "cpulist", "cpubf"]:
+            res = run_ref(mode, path, 10)
+            fx["modes"][mode] = {"returncode": res["returncode"], "final": res["final"],
+                                 "rounds": len(res["rounds"])}
+    os.makedirs(os.path.join(HERE, "edge"), exist_ok=True)
+    with open(os.path.join(HERE, "edge", "empty.json"), "w") as fh:
+        json.dump(fx, fh, separators=(",", ":"))
+    print("empty", {m: (v["final"]["iter"], v["final"]["br"]) for m, v in fx["modes"].items()})
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
@@ -162,6 +180,9 @@ def main():
         return
     if "--bv-only" in sys.argv:
         make_bv()
+        return
+    if "--empty-only" in sys.argv:
+        make_empty()
         return
     all_modes = ["cpu", "cpu-reset", "cpubv", "cpulist", "cpubf"]
     # 1. the reference's own input fixture
@@ -190,6 +211,8 @@ def main():
     make_bf()
     # 7. CPUBV (cpubv) fixtures
     make_bv()
+    # 8. the game with no vertices
+    make_empty()
 
 
 if __name__ == "__main__":
