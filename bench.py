@@ -476,7 +476,10 @@ def serve_merge(paths):
     """Pools the per-request dumps of stream slices served in separate runs
     (serve_bench(dump=...)): latency percentiles over every request, and the
     sums of the slices' makespans and vertex-iterations."""
-    parts = [json.load(open(p)) for p in paths]
+    parts = []
+    for p in paths:
+        with open(p) as fh:
+            parts.append(json.load(fh))
     rows = [r for d in parts for r in d["rows"]]
     ks = sorted(r[0] for r in rows)
     lat = np.array([r[5] for r in rows])
